@@ -1,0 +1,1040 @@
+// api.hip -- C-ABI of libane_ddgi (host side): scene build/refit, probe-grid
+// contexts, the three probe passes, state access and diagnostics.
+// The entry points and the reference calls they replace are listed in
+// include/ane_ddgi.h.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ane {
+
+static thread_local std::string g_err;
+
+int fail(int status, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return status;
+}
+
+int fail_hip(hipError_t e, const char* expr, const char* file, int line)
+{
+  (void)file;
+  int st = (e == hipErrorOutOfMemory) ? ANE_ERR_OUT_OF_MEMORY : ANE_ERR_DEVICE;
+  return fail(st, "%s failed: %s (%s:%d)", expr, hipGetErrorString(e), "libane_ddgi", line);
+}
+
+template <typename T>
+static int dalloc(T** p, int64_t count, int64_t* bytes)
+{
+  *p = nullptr;
+  size_t b = (size_t)(count > 0 ? count : 1) * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? ANE_ERR_OUT_OF_MEMORY : ANE_ERR_DEVICE,
+                "hipMalloc of %zu bytes failed: %s", b, hipGetErrorString(e));
+  }
+  if (bytes) *bytes += (int64_t)b;
+  return ANE_OK;
+}
+
+template <typename T>
+static void dfree(T*& p)
+{
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+#define ANE_TRY(expr)          \
+  do {                         \
+    int _s = (expr);           \
+    if (_s != ANE_OK) return _s; \
+  } while (0)
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Affine inverse of a 3x4 row-major transform: cofactors in binary64, rounded once.
+static void invert_3x4(const float* m, float* out)
+{
+  double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+  double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+  double det = a * A + b * B + c * C;
+  double inv[9];
+  if (det == 0.0) det = 1e-30;
+  inv[0] = A / det; inv[1] = -(b * i - c * h) / det; inv[2] = (b * f - c * e) / det;
+  inv[3] = B / det; inv[4] = (a * i - c * g) / det;  inv[5] = -(a * f - c * d) / det;
+  inv[6] = C / det; inv[7] = -(a * h - b * g) / det; inv[8] = (a * e - b * d) / det;
+  double t0 = m[3], t1 = m[7], t2 = m[11];
+  for (int r = 0; r < 3; ++r) {
+    out[4 * r + 0] = (float)inv[3 * r + 0];
+    out[4 * r + 1] = (float)inv[3 * r + 1];
+    out[4 * r + 2] = (float)inv[3 * r + 2];
+    out[4 * r + 3] = (float)(-(inv[3 * r + 0] * t0 + inv[3 * r + 1] * t1 + inv[3 * r + 2] * t2));
+  }
+}
+
+// Fibonacci lattice, rgen:35-46.  Contract: binary64 libm evaluation, binary64
+// normalization, one rounding to binary32.
+static void fibonacci_dirs(int n, float* out)
+{
+  const double golden = 1.6180339887, pi = 3.14159265359;
+  for (int i = 0; i < n; ++i) {
+    double theta = 2.0 * pi * (double)i / golden;
+    double phi = acos(1.0 - 2.0 * ((double)i + 0.5) / (double)n);
+    double x = cos(theta) * sin(phi), y = sin(theta) * sin(phi), z = cos(phi);
+    double l = sqrt(x * x + y * y + z * z);
+    out[3 * i + 0] = (float)(x / l);
+    out[3 * i + 1] = (float)(y / l);
+    out[3 * i + 2] = (float)(z / l);
+  }
+}
+
+static float to_linear_host(float c)
+{
+  return (c < 0.04045f) ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+}
+
+// Ring of hipEvent pairs per stage: recording never waits on the device; a
+// slot is harvested (elapsed time accumulated) before it is reused or when
+// statistics are read.
+struct StageTimer {
+  static constexpr int kSlots = 32;
+  hipEvent_t s[kSlots] = {};
+  hipEvent_t e[kSlots] = {};
+  bool pending[kSlots] = {};
+  int next = 0, cur = -1;
+  double acc_ms = 0.0;
+  uint64_t count = 0;
+  float last_ms = 0.0f;
+
+  int init()
+  {
+    for (int i = 0; i < kSlots; ++i) {
+      ANE_CHECK_HIP(hipEventCreate(&s[i]));
+      ANE_CHECK_HIP(hipEventCreate(&e[i]));
+    }
+    return 0;
+  }
+  void destroy()
+  {
+    for (int i = 0; i < kSlots; ++i) {
+      if (s[i]) (void)hipEventDestroy(s[i]);
+      if (e[i]) (void)hipEventDestroy(e[i]);
+      s[i] = e[i] = nullptr;
+    }
+  }
+  int harvest(int i)
+  {
+    if (!pending[i]) return 0;
+    ANE_CHECK_HIP(hipEventSynchronize(e[i]));
+    float t = 0.0f;
+    ANE_CHECK_HIP(hipEventElapsedTime(&t, s[i], e[i]));
+    acc_ms += t;
+    ++count;
+    last_ms = t;
+    pending[i] = false;
+    return 0;
+  }
+  int harvest_all()
+  {
+    for (int k = 0; k < kSlots; ++k) {
+      int i = (next + k) % kSlots;  // oldest first
+      int st = harvest(i);
+      if (st) return st;
+    }
+    return 0;
+  }
+  int begin(hipStream_t st)
+  {
+    cur = next;
+    int r = harvest(cur);
+    if (r) return r;
+    ANE_CHECK_HIP(hipEventRecord(s[cur], st));
+    return 0;
+  }
+  int end(hipStream_t st)
+  {
+    ANE_CHECK_HIP(hipEventRecord(e[cur], st));
+    pending[cur] = true;
+    next = (cur + 1) % kSlots;
+    return 0;
+  }
+  void reset()
+  {
+    acc_ms = 0.0;
+    count = 0;
+  }
+};
+
+enum { kStSetup = 0, kStTrace, kStShadow, kStShade, kStConv, kStTrans, kStages };
+
+}  // namespace ane
+
+using namespace ane;
+
+// ===========================================================================
+// scene
+// ===========================================================================
+struct ane_scene {
+  int device = 0;
+  int64_t nvert = 0, nidx = 0;
+  int32_t nmesh = 0, nmat = 0, ninst = 0, ntri = 0;
+  std::vector<ane_instance> inst;
+  std::vector<ane_mesh> meshes;
+  float* d_vertices = nullptr;
+  uint32_t* d_indices = nullptr;
+  ane_mesh* d_meshes = nullptr;
+  ane_material* d_mats = nullptr;
+  ane_instance* d_inst = nullptr;
+  float* d_inv = nullptr;
+  uint32_t* d_tri_inst = nullptr;
+  uint32_t* d_inst_off = nullptr;
+  float* d_world = nullptr;
+  uint64_t* d_keys = nullptr;
+  uint64_t* d_keys2 = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint32_t* d_ids2 = nullptr;
+  NodeBuild* d_nodes = nullptr;
+  int32_t* d_leaf_parent = nullptr;
+  float4* d_tris = nullptr;
+  uint32_t* d_arrive = nullptr;
+  float* d_node_box = nullptr;
+  uint32_t* d_cbounds = nullptr;
+  uint32_t* d_depth = nullptr;
+  uint32_t* d_scratch = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int32_t depth = 0;
+  float build_ms = 0.0f;
+  int64_t bytes = 0;
+  SceneK view{};
+};
+
+static void scene_free(ane_scene* s)
+{
+  dfree(s->d_vertices); dfree(s->d_indices); dfree(s->d_meshes); dfree(s->d_mats);
+  dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_inst_off);
+  dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
+  dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
+  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_depth); dfree(s->d_scratch);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+}
+
+static int validate_instances(const ane_scene* s, const ane_instance* inst, int32_t ninst)
+{
+  for (int32_t k = 0; k < ninst; ++k) {
+    uint32_t m = mesh_of(inst[k].custom_index_and_mask);
+    uint32_t mt = mat_of(inst[k].custom_index_and_mask);
+    if ((int32_t)m >= s->nmesh)
+      return fail(ANE_ERR_INVALID_ARG, "instance %d references mesh %u of %d", k, m, s->nmesh);
+    if ((int32_t)mt >= s->nmat)
+      return fail(ANE_ERR_INVALID_ARG, "instance %d references material %u of %d", k, mt, s->nmat);
+    for (int j = 0; j < 12; ++j)
+      if (!std::isfinite(inst[k].transform[j]))
+        return fail(ANE_ERR_INVALID_ARG, "instance %d has a non-finite transform", k);
+  }
+  return ANE_OK;
+}
+
+static int upload_instances(ane_scene* s, const ane_instance* inst, hipStream_t st)
+{
+  s->inst.assign(inst, inst + s->ninst);
+  std::vector<float> inv((size_t)12 * (s->ninst > 0 ? s->ninst : 1));
+  for (int32_t k = 0; k < s->ninst; ++k) invert_3x4(inst[k].transform, inv.data() + 12 * k);
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inst, inst, sizeof(ane_instance) * s->ninst, hipMemcpyHostToDevice, st));
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inv, inv.data(), sizeof(float) * 12 * s->ninst, hipMemcpyHostToDevice, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+// full LBVH build from the uploaded instances
+static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
+{
+  ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
+  const int n = s->ntri;
+  const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_cbounds, write_inst, st);
+  launch_morton(s->d_world, n, s->d_cbounds, s->d_keys, s->d_ids, st);
+  radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
+  launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, st);
+  launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
+  launch_fit(s->d_tris, s->d_leaf_parent, n, s->d_nodes, s->d_arrive, s->d_node_box, st);
+  launch_depth(s->d_leaf_parent, s->d_nodes, n, s->d_depth, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
+  uint32_t depth = 0;
+  ANE_CHECK_HIP(hipMemcpyAsync(&depth, s->d_depth, 4, hipMemcpyDeviceToHost, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  s->depth = (int32_t)depth;
+  ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
+  return ANE_OK;
+}
+
+static int scene_refit_dev(ane_scene* s, hipStream_t st)
+{
+  ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
+  const int n = s->ntri;
+  const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_cbounds, false, st);
+  launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
+  launch_fit(s->d_tris, s->d_leaf_parent, n, s->d_nodes, s->d_arrive, s->d_node_box, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
+  ANE_CHECK_HIP(hipEventSynchronize(s->ev1));
+  ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, const uint32_t* indices,
+                                int64_t num_indices, const ane_mesh* meshes, int32_t num_meshes,
+                                const ane_material* materials, int32_t num_materials,
+                                const ane_instance* instances, int32_t num_instances, void* stream,
+                                ane_scene** out)
+{
+  if (!out) return fail(ANE_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (num_vertices < 0 || num_indices < 0 || num_meshes < 0 || num_materials < 0 || num_instances < 0)
+    return fail(ANE_ERR_INVALID_ARG, "negative count");
+  if ((num_vertices && !vertices) || (num_indices && !indices) || (num_meshes && !meshes) ||
+      (num_materials && !materials) || (num_instances && !instances))
+    return fail(ANE_ERR_INVALID_ARG, "NULL array with a non-zero count");
+  if (num_meshes > (1 << 14)) return fail(ANE_ERR_INVALID_ARG, "at most 16384 meshes (14-bit mesh index)");
+  if (num_materials > (1 << 10)) return fail(ANE_ERR_INVALID_ARG, "at most 1024 materials (10-bit material index)");
+  for (int32_t m = 0; m < num_meshes; ++m) {
+    const ane_mesh& me = meshes[m];
+    if (me.index_count % 3 != 0)
+      return fail(ANE_ERR_INVALID_ARG, "mesh %d index_count %u is not a multiple of 3", m, me.index_count);
+    if ((int64_t)me.index_offset + me.index_count > num_indices)
+      return fail(ANE_ERR_INVALID_ARG, "mesh %d indices exceed the index array", m);
+    for (uint32_t i = 0; i < me.index_count; ++i)
+      if ((int64_t)me.vertex_offset + indices[me.index_offset + i] >= num_vertices)
+        return fail(ANE_ERR_INVALID_ARG, "mesh %d index %u out of the vertex array", m, i);
+  }
+  for (int32_t k = 0; k < num_materials; ++k)
+    for (int j = 0; j < 4; ++j)
+      if (materials[k].bindless_indices[j] != -1)
+        return fail(ANE_ERR_UNSUPPORTED, "material %d uses a texture (bindless index != -1)", k);
+  ane_scene* s = new ane_scene();
+  s->nvert = num_vertices; s->nidx = num_indices; s->nmesh = num_meshes; s->nmat = num_materials;
+  s->ninst = num_instances;
+  s->meshes.assign(meshes, meshes + num_meshes);
+  int st = validate_instances(s, instances, num_instances);
+  if (st != ANE_OK) { delete s; return st; }
+  std::vector<uint32_t> off((size_t)num_instances + 1);
+  int64_t nt = 0;
+  for (int32_t k = 0; k < num_instances; ++k) {
+    off[k] = (uint32_t)nt;
+    nt += meshes[mesh_of(instances[k].custom_index_and_mask)].index_count / 3;
+    if (nt >= (int64_t)1 << 31) { delete s; return fail(ANE_ERR_INVALID_ARG, "more than 2^31 triangles"); }
+  }
+  off[num_instances] = (uint32_t)nt;
+  s->ntri = (int32_t)nt;
+  (void)hipGetDevice(&s->device);
+  hipStream_t sm = S(stream);
+  const int64_t n = s->ntri;
+#define ANE_A(p, c) do { st = dalloc(&s->p, (c), &s->bytes); if (st) { scene_free(s); delete s; return st; } } while (0)
+  ANE_A(d_vertices, num_vertices * 24);
+  ANE_A(d_indices, num_indices);
+  ANE_A(d_meshes, num_meshes);
+  ANE_A(d_mats, num_materials);
+  ANE_A(d_inst, num_instances);
+  ANE_A(d_inv, (int64_t)num_instances * 12);
+  ANE_A(d_tri_inst, n);
+  ANE_A(d_inst_off, (int64_t)num_instances + 1);
+  ANE_A(d_world, n * 9);
+  ANE_A(d_keys, n);
+  ANE_A(d_keys2, n);
+  ANE_A(d_ids, n);
+  ANE_A(d_ids2, n);
+  ANE_A(d_nodes, n > 1 ? n - 1 : 1);
+  ANE_A(d_leaf_parent, n);
+  ANE_A(d_tris, n * 3);
+  ANE_A(d_arrive, n);
+  ANE_A(d_node_box, (n > 1 ? n - 1 : 1) * 6);
+  ANE_A(d_cbounds, 6);
+  ANE_A(d_depth, 1);
+  ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
+#undef ANE_A
+  auto bail = [&](int code) { scene_free(s); delete s; return code; };
+  hipError_t e = hipSuccess;
+  if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess)
+    return bail(fail_hip(e, "hipEventCreate", __FILE__, __LINE__));
+  if ((e = hipMemcpyAsync(s->d_vertices, vertices, sizeof(float) * 24 * num_vertices, hipMemcpyHostToDevice, sm)) ||
+      (e = hipMemcpyAsync(s->d_indices, indices, sizeof(uint32_t) * num_indices, hipMemcpyHostToDevice, sm)) ||
+      (e = hipMemcpyAsync(s->d_meshes, meshes, sizeof(ane_mesh) * num_meshes, hipMemcpyHostToDevice, sm)) ||
+      (e = hipMemcpyAsync(s->d_mats, materials, sizeof(ane_material) * num_materials, hipMemcpyHostToDevice, sm)) ||
+      (e = hipMemcpyAsync(s->d_inst_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, sm)))
+    return bail(fail_hip(e, "hipMemcpyAsync", __FILE__, __LINE__));
+  // nodes start zeroed so that unused fields are deterministic
+  if ((e = hipMemsetAsync(s->d_nodes, 0, sizeof(NodeBuild) * (n > 1 ? n - 1 : 1), sm)))
+    return bail(fail_hip(e, "hipMemsetAsync", __FILE__, __LINE__));
+  if (n == 1) {
+    int32_t m1 = -1;
+    if ((e = hipMemcpyAsync(s->d_leaf_parent, &m1, 4, hipMemcpyHostToDevice, sm)))
+      return bail(fail_hip(e, "hipMemcpyAsync", __FILE__, __LINE__));
+  }
+  st = upload_instances(s, instances, sm);
+  if (st) return bail(st);
+  SceneK& v = s->view;
+  v.vertices = s->d_vertices; v.indices = s->d_indices; v.meshes = s->d_meshes; v.mats = s->d_mats;
+  v.inst = s->d_inst; v.inv_xform = s->d_inv; v.tri_inst = s->d_tri_inst; v.inst_tri_off = s->d_inst_off;
+  v.nodes = reinterpret_cast<const float4*>(s->d_nodes); v.tris = s->d_tris;
+  v.ntri = s->ntri; v.ninst = s->ninst;
+  st = scene_build(s, sm, true);
+  if (st) return bail(st);
+  *out = s;
+  return ANE_OK;
+}
+
+static int check_same_meshes(const ane_scene* s, const ane_instance* inst, int32_t n)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "scene is NULL");
+  if (n != s->ninst) return fail(ANE_ERR_INVALID_ARG, "instance count %d != %d", n, s->ninst);
+  if (n && !inst) return fail(ANE_ERR_INVALID_ARG, "instances is NULL");
+  for (int32_t k = 0; k < n; ++k)
+    if (mesh_of(inst[k].custom_index_and_mask) != mesh_of(s->inst[k].custom_index_and_mask))
+      return fail(ANE_ERR_INVALID_ARG, "instance %d changed its mesh", k);
+  return validate_instances(s, inst, n);
+}
+
+extern "C" int ane_scene_refit(ane_scene* s, const ane_instance* inst, int32_t n, void* stream)
+{
+  ANE_TRY(check_same_meshes(s, inst, n));
+  ANE_TRY(upload_instances(s, inst, S(stream)));
+  return scene_refit_dev(s, S(stream));
+}
+
+extern "C" int ane_scene_rebuild(ane_scene* s, const ane_instance* inst, int32_t n, void* stream)
+{
+  ANE_TRY(check_same_meshes(s, inst, n));
+  ANE_TRY(upload_instances(s, inst, S(stream)));
+  return scene_build(s, S(stream), false);
+}
+
+extern "C" int ane_scene_info_get(const ane_scene* s, ane_scene_info* out)
+{
+  if (!s || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  out->num_triangles = s->ntri;
+  out->num_instances = s->ninst;
+  out->bvh_depth = s->depth;
+  out->device_bytes = s->bytes;
+  out->build_ms = s->build_ms;
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_destroy(ane_scene* s)
+{
+  if (!s) return ANE_OK;
+  (void)hipDeviceSynchronize();
+  scene_free(s);
+  delete s;
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_debug_bvh(const ane_scene* s, uint64_t* keys, uint32_t* ids, int32_t* left,
+                                   int32_t* right, int32_t* parent, float* boxes)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "scene is NULL");
+  const int64_t n = s->ntri;
+  ANE_CHECK_HIP(hipDeviceSynchronize());
+  if (keys) ANE_CHECK_HIP(hipMemcpy(keys, s->d_keys, 8 * n, hipMemcpyDeviceToHost));
+  if (ids) ANE_CHECK_HIP(hipMemcpy(ids, s->d_ids, 4 * n, hipMemcpyDeviceToHost));
+  if (n < 2) return ANE_OK;
+  std::vector<NodeBuild> nodes((size_t)(n - 1));
+  ANE_CHECK_HIP(hipMemcpy(nodes.data(), s->d_nodes, sizeof(NodeBuild) * (n - 1), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n - 1; ++i) {
+    if (left) left[i] = nodes[i].d.x;
+    if (right) right[i] = nodes[i].d.y;
+    if (parent) parent[i] = nodes[i].d.z;
+  }
+  if (boxes) ANE_CHECK_HIP(hipMemcpy(boxes, s->d_node_box, sizeof(float) * 6 * (n - 1), hipMemcpyDeviceToHost));
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_debug_world_tris(const ane_scene* s, float* tris9)
+{
+  if (!s || !tris9) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  ANE_CHECK_HIP(hipDeviceSynchronize());
+  ANE_CHECK_HIP(hipMemcpy(tris9, s->d_world, sizeof(float) * 9 * (size_t)s->ntri, hipMemcpyDeviceToHost));
+  return ANE_OK;
+}
+
+// ===========================================================================
+// probe-grid context
+// ===========================================================================
+struct ane_ddgi_ctx {
+  ane_ddgi_config cfg{};
+  GridK g{};
+  int device = 0;
+  int64_t P = 0, N = 0, texels = 0;
+  float* d_atlas[2] = {nullptr, nullptr};
+  int cur = 0;
+  float* d_base_dirs = nullptr;
+  float* d_dirs = nullptr;
+  float* d_w32 = nullptr;
+  float* d_wsum = nullptr;
+  uint16_t* d_wbf16 = nullptr;
+  uint8_t* d_wfp8 = nullptr;
+  float* d_wfp8_scale = nullptr;
+  uint32_t* d_order = nullptr;
+  ane_hit* d_hits = nullptr;
+  uint8_t* d_shadow = nullptr;
+  void* d_rad = nullptr;
+  float* d_rad_scale = nullptr;
+  float* d_stage = nullptr;  // FP32 staging for radiance get/set
+  uint32_t* d_work = nullptr;
+  unsigned long long* d_stats = nullptr;
+  uint32_t* d_overflow = nullptr;
+  int32_t* d_spill = nullptr;
+  int64_t spill_words = 0;
+  StageTimer timers[kStages];
+  int32_t last_cell[2] = {0, 0};
+  int32_t rad_begin = -1, rad_count = 0;  // layers held in the radiance buffer
+  bool rad_bake = false;
+  int32_t last_begin = 0, last_count = 0;
+  bool counting = false;
+  ane_ddgi_stats stats{};
+  int64_t bytes = 0;
+};
+
+static void ctx_free(ane_ddgi_ctx* c)
+{
+  dfree(c->d_atlas[0]); dfree(c->d_atlas[1]); dfree(c->d_base_dirs); dfree(c->d_dirs);
+  dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
+  dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
+  dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
+  for (auto& t : c->timers) t.destroy();
+}
+
+static int validate_config(const ane_ddgi_config* c)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "config is NULL");
+  if (c->probes_x <= 0 || c->probes_y <= 0 || c->probes_z <= 0)
+    return fail(ANE_ERR_INVALID_ARG, "probe grid %d x %d x %d must be positive", c->probes_x, c->probes_y, c->probes_z);
+  int64_t P = (int64_t)c->probes_x * c->probes_y * c->probes_z;
+  if (P > (1 << 22)) return fail(ANE_ERR_INVALID_ARG, "too many probes (%lld)", (long long)P);
+  if ((int64_t)10 * c->probes_z * c->probes_y > (1 << 24) || 10 * (int64_t)c->probes_x > (1 << 24))
+    return fail(ANE_ERR_INVALID_ARG, "atlas too large");
+  if (c->rays_per_probe <= 0 || c->rays_per_probe > 4096)
+    return fail(ANE_ERR_INVALID_ARG, "rays_per_probe %d outside 1..4096", c->rays_per_probe);
+  if (P * c->rays_per_probe > ((int64_t)1 << 31) - 1)
+    return fail(ANE_ERR_INVALID_ARG, "probes x rays exceeds 2^31");
+  for (int i = 0; i < 3; ++i)
+    if (!(c->probe_step[i] > 0.0f) || !std::isfinite(c->probe_step[i]))
+      return fail(ANE_ERR_INVALID_ARG, "probe_step[%d] must be positive", i);
+  if (!(c->hysteresis >= 0.0f && c->hysteresis <= 1.0f))
+    return fail(ANE_ERR_INVALID_ARG, "hysteresis must be in [0, 1]");
+  if (!(c->t_min >= 0.0f) || !(c->t_max > c->t_min))
+    return fail(ANE_ERR_INVALID_ARG, "need 0 <= t_min < t_max");
+  if (c->precision < ANE_PRECISION_FP32 || c->precision > ANE_PRECISION_FP8)
+    return fail(ANE_ERR_INVALID_ARG, "unknown precision %d", c->precision);
+  if (c->precision != ANE_PRECISION_FP32 && c->rays_per_probe % 32 != 0)
+    return fail(ANE_ERR_UNSUPPORTED, "BF16/FP8 convolution needs rays_per_probe %% 32 == 0");
+  if (c->quirks != 0) return fail(ANE_ERR_UNSUPPORTED, "quirks must be 0");
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
+{
+  if (!out) return fail(ANE_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  ANE_TRY(validate_config(cfg));
+  ane_ddgi_ctx* c = new ane_ddgi_ctx();
+  c->cfg = *cfg;
+  (void)hipGetDevice(&c->device);
+  GridK& g = c->g;
+  g.nx = cfg->probes_x; g.ny = cfg->probes_y; g.nz = cfg->probes_z;
+  for (int i = 0; i < 3; ++i) g.step[i] = cfg->probe_step[i];
+  g.rays = cfg->rays_per_probe; g.hysteresis = cfg->hysteresis;
+  g.t_min = cfg->t_min; g.t_max = cfg->t_max;
+  g.atlas_w = 10 * g.nx; g.atlas_h = 10 * g.nz * g.ny;
+  c->P = (int64_t)g.nx * g.ny * g.nz;
+  c->N = g.rays;
+  c->texels = (int64_t)g.atlas_w * g.atlas_h;
+  int st = ANE_OK;
+  const int64_t PN = c->P * c->N;
+  const int esz = cfg->precision == ANE_PRECISION_FP32 ? 4 : (cfg->precision == ANE_PRECISION_BF16 ? 2 : 1);
+#define ANE_A(p, cnt) do { st = dalloc(&c->p, (cnt), &c->bytes); if (st) { ctx_free(c); delete c; return st; } } while (0)
+  ANE_A(d_atlas[0], c->texels * 4);
+  ANE_A(d_atlas[1], c->texels * 4);
+  ANE_A(d_base_dirs, c->N * 3);
+  ANE_A(d_dirs, c->N * 3);
+  ANE_A(d_w32, 64 * c->N);
+  ANE_A(d_wsum, 64);
+  ANE_A(d_wbf16, 64 * c->N);
+  ANE_A(d_wfp8, 64 * c->N);
+  ANE_A(d_wfp8_scale, 64);
+  ANE_A(d_order, c->N);
+  ANE_A(d_hits, PN);
+  ANE_A(d_shadow, PN);
+  {
+    uint8_t* r = nullptr;
+    st = dalloc(&r, PN * 3 * esz, &c->bytes);
+    if (st) { ctx_free(c); delete c; return st; }
+    c->d_rad = r;
+  }
+  ANE_A(d_rad_scale, c->P);
+  ANE_A(d_work, 4);
+  ANE_A(d_stats, 4);
+  ANE_A(d_overflow, 1);
+#undef ANE_A
+  hipError_t e;
+  for (auto& t : c->timers)
+    if ((st = t.init()) != ANE_OK) {
+      ctx_free(c); delete c; return st;
+    }
+  // atlas cleared to (0, 0, 0, 1): IrradianceProbeConvolvePass.cpp:20-24
+  std::vector<float> clear((size_t)c->texels * 4);
+  for (int64_t i = 0; i < c->texels; ++i) {
+    clear[4 * i + 0] = 0.0f; clear[4 * i + 1] = 0.0f; clear[4 * i + 2] = 0.0f; clear[4 * i + 3] = 1.0f;
+  }
+  std::vector<float> dirs((size_t)c->N * 3);
+  fibonacci_dirs((int)c->N, dirs.data());
+  std::vector<uint32_t> order((size_t)c->N);
+  for (int64_t i = 0; i < c->N; ++i) order[i] = (uint32_t)i;
+  if ((e = hipMemcpy(c->d_atlas[0], clear.data(), clear.size() * 4, hipMemcpyHostToDevice)) ||
+      (e = hipMemcpy(c->d_atlas[1], clear.data(), clear.size() * 4, hipMemcpyHostToDevice)) ||
+      (e = hipMemcpy(c->d_base_dirs, dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice)) ||
+      (e = hipMemcpy(c->d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice)) ||
+      (e = hipMemset(c->d_overflow, 0, 4))) {
+    st = fail_hip(e, "hipMemcpy", __FILE__, __LINE__);
+    ctx_free(c); delete c; return st;
+  }
+  *out = c;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_destroy(ane_ddgi_ctx* c)
+{
+  if (!c) return ANE_OK;
+  (void)hipDeviceSynchronize();
+  ctx_free(c);
+  delete c;
+  return ANE_OK;
+}
+
+static int frame_to_k(const ane_ddgi_ctx* c, const ane_ddgi_frame* f, FrameK* k)
+{
+  if (!f) return fail(ANE_ERR_INVALID_ARG, "frame is NULL");
+  if (f->layer_count <= 0 || f->layer_begin < 0 || f->layer_begin + f->layer_count > c->g.ny)
+    return fail(ANE_ERR_INVALID_ARG, "layer range [%d, %d) outside 0..%d", f->layer_begin,
+                f->layer_begin + f->layer_count, c->g.ny);
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(f->camera_pos[i]) || !std::isfinite(f->light_dir[i]))
+      return fail(ANE_ERR_INVALID_ARG, "non-finite camera or light");
+  if ((f->flags & ANE_FLAG_BAKE) && f->bake_tile[2] <= 0)
+    return fail(ANE_ERR_INVALID_ARG, "bake mode needs a positive tile size");
+  for (int i = 0; i < 4; ++i) {
+    k->cam[i] = f->camera_pos[i];
+    k->light_dir[i] = f->light_dir[i];
+    k->bake_tile[i] = f->bake_tile[i];
+  }
+  // rmiss:20 toLinear(skyIntensity * skyColor)
+  for (int i = 0; i < 3; ++i) k->sky[i] = to_linear_host(f->sky_intensity * f->sky_color[i]);
+  k->sun_intensity = f->sun_intensity;
+  k->flags = f->flags;
+  k->layer_begin = f->layer_begin;
+  k->layer_count = f->layer_count;
+  for (int i = 0; i < 16; ++i) k->rot[i] = f->rotation[i];
+  return ANE_OK;
+}
+
+static int ensure_spill(ane_ddgi_ctx* c, int32_t depth)
+{
+  const int64_t lanes = trace_lanes_per_launch();
+  const int64_t d = depth - trace_stack_lds() + 1;
+  const int64_t need = d > 0 ? d * lanes : 0;
+  if (need <= c->spill_words) return ANE_OK;
+  dfree(c->d_spill);
+  c->spill_words = 0;
+  ANE_TRY(dalloc(&c->d_spill, need, nullptr));
+  c->spill_words = need;
+  return ANE_OK;
+}
+
+static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& fk)
+{
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g = c->g;
+  a.f = fk;
+  a.dirs = c->d_dirs;
+  a.order = c->d_order;
+  a.hits = c->d_hits;
+  a.shadow = c->d_shadow;
+  a.stats = c->d_stats;
+  a.overflow = c->d_overflow;
+  a.spill = c->d_spill;
+  const int64_t lanes = trace_lanes_per_launch();
+  a.spill_depth = (int32_t)(lanes ? c->spill_words / lanes : 0);
+  a.nrays = (int32_t)((int64_t)fk.layer_count * c->g.nx * c->g.nz * c->N);
+  a.probe_base = fk.layer_begin * c->g.nx * c->g.nz;
+  f3 sun = normalize(mk(-fk.light_dir[0], -fk.light_dir[1], -fk.light_dir[2]));
+  a.sun[0] = sun.x; a.sun[1] = sun.y; a.sun[2] = sun.z;
+  return a;
+}
+
+static int do_frame_setup(ane_ddgi_ctx* c, const FrameK& fk, bool bake, hipStream_t st)
+{
+  launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_dirs, c->d_w32, c->d_wsum, c->d_wbf16,
+                     c->d_wfp8, c->d_wfp8_scale, bake, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, void* stream)
+{
+  if (!c || !s) return fail(ANE_ERR_INVALID_ARG, "NULL context or scene");
+  FrameK fk;
+  ANE_TRY(frame_to_k(c, f, &fk));
+  ANE_TRY(ensure_spill(c, s->depth));
+  hipStream_t st = S(stream);
+  const bool bake = (fk.flags & ANE_FLAG_BAKE) != 0;
+  const int nprobes = fk.layer_count * c->g.nx * c->g.nz;
+  ANE_TRY(c->timers[kStSetup].begin(st));
+  ANE_TRY(do_frame_setup(c, fk, bake, st));
+  fill_u32(c->d_work, 0u, 4, st);
+  if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->d_stats), 0u, 8, st);
+  ANE_TRY(c->timers[kStSetup].end(st));
+  TraceArgs a = trace_args(c, s, fk);
+  a.work = c->d_work;
+  ANE_TRY(c->timers[kStTrace].begin(st));
+  launch_trace(a, false, false, c->counting, st);
+  ANE_TRY(c->timers[kStTrace].end(st));
+  TraceArgs b = a;
+  b.work = c->d_work + 1;
+  b.stats = c->d_stats + 2;
+  ANE_TRY(c->timers[kStShadow].begin(st));
+  launch_trace(b, true, false, c->counting, st);
+  ANE_TRY(c->timers[kStShadow].end(st));
+  ANE_TRY(c->timers[kStShade].begin(st));
+  launch_shade(s->view, c->g, fk, c->d_hits, c->d_shadow, c->d_atlas[c->cur], c->cfg.precision,
+               c->d_rad, c->d_rad_scale, nprobes, a.probe_base, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_TRY(c->timers[kStShade].end(st));
+  c->rad_begin = fk.layer_begin;
+  c->rad_count = fk.layer_count;
+  c->rad_bake = bake;
+  c->last_begin = fk.layer_begin;
+  c->last_count = fk.layer_count;
+  c->stats.rays = (uint64_t)nprobes * c->N;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_convolve(ane_ddgi_ctx* c, const ane_ddgi_frame* f, void* stream)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  FrameK fk;
+  ANE_TRY(frame_to_k(c, f, &fk));
+  if (c->rad_begin != fk.layer_begin || c->rad_count != fk.layer_count)
+    return fail(ANE_ERR_NOT_READY, "no radiance for layers [%d, %d): trace or set them first",
+                fk.layer_begin, fk.layer_begin + fk.layer_count);
+  const bool bake = (fk.flags & ANE_FLAG_BAKE) != 0;
+  if (bake != c->rad_bake)
+    return fail(ANE_ERR_INVALID_ARG, "bake flag differs from the one the radiance was produced with");
+  hipStream_t st = S(stream);
+  const int nprobes = fk.layer_count * c->g.nx * c->g.nz;
+  const int probe_base = fk.layer_begin * c->g.nx * c->g.nz;
+  ANE_TRY(c->timers[kStConv].begin(st));
+  launch_convolve(c->g, fk, c->cfg.precision, c->d_rad, c->d_rad_scale, c->d_w32, c->d_wsum,
+                  c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, c->d_atlas[c->cur], nprobes, probe_base,
+                  bake, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_TRY(c->timers[kStConv].end(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_update(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, void* stream)
+{
+  ANE_TRY(ane_ddgi_trace(c, s, f, stream));
+  return ane_ddgi_convolve(c, f, stream);
+}
+
+extern "C" int ane_ddgi_translate(ane_ddgi_ctx* c, const float* cam, int32_t* shifted, void* stream)
+{
+  if (!c || !cam) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (shifted) *shifted = 0;
+  if (!std::isfinite(cam[0]) || !std::isfinite(cam[2])) return fail(ANE_ERR_INVALID_ARG, "non-finite camera");
+  // IrradianceProbeTranslationPass.cpp:206-222
+  const int ix = (int)(cam[0] / c->g.step[0]);
+  const int iz = (int)(cam[2] / c->g.step[2]);
+  const int dx = ix - c->last_cell[0], dz = iz - c->last_cell[1];
+  if (dx == 0 && dz == 0) return ANE_OK;
+  c->last_cell[0] = ix;
+  c->last_cell[1] = iz;
+  hipStream_t st = S(stream);
+  ANE_TRY(c->timers[kStTrans].begin(st));
+  launch_translate(c->g, reinterpret_cast<const float4*>(c->d_atlas[c->cur]),
+                   reinterpret_cast<float4*>(c->d_atlas[1 - c->cur]), dx, dz, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_TRY(c->timers[kStTrans].end(st));
+  c->cur = 1 - c->cur;
+  if (shifted) *shifted = 1;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_translation_state(ane_ddgi_ctx* c, int32_t* cell, int32_t set)
+{
+  if (!c || !cell) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (set) { c->last_cell[0] = cell[0]; c->last_cell[1] = cell[1]; }
+  else { cell[0] = c->last_cell[0]; cell[1] = c->last_cell[1]; }
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_atlas_get(ane_ddgi_ctx* c, float* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  ANE_CHECK_HIP(hipMemcpyAsync(host, c->d_atlas[c->cur], sizeof(float) * 4 * c->texels, hipMemcpyDeviceToHost, S(stream)));
+  ANE_CHECK_HIP(hipStreamSynchronize(S(stream)));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_atlas_set(ane_ddgi_ctx* c, const float* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  ANE_CHECK_HIP(hipMemcpyAsync(c->d_atlas[c->cur], host, sizeof(float) * 4 * c->texels, hipMemcpyHostToDevice, S(stream)));
+  ANE_CHECK_HIP(hipStreamSynchronize(S(stream)));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_atlas_device(ane_ddgi_ctx* c, float** dev, int64_t* w, int64_t* h)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  if (dev) *dev = c->d_atlas[c->cur];
+  if (w) *w = c->g.atlas_w;
+  if (h) *h = c->g.atlas_h;
+  return ANE_OK;
+}
+
+static int ensure_stage(ane_ddgi_ctx* c)
+{
+  if (c->d_stage) return ANE_OK;
+  return dalloc(&c->d_stage, c->P * 3 * c->N, &c->bytes);
+}
+
+extern "C" int ane_ddgi_radiance_set(ane_ddgi_ctx* c, const ane_ddgi_frame* f, const float* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  FrameK fk;
+  ANE_TRY(frame_to_k(c, f, &fk));
+  hipStream_t st = S(stream);
+  const int nprobes = fk.layer_count * c->g.nx * c->g.nz;
+  const int64_t cnt = (int64_t)nprobes * 3 * c->N;
+  const bool bake = (fk.flags & ANE_FLAG_BAKE) != 0;
+  ANE_TRY(do_frame_setup(c, fk, bake, st));
+  if (c->cfg.precision == ANE_PRECISION_FP32) {
+    ANE_CHECK_HIP(hipMemcpyAsync(c->d_rad, host, sizeof(float) * cnt, hipMemcpyHostToDevice, st));
+  } else {
+    ANE_TRY(ensure_stage(c));
+    ANE_CHECK_HIP(hipMemcpyAsync(c->d_stage, host, sizeof(float) * cnt, hipMemcpyHostToDevice, st));
+    launch_quantize(c->cfg.precision, c->d_stage, c->d_rad, c->d_rad_scale, nprobes, (int)c->N, st);
+    ANE_CHECK_HIP(hipGetLastError());
+  }
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  c->rad_begin = fk.layer_begin;
+  c->rad_count = fk.layer_count;
+  c->rad_bake = bake;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_radiance_get(ane_ddgi_ctx* c, float* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (c->rad_begin < 0) return fail(ANE_ERR_NOT_READY, "no radiance yet");
+  hipStream_t st = S(stream);
+  const int nprobes = c->rad_count * c->g.nx * c->g.nz;
+  const int64_t cnt = (int64_t)nprobes * 3 * c->N;
+  if (c->cfg.precision == ANE_PRECISION_FP32) {
+    ANE_CHECK_HIP(hipMemcpyAsync(host, c->d_rad, sizeof(float) * cnt, hipMemcpyDeviceToHost, st));
+  } else {
+    ANE_TRY(ensure_stage(c));
+    launch_dequantize(c->cfg.precision, c->d_rad, c->d_rad_scale, c->d_stage, nprobes, (int)c->N, st);
+    ANE_CHECK_HIP(hipGetLastError());
+    ANE_CHECK_HIP(hipMemcpyAsync(host, c->d_stage, sizeof(float) * cnt, hipMemcpyDeviceToHost, st));
+  }
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_debug_rays(ane_ddgi_ctx* c, ane_hit* hits, uint8_t* shadowed, void* stream)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  const int64_t cnt = (int64_t)c->last_count * c->g.nx * c->g.nz * c->N;
+  hipStream_t st = S(stream);
+  if (hits) ANE_CHECK_HIP(hipMemcpyAsync(hits, c->d_hits, sizeof(ane_hit) * cnt, hipMemcpyDeviceToHost, st));
+  if (shadowed) ANE_CHECK_HIP(hipMemcpyAsync(shadowed, c->d_shadow, cnt, hipMemcpyDeviceToHost, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_set_counting(ane_ddgi_ctx* c, int32_t enable)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  c->counting = enable != 0;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_stats_get(const ane_ddgi_ctx* cc, ane_ddgi_stats* out)
+{
+  if (!cc || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  ane_ddgi_ctx* c = const_cast<ane_ddgi_ctx*>(cc);
+  ane_ddgi_stats s = c->stats;
+  s.device_bytes = c->bytes + c->spill_words * 4;
+  for (int k = 0; k < kStages; ++k) {
+    StageTimer& t = c->timers[k];
+    ANE_TRY(t.harvest_all());
+    s.last_ms[k] = t.last_ms;
+    s.mean_ms[k] = t.count ? (float)(t.acc_ms / (double)t.count) : 0.0f;
+    s.calls[k] = t.count;
+  }
+  (void)hipGetLastError();
+  if (c->counting) {
+    unsigned long long v[4] = {0, 0, 0, 0};
+    ANE_CHECK_HIP(hipMemcpy(v, c->d_stats, sizeof(v), hipMemcpyDeviceToHost));
+    s.primary_nodes = v[0]; s.primary_tris = v[1]; s.shadow_nodes = v[2]; s.shadow_tris = v[3];
+  }
+  uint32_t ov = 0;
+  ANE_CHECK_HIP(hipMemcpy(&ov, c->d_overflow, 4, hipMemcpyDeviceToHost));
+  if (ov) return fail(ANE_ERR_DEVICE, "traversal stack exceeded its spill capacity");
+  *out = s;
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_stats_reset(ane_ddgi_ctx* c)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  for (auto& t : c->timers) {
+    ANE_TRY(t.harvest_all());
+    t.reset();
+  }
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_trace_rays(const ane_scene* s, const float* origins, const float* dirs,
+                                    int64_t num_rays, float t_min, float t_max, int32_t any_hit,
+                                    ane_hit* hits, uint64_t* counters2, void* stream)
+{
+  if (!s || !origins || !dirs || !hits) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (num_rays <= 0) return ANE_OK;
+  if (num_rays >= ((int64_t)1 << 31)) return fail(ANE_ERR_INVALID_ARG, "too many rays");
+  if (!(t_min >= 0.0f) || !(t_max > t_min)) return fail(ANE_ERR_INVALID_ARG, "need 0 <= t_min < t_max");
+  hipStream_t st = S(stream);
+  float *d_o = nullptr, *d_d = nullptr;
+  ane_hit* d_h = nullptr;
+  uint32_t* d_work = nullptr;
+  unsigned long long* d_stats = nullptr;
+  uint32_t* d_ov = nullptr;
+  int32_t* d_spill = nullptr;
+  const int64_t lanes = trace_lanes_per_launch();
+  const int64_t sd = s->depth - trace_stack_lds() + 1 > 0 ? s->depth - trace_stack_lds() + 1 : 0;
+  int st_ = ANE_OK;
+  auto cleanup = [&]() { dfree(d_o); dfree(d_d); dfree(d_h); dfree(d_work); dfree(d_stats); dfree(d_ov); dfree(d_spill); };
+  if ((st_ = dalloc(&d_o, num_rays * 3, nullptr)) || (st_ = dalloc(&d_d, num_rays * 3, nullptr)) ||
+      (st_ = dalloc(&d_h, num_rays, nullptr)) || (st_ = dalloc(&d_work, 1, nullptr)) ||
+      (st_ = dalloc(&d_stats, 2, nullptr)) || (st_ = dalloc(&d_ov, 1, nullptr)) ||
+      (st_ = dalloc(&d_spill, sd * lanes, nullptr))) {
+    cleanup();
+    return st_;
+  }
+  hipError_t e;
+  if ((e = hipMemcpyAsync(d_o, origins, 12 * num_rays, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(d_d, dirs, 12 * num_rays, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemsetAsync(d_stats, 0, 16, st)) || (e = hipMemsetAsync(d_ov, 0, 4, st))) {
+    cleanup();
+    return fail_hip(e, "upload", __FILE__, __LINE__);
+  }
+  fill_u32(d_work, 0u, 1, st);
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g.t_min = t_min;
+  a.g.t_max = t_max;
+  a.g.rays = 1;
+  a.hits = d_h;
+  a.work = d_work;
+  a.stats = d_stats;
+  a.overflow = d_ov;
+  a.spill = d_spill;
+  a.spill_depth = (int32_t)sd;
+  a.nrays = (int32_t)num_rays;
+  a.ray_orig = d_o;
+  a.ray_dir = d_d;
+  a.t_min = t_min;
+  a.t_max = t_max;
+  launch_trace(a, any_hit != 0, true, counters2 != nullptr, st);
+  unsigned long long c2[2] = {0, 0};
+  uint32_t ov = 0;
+  if ((e = hipGetLastError()) ||
+      (e = hipMemcpyAsync(hits, d_h, sizeof(ane_hit) * num_rays, hipMemcpyDeviceToHost, st)) ||
+      (e = hipMemcpyAsync(c2, d_stats, 16, hipMemcpyDeviceToHost, st)) ||
+      (e = hipMemcpyAsync(&ov, d_ov, 4, hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st))) {
+    cleanup();
+    return fail_hip(e, "trace", __FILE__, __LINE__);
+  }
+  cleanup();
+  if (ov) return fail(ANE_ERR_DEVICE, "traversal stack exceeded its spill capacity");
+  if (counters2) { counters2[0] = c2[0]; counters2[1] = c2[1]; }
+  return ANE_OK;
+}
+
+// ===========================================================================
+// library
+// ===========================================================================
+extern "C" int ane_abi_version(void) { return ANE_DDGI_ABI_VERSION; }
+extern "C" const char* ane_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* ane_status_string(int s)
+{
+  switch (s) {
+    case ANE_OK: return "ok";
+    case ANE_ERR_INVALID_ARG: return "invalid argument";
+    case ANE_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case ANE_ERR_ADMISSION_REFUSED: return "admission refused";
+    case ANE_ERR_DEVICE: return "device error";
+    case ANE_ERR_NOT_READY: return "not ready";
+    case ANE_ERR_UNSUPPORTED: return "unsupported";
+    case ANE_ERR_CANCELLED: return "cancelled";
+  }
+  return "unknown status";
+}
+
+extern "C" int ane_device(int32_t* device)
+{
+  if (!device) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  int d = 0;
+  ANE_CHECK_HIP(hipGetDevice(&d));
+  *device = d;
+  return ANE_OK;
+}
+
+extern "C" int ane_stream_synchronize(void* stream)
+{
+  ANE_CHECK_HIP(hipStreamSynchronize(S(stream)));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_ray_directions(int32_t n, const float* rot16, float* out)
+{
+  if (n <= 0 || !rot16 || !out) return fail(ANE_ERR_INVALID_ARG, "bad arguments");
+  std::vector<float> base((size_t)n * 3);
+  fibonacci_dirs(n, base.data());
+  for (int i = 0; i < n; ++i) {
+    f3 d = rot3(rot16, ld3(base.data() + 3 * i));
+    out[3 * i + 0] = d.x;
+    out[3 * i + 1] = d.y;
+    out[3 * i + 2] = d.z;
+  }
+  return ANE_OK;
+}

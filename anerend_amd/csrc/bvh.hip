@@ -1,0 +1,497 @@
+// bvh.hip -- LBVH build and refit on gfx950.
+//
+// Replaces the driver's acceleration-structure build
+// (VulkanRenderer.cpp:1267-1417 BLAS, UpdateTLASPass.cpp:184-228 TLAS,
+// UpdateBlasPass.cpp:131-138 refit).  Instances are flattened to world-space
+// triangles and one LBVH (Karras 2012) is built over them:
+//   flatten -> centroid bounds -> 63-bit Morton keys -> stable LSD radix sort
+//   (8 x 8-bit passes over (key, id)) -> Karras hierarchy -> triangle reorder
+//   -> bottom-up child boxes (atomic arrival counters) -> depth.
+// Every step is integer or exact min/max arithmetic on binary32 values
+// computed with the same float contract as the oracle, so the topology and
+// the boxes are bit-exact with the CPU build.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ane {
+
+static constexpr int kBlock = 256;
+static constexpr int kItems = 8;
+static constexpr int kTile = kBlock * kItems;
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2o(float f)
+{
+  uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float o2f(uint32_t o)
+{
+  uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+  return __uint_as_float(b);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+}
+
+// ---------------------------------------------------------------------------
+// flatten: world-space triangles in input order + centroid bounds
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32_t* tri_inst,
+                                                 uint32_t* cbounds, int32_t write_inst)
+{
+  __shared__ uint32_t sb[6];
+  if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
+  __syncthreads();
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < sc.ntri) {
+    // instance k: last k with inst_tri_off[k] <= t
+    int lo = 0, hi = sc.ninst - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (sc.inst_tri_off[mid] <= (uint32_t)t) lo = mid; else hi = mid - 1;
+    }
+    const ane_instance& in = sc.inst[lo];
+    const ane_mesh& m = sc.meshes[mesh_of(in.custom_index_and_mask)];
+    uint32_t prim = (uint32_t)t - sc.inst_tri_off[lo];
+    float w[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      uint32_t idx = sc.indices[m.index_offset + 3 * prim + j];
+      f3 p = ld3(sc.vertices + (int64_t)(m.vertex_offset + idx) * 24);
+      f3 q = xform_point(in.transform, p);
+      w[3 * j + 0] = q.x;
+      w[3 * j + 1] = q.y;
+      w[3 * j + 2] = q.z;
+    }
+    float* dst = world + (int64_t)9 * t;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) dst[j] = w[j];
+    if (write_inst) tri_inst[t] = (uint32_t)lo;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float l = fminf(fminf(w[a], w[3 + a]), w[6 + a]);
+      float h = fmaxf(fmaxf(w[a], w[3 + a]), w[6 + a]);
+      float c = (l + h) * 0.5f;
+      atomicMin(&sb[a], f2o(c));
+      atomicMax(&sb[3 + a], f2o(c));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMin(&cbounds[threadIdx.x], sb[threadIdx.x]);
+  else if (threadIdx.x < 6) atomicMax(&cbounds[threadIdx.x], sb[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys (contract: quantization of the centroid in a cube of side
+// max extent, 21 bits per axis, x most significant)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t spread21(uint64_t v)
+{
+  v &= 0x1FFFFFull;
+  v = (v | (v << 32)) & 0x1F00000000FFFFull;
+  v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quant21(float c, float cmin, float e)
+{
+  float q = (c - cmin) / e * 2097152.0f;
+  q = fminf(fmaxf(q, 0.0f), 2097151.0f);
+  return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float* world, int n, const uint32_t* cbounds,
+                                                uint64_t* keys, uint32_t* ids)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float cmin[3], cmax[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cmin[a] = o2f(cbounds[a]);
+    cmax[a] = o2f(cbounds[3 + a]);
+  }
+  float e = fmaxf(fmaxf(cmax[0] - cmin[0], cmax[1] - cmin[1]), cmax[2] - cmin[2]);
+  if (!(e > 0.0f)) e = 1.0f;
+  const float* w = world + (int64_t)9 * t;
+  uint64_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float l = fminf(fminf(w[a], w[3 + a]), w[6 + a]);
+    float h = fmaxf(fmaxf(w[a], w[3 + a]), w[6 + a]);
+    float c = (l + h) * 0.5f;
+    q[a] = quant21(c, cmin[a], e);
+  }
+  keys[t] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+  ids[t] = (uint32_t)t;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (uint32), three-kernel block scan, recursive over levels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+{
+  __shared__ uint32_t ws[kBlock / 64];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wave] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) wbase += ws[w];
+    tot += ws[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return wbase + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_block(uint32_t* data, int64_t n, uint32_t* sums)
+{
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    v[i] = (base + i < n) ? data[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan(s, &total);
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    if (base + i < n) data[base + i] = pre;
+    pre += v[i];
+  }
+  if (threadIdx.x == 0 && sums) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* data, int64_t n, const uint32_t* sums)
+{
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  uint32_t add = sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i)
+    if (base + i < n) data[base + i] += add;
+}
+
+// scratch must hold scan_scratch_words(n) words
+int64_t scan_scratch_words(int64_t n)
+{
+  int64_t w = 0;
+  while (n > kTile) {
+    n = (n + kTile - 1) / kTile;
+    w += n;
+  }
+  return w + 1;
+}
+
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t s)
+{
+  int64_t blocks = (n + kTile - 1) / kTile;
+  if (blocks <= 1) {
+    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kBlock), 0, s, data, n, (uint32_t*)nullptr);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_block, dim3((unsigned)blocks), dim3(kBlock), 0, s, data, n, scratch);
+  exclusive_scan(scratch, blocks, scratch + blocks, s);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)blocks), dim3(kBlock), 0, s, data, n,
+                     (const uint32_t*)scratch);
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (uint64 key, uint32 value), 8-bit digits
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* keys, int64_t n, int shift,
+                                                      uint32_t* hist, int64_t ntiles)
+{
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    int64_t idx = base + i * kBlock + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin,
+                                                         uint64_t* kout, uint32_t* vout, int64_t n,
+                                                         int shift, const uint32_t* offs,
+                                                         int64_t ntiles)
+{
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[kBlock / 64][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  run[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int r = 0; r < kItems; ++r) {
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    int64_t idx = base + r * kBlock + threadIdx.x;
+    bool valid = idx < n;
+    uint64_t key = valid ? kin[idx] : 0ull;
+    uint32_t val = valid ? vin[idx] : 0u;
+    uint32_t d = (uint32_t)(key >> shift) & 255u;
+    uint64_t mask = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bal = __ballot(bit);
+      mask &= bit ? bal : ~bal;
+    }
+    uint32_t rank = (uint32_t)__popcll(mask & lt);
+    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = run[d];
+      for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
+      uint32_t dst = pre + rank;
+      if (dst < (uint64_t)n) {
+        kout[dst] = key;
+        vout[dst] = val;
+      }
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) add += wcnt[w][threadIdx.x];
+    run[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+int64_t radix_scratch_words(int64_t n)
+{
+  int64_t ntiles = (n + kTile - 1) / kTile;
+  int64_t h = 256 * ntiles;
+  return h + scan_scratch_words(h);
+}
+
+// Sorts (keys, vals) in place using (k2, v2) as ping-pong buffers (8 passes:
+// the result ends in keys/vals).  scratch: radix_scratch_words(n) words.
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2, int64_t n,
+                      uint32_t* scratch, hipStream_t s)
+{
+  if (n <= 1) return;
+  int64_t ntiles = (n + kTile - 1) / kTile;
+  uint32_t* hist = scratch;
+  uint32_t* sc = scratch + 256 * ntiles;
+  uint64_t* ka = keys;
+  uint32_t* va = vals;
+  uint64_t* kb = k2;
+  uint32_t* vb = v2;
+  for (int pass = 0; pass < 8; ++pass) {
+    int shift = 8 * pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s, (const uint64_t*)ka,
+                       n, shift, hist, ntiles);
+    exclusive_scan(hist, 256 * ntiles, sc, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
+                       (const uint64_t*)ka, (const uint32_t*)va, kb, vb, n, shift,
+                       (const uint32_t*)hist, ntiles);
+    uint64_t* tk = ka; ka = kb; kb = tk;
+    uint32_t* tv = va; va = vb; vb = tv;
+  }
+  // 8 passes: data is back in (keys, vals)
+}
+
+// ---------------------------------------------------------------------------
+// Karras 2012 hierarchy
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kdelta(const uint64_t* k, int n, int i, int j)
+{
+  if (j < 0 || j >= n) return -1;
+  uint64_t a = k[i], b = k[j];
+  if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint64_t* k, int n, NodeBuild* nodes,
+                                                int32_t* leaf_parent)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (kdelta(k, n, i, i + 1) - kdelta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = kdelta(k, n, i, i - d);
+  int lmax = 2;
+  while (kdelta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(k, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = kdelta(k, n, i, j);
+  int sp = 0;
+  int div = 2;
+  int t = (l + div - 1) / div;
+  for (;;) {
+    if (kdelta(k, n, i, i + (sp + t) * d) > dnode) sp += t;
+    if (t <= 1) break;
+    div *= 2;
+    t = (l + div - 1) / div;
+  }
+  int gamma = i + sp * d + (d < 0 ? d : 0);
+  int lo = min(i, j), hi = max(i, j);
+  int L = (lo == gamma) ? ~gamma : gamma;
+  int R = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  nodes[i].d.x = L;
+  nodes[i].d.y = R;
+  if (i == 0) nodes[0].d.z = -1;
+  if (L < 0) leaf_parent[~L] = i; else nodes[L].d.z = i;
+  if (R < 0) leaf_parent[~R] = i; else nodes[R].d.z = i;
+}
+
+// ---------------------------------------------------------------------------
+// triangle reorder and bottom-up boxes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reorder(const float* world, const uint32_t* ids, int n,
+                                                 float4* tris)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t id = ids[p];
+  const float* w = world + (int64_t)9 * id;
+  tris[3 * (int64_t)p + 0] = make_float4(w[0], w[1], w[2], __uint_as_float(id));
+  tris[3 * (int64_t)p + 1] = make_float4(w[3], w[4], w[5], 0.0f);
+  tris[3 * (int64_t)p + 2] = make_float4(w[6], w[7], w[8], 0.0f);
+}
+
+__device__ __forceinline__ void write_child_box(NodeBuild* nd, int slot, const float* lo,
+                                                const float* hi)
+{
+  if (slot == 0) {
+    nd->a.x = lo[0]; nd->a.y = hi[0]; nd->a.z = lo[1]; nd->a.w = hi[1];
+    nd->c.x = lo[2]; nd->c.y = hi[2];
+  } else {
+    nd->b.x = lo[0]; nd->b.y = hi[0]; nd->b.z = lo[1]; nd->b.w = hi[1];
+    nd->c.z = lo[2]; nd->c.w = hi[2];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fit(const float4* tris, const int32_t* leaf_parent, int n,
+                                             NodeBuild* nodes, uint32_t* arrive, float* node_box)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float4 v0 = tris[3 * (int64_t)p], v1 = tris[3 * (int64_t)p + 1], v2 = tris[3 * (int64_t)p + 2];
+  float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y),
+                 fminf(fminf(v0.z, v1.z), v2.z)};
+  float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y),
+                 fmaxf(fmaxf(v0.z, v1.z), v2.z)};
+  int child = ~p;
+  int node = leaf_parent[p];
+  for (int level = 0; node >= 0 && node < n - 1 && level < 128; ++level) {
+    NodeBuild* nd = &nodes[node];
+    int slot = (nd->d.x == child) ? 0 : 1;
+    write_child_box(nd, slot, lo, hi);
+    __threadfence();
+    uint32_t old = atomicAdd(&arrive[node], 1u);
+    if (old == 0) return;
+    __threadfence();
+    volatile NodeBuild* vn = nd;
+    float l0x = vn->a.x, h0x = vn->a.y, l0y = vn->a.z, h0y = vn->a.w, l0z = vn->c.x, h0z = vn->c.y;
+    float l1x = vn->b.x, h1x = vn->b.y, l1y = vn->b.z, h1y = vn->b.w, l1z = vn->c.z, h1z = vn->c.w;
+    lo[0] = fminf(l0x, l1x); lo[1] = fminf(l0y, l1y); lo[2] = fminf(l0z, l1z);
+    hi[0] = fmaxf(h0x, h1x); hi[1] = fmaxf(h0y, h1y); hi[2] = fmaxf(h0z, h1z);
+    float* nb = node_box + 6 * (int64_t)node;
+    nb[0] = lo[0]; nb[1] = lo[1]; nb[2] = lo[2]; nb[3] = hi[0]; nb[4] = hi[1]; nb[5] = hi[2];
+    child = node;
+    node = vn->d.z;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_depth(const int32_t* leaf_parent, const NodeBuild* nodes,
+                                               int n, uint32_t* depth)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t d = 0;
+  int node = leaf_parent[p];
+  while (node >= 0 && node < n - 1 && d < 1024u) {
+    ++d;
+    node = nodes[node].d.z;
+  }
+  atomicMax(depth, d);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, uint32_t* cbounds,
+                    bool write_inst, hipStream_t s)
+{
+  if (sc.ntri <= 0) return;
+  hipLaunchKernelGGL(k_flatten, dim3(blocks_for(sc.ntri, 256)), dim3(256), 0, s, sc, world,
+                     tri_inst, cbounds, (int32_t)write_inst);
+}
+
+void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
+                   uint32_t* ids, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_morton, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, n, cbounds, keys,
+                     ids);
+}
+
+void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
+                   hipStream_t s)
+{
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, keys, n, nodes,
+                     leaf_parent);
+}
+
+void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, ids, n, tris);
+}
+
+void launch_fit(const float4* tris, const int32_t* leaf_parent, int n, NodeBuild* nodes,
+                uint32_t* arrive, float* node_box, hipStream_t s)
+{
+  if (n < 2) return;
+  fill_u32(arrive, 0u, n - 1, s);
+  hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, tris, leaf_parent, n, nodes,
+                     arrive, node_box);
+}
+
+void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,
+                  hipStream_t s)
+{
+  fill_u32(depth, 0u, 1, s);
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_depth, dim3(blocks_for(n, 256)), dim3(256), 0, s, leaf_parent, nodes, n,
+                     depth);
+}
+
+}  // namespace ane

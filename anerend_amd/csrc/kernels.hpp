@@ -1,0 +1,123 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+
+#include "common.hpp"
+
+namespace ane {
+
+// bvh.hip
+void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+int64_t scan_scratch_words(int64_t n);
+void exclusive_scan(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t s);
+int64_t radix_scratch_words(int64_t n);
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2, int64_t n,
+                      uint32_t* scratch, hipStream_t s);
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, uint32_t* cbounds,
+                    bool write_inst, hipStream_t s);
+void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
+                   uint32_t* ids, hipStream_t s);
+void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
+                   hipStream_t s);
+void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s);
+void launch_fit(const float4* tris, const int32_t* leaf_parent, int n, NodeBuild* nodes,
+                uint32_t* arrive, float* node_box, hipStream_t s);
+void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,
+                  hipStream_t s);
+
+// trace.hip
+struct TraceArgs {
+  SceneK sc;
+  GridK g;
+  FrameK f;
+  const float* dirs;      // rotated directions [N][3]
+  const uint32_t* order;  // processing order of ray indices [N]
+  ane_hit* hits;          // closest: out; shadow: in
+  uint8_t* shadow;        // shadow: out (0 lit, 1 shadowed, 255 miss)
+  uint32_t* work;         // dynamic ray counter (zeroed before launch)
+  unsigned long long* stats; // [2] nodes, triangles (counting kernels)
+  uint32_t* overflow;     // set to 1 if a stack would exceed the spill capacity
+  int32_t* spill;         // [spill_depth][total_lanes]
+  int32_t spill_depth;
+  int32_t total_lanes;
+  int32_t nrays;          // probes_in_range * N
+  int32_t probe_base;     // first probe of the range
+  float sun[3];           // normalize(-light_dir)
+  // arbitrary-ray mode (origins/dirs given per ray)
+  const float* ray_orig;
+  const float* ray_dir;
+  float t_min, t_max;
+};
+
+int trace_lanes_per_launch();
+int trace_stack_lds();
+void launch_trace(const TraceArgs& a, bool shadow, bool any_hit_rays, bool count, hipStream_t s);
+void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
+                        float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8, float* wfp8_scale,
+                        bool bake, hipStream_t s);
+void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
+                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
+                  float* rad_scale, int num_probes, int probe_base, hipStream_t s);
+
+// conv.hip
+void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,
+                     const float* rad_scale, const float* w32, const float* wsum,
+                     const uint16_t* wbf16, const uint8_t* wfp8, const float* wfp8_scale,
+                     float* atlas, int num_probes, int probe_base, bool bake, hipStream_t s);
+void launch_quantize(int precision, const float* src, void* dst, float* scale, int num_probes,
+                     int n, hipStream_t s);
+void launch_dequantize(int precision, const void* src, const float* scale, float* dst,
+                       int num_probes, int n, hipStream_t s);
+
+// translate.hip
+void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, int dz,
+                      hipStream_t s);
+
+// numeric formats (software RNE conversions shared by shade and quantize)
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float x)
+{
+  uint32_t b;
+  __builtin_memcpy(&b, &x, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+__host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
+{
+  uint32_t b = (uint32_t)h << 16;
+  float x;
+  __builtin_memcpy(&x, &b, 4);
+  return x;
+}
+
+// OCP FP8 E4M3 (e4m3fn: bias 7, max 448, no infinities), round to nearest even,
+// saturating at 448.
+__host__ __device__ __forceinline__ uint8_t f32_to_e4m3(float x)
+{
+  uint32_t b;
+  __builtin_memcpy(&b, &x, 4);
+  uint32_t s = (b >> 24) & 0x80u;
+  uint32_t a = b & 0x7FFFFFFFu;
+  if (a >= 0x43E00000u) return (uint8_t)(s | 0x7Eu);  // >= 448
+  if (a < 0x3C800000u) {                               // < 2^-6: subnormal m * 2^-9
+    float ax;
+    __builtin_memcpy(&ax, &a, 4);
+    float m = rintf(ax * 512.0f);
+    return (uint8_t)(s | (uint32_t)m);
+  }
+  uint32_t r = a + 0x7FFFFu + ((a >> 20) & 1u);
+  uint32_t e = (r >> 23) - 127u + 7u;
+  uint32_t m = (r >> 20) & 7u;
+  uint32_t code = (e << 3) | m;
+  if (code > 0x7Eu) code = 0x7Eu;
+  return (uint8_t)(s | code);
+}
+
+__host__ __device__ __forceinline__ float e4m3_to_f32(uint8_t q)
+{
+  uint32_t s = (q & 0x80u) ? 1u : 0u;
+  uint32_t e = (q >> 3) & 15u, m = q & 7u;
+  float v = (e == 0) ? (float)m * 0.001953125f : ldexpf(1.0f + (float)m * 0.125f, (int)e - 7);
+  return s ? -v : v;
+}
+
+}  // namespace ane

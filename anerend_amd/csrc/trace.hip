@@ -1,0 +1,634 @@
+// trace.hip -- probe ray traversal and hit shading on gfx950.
+//
+// Replaces vkCmdTraceRaysKHR of IrradianceProbeRT (IrradianceProbeRayTracingPass.cpp:95-165)
+// and its shaders irradiance_probe_update.{rgen,rchit,rmiss}:
+//   k_frame_setup  rotated Fibonacci directions (rgen:35-46,86-88) and the 64 x N
+//                  cosine-lobe weight matrix of the convolution (probe_conv.comp:37-61)
+//   k_trace        persistent wave-wide BVH traversal, one ray per lane, ballot-based
+//                  refill of finished lanes, LDS ring stack with a global spill area;
+//                  closest hit (rgen:90-106) or any hit toward the sun (rchit:44-66)
+//   k_shade        one block per probe: attribute fetch, Cook-Torrance direct light,
+//                  indirect light from the previous atlas, emissive, sky on miss
+//                  (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
+//                  context precision (FP8: per-probe scale from the block maximum)
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace ane {
+
+static constexpr int kTB = 128;     // threads per traversal block (2 waves)
+static constexpr int kSS = 16;      // LDS stack entries per lane (ring)
+static constexpr int kSteps = 32;   // traversal steps between refill checks
+static constexpr int kRefill = 16;  // idle lanes that trigger a refill
+
+int trace_stack_lds() { return kSS; }
+
+// ---------------------------------------------------------------------------
+// frame setup: rotated directions and convolution weights
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, const float* base,
+                                                     float* dirs, float* w32, float* wsum,
+                                                     uint16_t* wbf16, uint8_t* wfp8,
+                                                     float* wfp8_scale, int32_t bake)
+{
+  const int t = blockIdx.x;  // texel 0..63
+  const int n = g.rays;
+  const f3 o = texel_dir(t);
+  __shared__ float red[256];
+  float s = 0.0f, mx = 0.0f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    f3 d = rot3(rot.m, ld3(base + 3 * r));
+    if (t == 0) {
+      dirs[3 * r + 0] = d.x;
+      dirs[3 * r + 1] = d.y;
+      dirs[3 * r + 2] = d.z;
+    }
+    float w = fmaxf(0.0f, dot(d, o));
+    if (!(w > 1e-6f)) w = 0.0f;
+    w32[(int64_t)t * n + r] = w;
+    s += w;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const float sum = red[0];
+  __syncthreads();
+  // normalized weights (E = sum W L / sum W); bake mode keeps them raw
+  const bool norm = !bake && sum > 1e-6f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    float w = w32[(int64_t)t * n + r];
+    if (norm) w = w / sum;
+    w32[(int64_t)t * n + r] = w;
+    wbf16[(int64_t)t * n + r] = f32_to_bf16(w);
+    mx = fmaxf(mx, w);
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  const float wmax = red[0];
+  const float sc = wmax > 0.0f ? wmax / 448.0f : 1.0f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x)
+    wfp8[(int64_t)t * n + r] = f32_to_e4m3(w32[(int64_t)t * n + r] / sc);
+  if (threadIdx.x == 0) {
+    wsum[t] = sum;
+    wfp8_scale[t] = sc;
+  }
+}
+
+void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
+                        float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8, float* wfp8_scale,
+                        bool bake, hipStream_t s)
+{
+  Mat3 rot;
+  for (int i = 0; i < 16; ++i) rot.m[i] = f.rot[i];
+  hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, rot, base_dirs, dirs, w32, wsum,
+                     wbf16, wfp8, wfp8_scale, (int32_t)bake);
+}
+
+// ---------------------------------------------------------------------------
+// hit position (rchit:30-42): object-space barycentric interpolation, then the
+// instance transform.  Bit-exact with the oracle (it is the shadow-ray origin).
+// ---------------------------------------------------------------------------
+struct HitGeom {
+  uint32_t inst;
+  uint32_t mesh;
+  uint32_t v[3];  // absolute vertex indices
+};
+
+__device__ __forceinline__ HitGeom hit_geom(const SceneK& sc, int32_t id)
+{
+  HitGeom h;
+  h.inst = sc.tri_inst[id];
+  const ane_instance& in = sc.inst[h.inst];
+  h.mesh = mesh_of(in.custom_index_and_mask);
+  const ane_mesh& m = sc.meshes[h.mesh];
+  uint32_t prim = (uint32_t)id - sc.inst_tri_off[h.inst];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) h.v[j] = m.vertex_offset + sc.indices[m.index_offset + 3 * prim + j];
+  return h;
+}
+
+__device__ __forceinline__ f3 hit_pos(const SceneK& sc, const HitGeom& h, float u, float v)
+{
+  f3 b = mk(1.0f - u - v, u, v);
+  f3 p0 = ld3(sc.vertices + (int64_t)h.v[0] * 24);
+  f3 p1 = ld3(sc.vertices + (int64_t)h.v[1] * 24);
+  f3 p2 = ld3(sc.vertices + (int64_t)h.v[2] * 24);
+  f3 pos = add(add(muls(p0, b.x), muls(p1, b.y)), muls(p2, b.z));
+  return xform_point(sc.inst[h.inst].transform, pos);
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+template <bool SHADOW, bool RAYS, bool COUNT>
+__global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
+{
+  __shared__ int32_t stk[kSS][kTB];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int gl = blockIdx.x * kTB + tid;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const SceneK& sc = a.sc;
+  const float tmin = RAYS ? a.t_min : a.g.t_min;
+  const float tmax0 = RAYS ? a.t_max : a.g.t_max;
+  const int n = a.g.rays;
+
+  int32_t ray = -1;  // index into [0, nrays) of the active ray, -1 idle
+  bool exhausted = false;
+  RayPre r;
+  float tbest = 0.0f, bu = 0.0f, bv = 0.0f;
+  int32_t bid = -1, node = 0, sp = 0, out = 0;
+  uint32_t steps_left = 0;  // per-ray step budget: a tree traversal visits < 2 * ntri nodes
+  unsigned long long cn = 0, ct = 0;
+
+  for (;;) {
+    // ---- refill idle lanes with new rays (one atomic per wave) ----
+    const bool idle = (ray < 0) && !exhausted;
+    const uint64_t im = __ballot(idle);
+    const uint64_t am = __ballot(ray >= 0);
+    if (im != 0ull && (am == 0ull || __popcll(im) >= kRefill)) {
+      const int leader = __ffsll((unsigned long long)im) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(a.work, (uint32_t)__popcll(im));
+      base = __shfl(base, leader, 64);
+      if (idle) {
+        const uint32_t gidx = base + (uint32_t)__popcll(im & lt);
+        if (gidx >= (uint32_t)a.nrays) {
+          exhausted = true;
+        } else {
+          f3 o, d;
+          bool skip = false;
+          if (RAYS) {
+            out = (int32_t)gidx;
+            o = ld3(a.ray_orig + 3 * (int64_t)gidx);
+            d = ld3(a.ray_dir + 3 * (int64_t)gidx);
+          } else {
+            const int ql = (int)(gidx / (uint32_t)n);
+            const int j = (int)(gidx - (uint32_t)ql * (uint32_t)n);
+            const int i = (int)a.order[j];
+            out = ql * n + i;
+            if (SHADOW) {
+              const ane_hit h = a.hits[out];
+              if (h.tri < 0) {
+                a.shadow[out] = 255;
+                skip = true;
+              } else {
+                HitGeom hg = hit_geom(sc, h.tri);
+                o = hit_pos(sc, hg, h.u, h.v);
+                d = mk(a.sun[0], a.sun[1], a.sun[2]);
+              }
+            } else {
+              const int q = a.probe_base + ql;
+              const int px = q % a.g.nx;
+              const int pz = (q / a.g.nx) % a.g.nz;
+              const int py = q / (a.g.nx * a.g.nz);
+              o = probe_origin(a.g, a.f, px, py, pz);
+              d = ld3(a.dirs + 3 * i);
+            }
+          }
+          if (!skip) {
+            ray_setup(r, o, d);
+            tbest = tmax0;
+            bid = -1;
+            bu = bv = 0.0f;
+            sp = 0;
+            steps_left = 2u * (uint32_t)sc.ntri + 8u;
+            ray = (int32_t)gidx;
+            if (sc.ntri >= 2) {
+              node = 0;
+            } else if (sc.ntri == 1) {
+              node = ~0;
+            } else {
+              node = 0x7FFFFFFF;  // empty scene: finish immediately
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(ray >= 0) == 0ull) {
+      if (__ballot(!exhausted) == 0ull) break;
+      continue;
+    }
+    // ---- advance the active rays by up to kSteps traversal steps ----
+    if (ray >= 0) {
+      bool done = (node == 0x7FFFFFFF);
+      for (int step = 0; step < kSteps && !done; ++step) {
+        bool pop = true;
+        // guards: a malformed hierarchy ends the ray and raises the error flag
+        // instead of reading out of bounds or looping forever
+        if (steps_left == 0u || (node >= 0 && node >= sc.ntri - 1) || (node < 0 && ~node >= sc.ntri)) {
+          *a.overflow = 1u;
+          done = true;
+          break;
+        }
+        --steps_left;
+        if (node >= 0) {
+          if (COUNT) ++cn;
+          const float4* nd = sc.nodes + 4 * (int64_t)node;
+          const float4 na = nd[0], nb = nd[1], nc = nd[2], nn = nd[3];
+          float tn0, tn1;
+          const bool b0 = box_test(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tbest, tn0);
+          const bool b1 = box_test(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tbest, tn1);
+          const int32_t L = __float_as_int(nn.x), R = __float_as_int(nn.y);
+          if (b0 || b1) {
+            pop = false;
+            if (b0 && b1) {
+              const bool sw = tn1 < tn0;
+              const int32_t nearc = sw ? R : L;
+              const int32_t farc = sw ? L : R;
+              // push far child (LDS ring, spill the oldest entry when full)
+              const int slot = sp & (kSS - 1);
+              if (sp >= kSS) {
+                const int si = sp - kSS;
+                if (si < a.spill_depth) {
+                  a.spill[(int64_t)si * a.total_lanes + gl] = stk[slot][tid];
+                } else {
+                  *a.overflow = 1u;
+                  done = true;
+                  break;
+                }
+              }
+              stk[slot][tid] = farc;
+              ++sp;
+              node = nearc;
+            } else {
+              node = b0 ? L : R;
+            }
+          }
+        } else {
+          if (COUNT) ++ct;
+          const int p = ~node;
+          const float4 v0 = sc.tris[3 * (int64_t)p], v1 = sc.tris[3 * (int64_t)p + 1],
+                       v2 = sc.tris[3 * (int64_t)p + 2];
+          float t, u, v;
+          if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin,
+                       tbest, t, u, v)) {
+            const int32_t id = __float_as_int(v0.w);
+            if (SHADOW) {
+              bid = id;
+              tbest = t;
+              bu = u;
+              bv = v;
+              done = true;
+              break;
+            }
+            if (t < tbest || (t == tbest && id < bid)) {
+              bid = id;
+              tbest = t;
+              bu = u;
+              bv = v;
+            }
+          }
+        }
+        if (pop) {
+          if (sp == 0) {
+            done = true;
+            break;
+          }
+          --sp;
+          const int slot = sp & (kSS - 1);
+          node = stk[slot][tid];
+          if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
+        }
+      }
+      if (done) {
+        if (SHADOW && !RAYS) {
+          a.shadow[out] = (bid >= 0) ? 1 : 0;
+        } else {
+          ane_hit h;
+          h.tri = bid;
+          h.t = (bid >= 0) ? tbest : 0.0f;
+          h.u = (bid >= 0) ? bu : 0.0f;
+          h.v = (bid >= 0) ? bv : 0.0f;
+          a.hits[out] = h;
+        }
+        ray = -1;
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&a.stats[0], cn);
+    atomicAdd(&a.stats[1], ct);
+  }
+}
+
+static int g_lanes = 0;
+
+int trace_lanes_per_launch()
+{
+  if (g_lanes) return g_lanes;
+  int dev = 0, cus = 0, per = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<false, false, false>, kTB, 0);
+  if (cus <= 0) cus = 256;
+  if (per <= 0) per = 8;
+  g_lanes = cus * per * kTB;
+  return g_lanes;
+}
+
+void launch_trace(const TraceArgs& a, bool shadow, bool rays, bool count, hipStream_t s)
+{
+  const int blocks = trace_lanes_per_launch() / kTB;
+  TraceArgs b = a;
+  b.total_lanes = blocks * kTB;
+#define ANE_TR(S, R, C) hipLaunchKernelGGL((k_trace<S, R, C>), dim3(blocks), dim3(kTB), 0, s, b)
+  if (rays) {
+    if (shadow) { if (count) ANE_TR(true, true, true); else ANE_TR(true, true, false); }
+    else { if (count) ANE_TR(false, true, true); else ANE_TR(false, true, false); }
+  } else {
+    if (shadow) { if (count) ANE_TR(true, false, true); else ANE_TR(true, false, false); }
+    else { if (count) ANE_TR(false, false, true); else ANE_TR(false, false, false); }
+  }
+#undef ANE_TR
+}
+
+// ---------------------------------------------------------------------------
+// shading
+// ---------------------------------------------------------------------------
+static constexpr float kPi = 3.14159265359f;  // pbr_light.glsl:8
+
+__device__ __forceinline__ float distribution_ggx(f3 N, f3 H, float rough)
+{
+  float a = rough * rough;
+  float a2 = a * a;
+  float ndoth = fmaxf(dot(N, H), 0.0f);
+  float ndoth2 = ndoth * ndoth;
+  float num = a2 + 0.001f;
+  float den = ndoth2 * (a2 - 1.0f) + 1.0f;
+  den = kPi * den * den;
+  return num / den;
+}
+
+__device__ __forceinline__ float schlick_ggx(float ndotv, float rough)
+{
+  float r = rough + 1.0f;
+  float k = (r * r) / 8.0f;
+  return ndotv / (ndotv * (1.0f - k) + k);
+}
+
+__device__ __forceinline__ float pow5_term(float c)
+{
+  return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f);
+}
+
+// calcLight, pbr_light.glsl:98-166 (directional, specular on)
+__device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float rough, f3 pos,
+                         f3 view)
+{
+  f3 light_color = muls(mk(1.0f, 1.0f, 1.0f), f.sun_intensity);
+  f3 V = normalize(sub(view, pos));
+  f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
+  f3 L = normalize(mk(-f.light_dir[0], -f.light_dir[1], -f.light_dir[2]));
+  f3 H = normalize(add(V, L));
+  f3 radiance = muls(light_color, 1.0f);
+  float ndf = distribution_ggx(N, H, rough);
+  float ndotv_ = fmaxf(dot(N, V), 0.0f);
+  float ndotl_ = fmaxf(dot(N, L), 0.0f);
+  float G = schlick_ggx(ndotl_, rough) * schlick_ggx(ndotv_, rough);
+  float p5 = pow5_term(fmaxf(dot(H, V), 0.0f));
+  f3 F = add(F0, muls(sub(mk(1.0f, 1.0f, 1.0f), F0), p5));
+  f3 kD = muls(sub(mk(1.0f, 1.0f, 1.0f), F), 1.0f - metallic);
+  float ndotl = fmaxf(dot(N, L), 0.0f);
+  float ndotv = fmaxf(dot(N, V), 0.0f);
+  f3 numer = muls(F, ndf * G);
+  float denom = 4.0f * ndotv * ndotl + 0.0001f;
+  f3 spec = divs(numer, denom);
+  spec = mk(clampf(spec.x, 0.0f, 1.0f), clampf(spec.y, 0.0f, 1.0f), clampf(spec.z, 0.0f, 1.0f));
+  f3 lo = add(divs(mul(kD, albedo), kPi), spec);
+  return muls(mul(lo, radiance), ndotl);
+}
+
+__device__ __forceinline__ int imod(int a, int m)
+{
+  int r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+// texture() with LINEAR filtering and REPEAT addressing (PipelineUtil.h:98-112),
+// binary32 weights.
+__device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float v)
+{
+  float x = u * (float)W - 0.5f;
+  float y = v * (float)H - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float ax = x - fx, ay = y - fy;
+  int x0 = imod((int)fx, W), x1 = imod((int)fx + 1, W);
+  int y0 = imod((int)fy, H), y1 = imod((int)fy + 1, H);
+  float4 t00 = atlas[(int64_t)y0 * W + x0];
+  float4 t10 = atlas[(int64_t)y0 * W + x1];
+  float4 t01 = atlas[(int64_t)y1 * W + x0];
+  float4 t11 = atlas[(int64_t)y1 * W + x1];
+  float r0 = t00.x * (1.0f - ax) + t10.x * ax, r1 = t01.x * (1.0f - ax) + t11.x * ax;
+  float g0 = t00.y * (1.0f - ax) + t10.y * ax, g1 = t01.y * (1.0f - ax) + t11.y * ax;
+  float b0 = t00.z * (1.0f - ax) + t10.z * ax, b1 = t01.z * (1.0f - ax) + t11.z * ax;
+  return mk(r0 * (1.0f - ay) + r1 * ay, g0 * (1.0f - ay) + g1 * ay, b0 * (1.0f - ay) + b1 * ay);
+}
+
+// sampleSingleProbe, pbr_light.glsl:168-188
+__device__ f3 sample_single_probe(const GridK& g, const float4* atlas, int ix, int iy, int iz, f3 n)
+{
+  const int W = g.atlas_w, H = g.atlas_h;
+  int sx = 10 * ix + 1;
+  int sy = 10 * g.nz * iy + 10 * iz + 1;
+  int ex = sx + 7, ey = sy + 7;
+  float su = (2.0f * (float)sx + 1.0f) / (2.0f * (float)W);
+  float sv = (2.0f * (float)sy + 1.0f) / (2.0f * (float)H);
+  float eu = (2.0f * (float)ex + 1.0f) / (2.0f * (float)W);
+  float ev = (2.0f * (float)ey + 1.0f) / (2.0f * (float)H);
+  float ox, oy;
+  oct_encode(normalize(n), ox, oy);
+  ox = (ox + 1.0f) * 0.5f;
+  oy = (oy + 1.0f) * 0.5f;
+  float u = su + (eu - su) * ox;
+  float v = sv + (ev - sv) * oy;
+  return bilinear_repeat(atlas, W, H, u, v);
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// sampleProbe, pbr_light.glsl:240-363 (no baked tiles)
+__device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas, f3 pos, f3 n)
+{
+  int px, py, pz;
+  f3 base, step;
+  if (f.flags & ANE_FLAG_BAKE) {
+    int ts = f.bake_tile[2];
+    f3 toff = mk((float)f.bake_tile[0] * (float)ts, 0.0f, (float)f.bake_tile[1] * (float)ts);
+    step = mk((float)ts / (float)g.nx, 1.0f, (float)ts / (float)g.nz);
+    f3 pit = sub(pos, toff);
+    px = clampi((int)floorf(pit.x / step.x), 0, g.nx - 1);
+    py = clampi((int)floorf(pit.y / step.y), 0, g.ny - 1);
+    pz = clampi((int)floorf(pit.z / step.z), 0, g.nz - 1);
+    base = add(mul(mk((float)px, (float)py, (float)pz), step), toff);
+  } else {
+    step = mk(g.step[0], g.step[1], g.step[2]);
+    f3 off = cam_offset(g, f.cam);
+    px = clampi((int)floorf((pos.x - off.x) / step.x), 0, g.nx - 1);
+    py = clampi((int)floorf(pos.y / step.y), 0, g.ny - 1);
+    pz = clampi((int)floorf((pos.z - off.z) / step.z), 0, g.nz - 1);
+    base = add(mul(mk((float)px, (float)py, (float)pz), step), off);
+  }
+  f3 alpha = mk(clampf((pos.x - base.x) / step.x, 0.0f, 1.0f),
+                clampf((pos.y - base.y) / step.y, 0.0f, 1.0f),
+                clampf((pos.z - base.z) / step.z, 0.0f, 1.0f));
+  // corner order of pbr_light.glsl:318-355, packed as ox | oy << 1 | oz << 2
+  const int corners = (0) | (7 << 3) | (3 << 6) | (1 << 9) | (6 << 12) | (2 << 15) | (4 << 18) |
+                      (5 << 21);
+  float sumw = 0.0f;
+  f3 sumi = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int c = (corners >> (3 * k)) & 7;
+    const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+    f3 ppos = add(base, mul(mk((float)ox, (float)oy, (float)oz), step));
+    f3 tri = mk(ox ? alpha.x : 1.0f - alpha.x, oy ? alpha.y : 1.0f - alpha.y,
+                oz ? alpha.z : 1.0f - alpha.z);
+    f3 dir = normalize(sub(ppos, pos));
+    float x = fmaxf(0.0001f, (dot(dir, n) + 1.0f) * 0.5f);
+    float w = x * x + 0.2f;
+    w = fmaxf(0.000001f, w);
+    if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
+    w = w * (tri.x * tri.y * tri.z);
+    f3 s = sample_single_probe(g, atlas, px + ox, py + oy, pz + oz, n);
+    s = mk(sqrtf(s.x), sqrtf(s.y), sqrtf(s.z));
+    sumi = add(sumi, muls(s, w));
+    sumw = sumw + w;
+  }
+  f3 net = divs(sumi, sumw);
+  net = mul(net, net);
+  return muls(muls(net, 0.5f), 3.1415926f);
+}
+
+// calcIndirectDiffuseLight, pbr_light.glsl:365-388
+__device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas, f3 N, f3 albedo,
+                            float metallic, float rough, f3 pos, f3 view)
+{
+  f3 V = normalize(sub(view, pos));
+  f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
+  float r = rough * rough;
+  float p5 = pow5_term(fmaxf(dot(N, V), 0.0f));
+  float omr = 1.0f - r;
+  f3 kS = mk(F0.x + (fmaxf(omr, F0.x) - F0.x) * p5, F0.y + (fmaxf(omr, F0.y) - F0.y) * p5,
+             F0.z + (fmaxf(omr, F0.z) - F0.z) * p5);
+  f3 kD = muls(sub(mk(1.0f, 1.0f, 1.0f), kS), 1.0f - metallic);
+  f3 irr = sample_probe(g, f, atlas, pos, N);
+  f3 diffuse = mul(irr, albedo);
+  return mul(kD, diffuse);
+}
+
+// rchit:24-110 (+ rmiss:17-21 for misses)
+__device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const float4* atlas,
+                        f3 origin, ane_hit h, uint8_t shadowed)
+{
+  if (h.tri < 0) return mk(f.sky[0], f.sky[1], f.sky[2]);
+  HitGeom hg = hit_geom(sc, h.tri);
+  const ane_instance& in = sc.inst[hg.inst];
+  const ane_material& mat = sc.mats[mat_of(in.custom_index_and_mask)];
+  f3 pos = hit_pos(sc, hg, h.u, h.v);
+  const bool lit = shadowed == 0;
+  float bw[3] = {1.0f - h.u - h.v, h.u, h.v};
+  f3 nrm = mk(0.0f, 0.0f, 0.0f), col = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float4* vp = reinterpret_cast<const float4*>(sc.vertices + (int64_t)hg.v[j] * 24);
+    float4 d0 = vp[0], d1 = vp[1];
+    float d2x = sc.vertices[(int64_t)hg.v[j] * 24 + 8];
+    f3 nj = mk(d1.z, d1.w, d2x);
+    f3 cj = mk(d0.w, d1.x, d1.y);
+    nrm = add(nrm, muls(nj, bw[j]));
+    col = add(col, muls(cj, bw[j]));
+  }
+  nrm = normalize(nrm);
+  const float* wi = sc.inv_xform + 12 * (int64_t)hg.inst;
+  f3 nw = mk(nrm.x * wi[0] + nrm.y * wi[4] + nrm.z * wi[8],
+             nrm.x * wi[1] + nrm.y * wi[5] + nrm.z * wi[9],
+             nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
+  nrm = normalize(nw);
+  col = mk(to_linear1(col.x), to_linear1(col.y), to_linear1(col.z));
+  f3 albedo = col;
+  if (mat.base_col_fac[0] + mat.base_col_fac[1] + mat.base_col_fac[2] > 0.1f)
+    albedo = mk(mat.base_col_fac[0], mat.base_col_fac[1], mat.base_col_fac[2]);
+  const float metallic = mat.met_rough[0];
+  const float rough = mat.met_rough[1];
+  f3 emissive = muls(mk(mat.emissive[0], mat.emissive[1], mat.emissive[2]), mat.emissive[3]);
+  f3 acc = mk(0.0f, 0.0f, 0.0f);
+  if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, pos, origin);
+  if (f.flags & ANE_FLAG_DDGI_MULTI)
+    acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, origin));
+  return add(acc, emissive);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_shade(SceneK sc, GridK g, FrameK f, const ane_hit* hits,
+                                               const uint8_t* shadow, const float4* atlas,
+                                               void* rad, float* rad_scale, int probe_base)
+{
+  extern __shared__ float sL[];  // FP8: [3][N] staging for the per-probe scale
+  __shared__ float red[256];
+  const int ql = blockIdx.x;
+  const int q = probe_base + ql;
+  const int n = g.rays;
+  const int px = q % g.nx, pz = (q / g.nx) % g.nz, py = q / (g.nx * g.nz);
+  const f3 origin = probe_origin(g, f, px, py, pz);
+  float mx = 0.0f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t idx = (int64_t)ql * n + i;
+    const f3 L = shade_ray(sc, g, f, atlas, origin, hits[idx], shadow[idx]);
+    if (PREC == ANE_PRECISION_FP32) {
+      float* o = (float*)rad + (int64_t)ql * 3 * n;
+      o[i] = L.x;
+      o[n + i] = L.y;
+      o[2 * n + i] = L.z;
+    } else if (PREC == ANE_PRECISION_BF16) {
+      uint16_t* o = (uint16_t*)rad + (int64_t)ql * 3 * n;
+      o[i] = f32_to_bf16(L.x);
+      o[n + i] = f32_to_bf16(L.y);
+      o[2 * n + i] = f32_to_bf16(L.z);
+    } else {
+      sL[i] = L.x;
+      sL[n + i] = L.y;
+      sL[2 * n + i] = L.z;
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(L.x), fabsf(L.y)), fabsf(L.z)));
+    }
+  }
+  if (PREC == ANE_PRECISION_FP8) {
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+      __syncthreads();
+    }
+    const float sc8 = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
+    uint8_t* o = (uint8_t*)rad + (int64_t)ql * 3 * n;
+    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) o[i] = f32_to_e4m3(sL[i] / sc8);
+    if (threadIdx.x == 0) rad_scale[ql] = sc8;
+  }
+}
+
+void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
+                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
+                  float* rad_scale, int num_probes, int probe_base, hipStream_t s)
+{
+  if (num_probes <= 0) return;
+  const float4* a4 = reinterpret_cast<const float4*>(atlas);
+  if (precision == ANE_PRECISION_FP32)
+    hipLaunchKernelGGL(k_shade<ANE_PRECISION_FP32>, dim3(num_probes), dim3(256), 0, s, sc, g, f,
+                       hits, shadow, a4, rad, rad_scale, probe_base);
+  else if (precision == ANE_PRECISION_BF16)
+    hipLaunchKernelGGL(k_shade<ANE_PRECISION_BF16>, dim3(num_probes), dim3(256), 0, s, sc, g, f,
+                       hits, shadow, a4, rad, rad_scale, probe_base);
+  else
+    hipLaunchKernelGGL(k_shade<ANE_PRECISION_FP8>, dim3(num_probes), dim3(256),
+                       (size_t)12 * g.rays, s, sc, g, f, hits, shadow, a4, rad, rad_scale,
+                       probe_base);
+}
+
+}  // namespace ane

@@ -1,0 +1,275 @@
+"""ctypes binding of libane_ddgi.so (the C-ABI declared in include/ane_ddgi.h).
+
+The library is built in-tree by ``python -m anerend_amd.build``.  There is no
+fallback: if the shared library is missing every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .types import HIT_DTYPE, Config, Frame, ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libane_ddgi.so")
+
+ANE_OK = 0
+STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "admission refused",
+          4: "device error", 5: "not ready", 6: "unsupported", 7: "cancelled"}
+
+
+class AneError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_int64), ("num_instances", ctypes.c_int32),
+                ("bvh_depth", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
+                ("build_ms", ctypes.c_float)]
+
+
+STAGES = ("setup", "trace", "shadow", "shade", "convolve", "translate")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("last_ms", ctypes.c_float * 6), ("mean_ms", ctypes.c_float * 6),
+                ("calls", ctypes.c_uint64 * 6),
+                ("primary_nodes", ctypes.c_uint64), ("primary_tris", ctypes.c_uint64),
+                ("shadow_nodes", ctypes.c_uint64), ("shadow_tris", ctypes.c_uint64),
+                ("rays", ctypes.c_uint64), ("device_bytes", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_[3:]}
+        d["last_ms"] = dict(zip(STAGES, self.last_ms))
+        d["mean_ms"] = dict(zip(STAGES, self.mean_ms))
+        d["calls"] = dict(zip(STAGES, self.calls))
+        return d
+
+
+# (name, restype, argtypes); every status-returning entry point is int
+_vp, _i32, _i64, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+_SIGS = [
+    ("ane_abi_version", ctypes.c_int, []),
+    ("ane_last_error", ctypes.c_char_p, []),
+    ("ane_status_string", ctypes.c_char_p, [ctypes.c_int]),
+    ("ane_device", ctypes.c_int, [_vp]),
+    ("ane_stream_synchronize", ctypes.c_int, [_vp]),
+    ("ane_scene_create", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp]),
+    ("ane_scene_refit", ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    ("ane_scene_rebuild", ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    ("ane_scene_info_get", ctypes.c_int, [_vp, _vp]),
+    ("ane_scene_destroy", ctypes.c_int, [_vp]),
+    ("ane_ddgi_create", ctypes.c_int, [_vp, _vp]),
+    ("ane_ddgi_destroy", ctypes.c_int, [_vp]),
+    ("ane_ddgi_translate", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_trace", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_convolve", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_update", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_atlas_get", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_atlas_set", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_atlas_device", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_translation_state", ctypes.c_int, [_vp, _vp, _i32]),
+    ("ane_ddgi_radiance_set", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_radiance_get", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_debug_rays", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("ane_ddgi_set_counting", ctypes.c_int, [_vp, _i32]),
+    ("ane_ddgi_stats_get", ctypes.c_int, [_vp, _vp]),
+    ("ane_ddgi_stats_reset", ctypes.c_int, [_vp]),
+    ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
+    ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
+    ("ane_scene_trace_rays", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
+    ("ane_ddgi_ray_directions", ctypes.c_int, [_i32, _vp, _vp]),
+]
+
+EXPORTED = [name for name, _, _ in _SIGS]
+
+_lib = None
+
+
+def load():
+    """Load libane_ddgi.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m anerend_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != ANE_OK:
+        raise AneError(status, load().ane_last_error().decode())
+    return status
+
+
+def ray_directions(n, rotation16):
+    """Host-computed rotated Fibonacci directions (no device work)."""
+    rot = np.ascontiguousarray(rotation16, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    check(load().ane_ddgi_ray_directions(n, ptr(rot), ptr(out)))
+    return out
+
+
+class DeviceScene:
+    """Geometry + LBVH resident in HBM (replaces the BLAS/TLAS builds)."""
+
+    def __init__(self, scene, stream=None):
+        L = load()
+        self._h = ctypes.c_void_p()
+        arrs = [np.ascontiguousarray(a) for a in
+                (scene.vertices, scene.indices, scene.meshes, scene.materials, scene.instances)]
+        v, i, m, mt, inst = arrs
+        check(L.ane_scene_create(ptr(v), len(v), ptr(i), len(i), ptr(m), len(m), ptr(mt), len(mt),
+                                 ptr(inst), len(inst), stream, ctypes.byref(self._h)))
+        self.instances = inst.copy()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().ane_scene_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    __del__ = close
+
+    def info(self):
+        s = SceneInfo()
+        check(load().ane_scene_info_get(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def refit(self, instances, stream=None):
+        inst = np.ascontiguousarray(instances)
+        check(load().ane_scene_refit(self._h, ptr(inst), len(inst), stream))
+
+    def rebuild(self, instances, stream=None):
+        inst = np.ascontiguousarray(instances)
+        check(load().ane_scene_rebuild(self._h, ptr(inst), len(inst), stream))
+
+    def bvh(self):
+        n = self.info()["num_triangles"]
+        m = max(n - 1, 0)
+        out = dict(keys=np.zeros(n, np.uint64), ids=np.zeros(n, np.uint32),
+                   left=np.zeros(m, np.int32), right=np.zeros(m, np.int32),
+                   parent=np.zeros(m, np.int32), boxes=np.zeros((m, 6), np.float32))
+        check(load().ane_scene_debug_bvh(self._h, ptr(out["keys"]), ptr(out["ids"]), ptr(out["left"]),
+                                         ptr(out["right"]), ptr(out["parent"]), ptr(out["boxes"])))
+        return out
+
+    def world_tris(self):
+        n = self.info()["num_triangles"]
+        out = np.zeros((n, 9), np.float32)
+        check(load().ane_scene_debug_world_tris(self._h, ptr(out)))
+        return out
+
+    def trace_rays(self, orig, dirs, tmin=0.1, tmax=50.0, any_hit=False, count=False, stream=None):
+        orig = np.ascontiguousarray(orig, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        hits = np.zeros(len(orig), HIT_DTYPE)
+        ctr = np.zeros(2, np.uint64) if count else None
+        check(load().ane_scene_trace_rays(self._h, ptr(orig), ptr(dirs), len(orig), tmin, tmax,
+                                          int(any_hit), ptr(hits), ptr(ctr), stream))
+        return (hits, ctr) if count else hits
+
+
+class ProbeGrid:
+    """One DDGI probe grid (atlas, radiance, per-pass state) on the device."""
+
+    def __init__(self, config: Config):
+        self.config = config
+        self._h = ctypes.c_void_p()
+        check(load().ane_ddgi_create(ctypes.byref(config), ctypes.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().ane_ddgi_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    __del__ = close
+
+    # -- passes --
+    def translate(self, camera_pos, stream=None):
+        cam = np.ascontiguousarray(camera_pos[:3], np.float32)
+        shifted = ctypes.c_int32(0)
+        check(load().ane_ddgi_translate(self._h, ptr(cam), ctypes.byref(shifted), stream))
+        return bool(shifted.value)
+
+    def trace(self, scene: DeviceScene, frame: Frame, stream=None):
+        check(load().ane_ddgi_trace(self._h, scene.handle, ctypes.byref(frame), stream))
+
+    def convolve(self, frame: Frame, stream=None):
+        check(load().ane_ddgi_convolve(self._h, ctypes.byref(frame), stream))
+
+    def update(self, scene: DeviceScene, frame: Frame, stream=None):
+        check(load().ane_ddgi_update(self._h, scene.handle, ctypes.byref(frame), stream))
+
+    # -- state --
+    @property
+    def atlas(self):
+        a = np.zeros(self.config.atlas_shape, np.float32)
+        check(load().ane_ddgi_atlas_get(self._h, ptr(a), None))
+        return a
+
+    @atlas.setter
+    def atlas(self, value):
+        a = np.ascontiguousarray(value, np.float32)
+        assert a.shape == self.config.atlas_shape
+        check(load().ane_ddgi_atlas_set(self._h, ptr(a), None))
+
+    def atlas_device(self):
+        p = ctypes.c_void_p()
+        w = ctypes.c_int64()
+        h = ctypes.c_int64()
+        check(load().ane_ddgi_atlas_device(self._h, ctypes.byref(p), ctypes.byref(w), ctypes.byref(h)))
+        return p.value, w.value, h.value
+
+    def translation_state(self, cell=None):
+        c = np.zeros(2, np.int32) if cell is None else np.ascontiguousarray(cell, np.int32)
+        check(load().ane_ddgi_translation_state(self._h, ptr(c), int(cell is not None)))
+        return c
+
+    def set_radiance(self, frame, radiance):
+        r = np.ascontiguousarray(radiance, np.float32)
+        check(load().ane_ddgi_radiance_set(self._h, ctypes.byref(frame), ptr(r), None))
+
+    def radiance(self, frame):
+        c = self.config
+        n = c.probes_x * c.probes_z * frame.layer_count
+        r = np.zeros((n, 3, c.rays_per_probe), np.float32)
+        check(load().ane_ddgi_radiance_get(self._h, ptr(r), None))
+        return r
+
+    def debug_rays(self, frame):
+        c = self.config
+        n = c.probes_x * c.probes_z * frame.layer_count * c.rays_per_probe
+        hits = np.zeros(n, HIT_DTYPE)
+        sh = np.zeros(n, np.uint8)
+        check(load().ane_ddgi_debug_rays(self._h, ptr(hits), ptr(sh), None))
+        return hits, sh
+
+    def set_counting(self, on=True):
+        check(load().ane_ddgi_set_counting(self._h, int(on)))
+
+    def stats(self):
+        s = Stats()
+        check(load().ane_ddgi_stats_get(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        check(load().ane_ddgi_stats_reset(self._h))
+
+
+def synchronize(stream=None):
+    check(load().ane_stream_synchronize(stream))

@@ -1,0 +1,422 @@
+"""Seeded synthetic DDGI scenes (inputs only; no reference data is involved).
+
+Randomness is PCG32 (O'Neill 2014, ``pcg32_srandom_r`` / ``pcg32_random_r``)
+vectorised over independent streams, with explicit maps:
+
+* ``uniform(a, b)  = a + (b - a) * u``, ``u = (x >> 8) * 2**-24`` in binary64, stored binary32
+* ``log_uniform(a, b) = exp(log a + (log b - log a) * u)``
+
+so the scene bytes do not depend on numpy's or the C++ library's
+distribution implementations.  Scene families:
+
+* ``room_scene``    a 10 x 4 x 10 m closed room plus random axis-aligned boxes
+* ``terrain_scene`` a value-noise heightfield plus randomly rotated and scaled
+                    box / icosphere instances
+
+Records use the layouts of :mod:`anerend_amd.types` (the reference's GPU
+structs).  Instance custom indices pack the material in bits 0-9 and the mesh
+in bits 10-23 as ``tlas_update.comp:89-92`` does.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .types import (FLAG_DDGI, FLAG_DDGI_MULTI, INSTANCE_DTYPE, MATERIAL_DTYPE, MESH_DTYPE,
+                    VERTEX_WORDS, Config, Frame)
+
+_MULT = np.uint64(6364136223846793005)
+
+
+class PCG32:
+    """PCG-XSH-RR 64/32, one generator per stream id, stepped in lockstep."""
+
+    def __init__(self, seed, streams):
+        streams = np.atleast_1d(np.asarray(streams, dtype=np.uint64))
+        self.inc = (streams << np.uint64(1)) | np.uint64(1)
+        self.state = np.zeros_like(self.inc)
+        self._step()
+        self.state = self.state + np.uint64(seed)
+        self._step()
+
+    def _step(self):
+        self.state = self.state * _MULT + self.inc
+
+    def next_u32(self):
+        old = self.state
+        self._step()
+        xorshifted = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)).astype(np.uint32)
+        rot = (old >> np.uint64(59)).astype(np.uint32)
+        return (xorshifted >> rot) | (xorshifted << ((np.uint32(32) - rot) & np.uint32(31)))
+
+    def unit(self):
+        return (self.next_u32() >> np.uint32(8)).astype(np.float64) * (2.0 ** -24)
+
+    def uniform(self, a, b):
+        return a + (b - a) * self.unit()
+
+    def log_uniform(self, a, b):
+        return np.exp(math.log(a) + (math.log(b) - math.log(a)) * self.unit())
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+def pack_vertices(pos, color=None, normal=None, uv=None):
+    """Pack into the 96-byte vertex (bindless.glsl:111-121 packVertex)."""
+    n = len(pos)
+    v = np.zeros((n, VERTEX_WORDS), np.float32)
+    color = np.ones((n, 3), np.float32) if color is None else color
+    normal = np.zeros((n, 3), np.float32) if normal is None else normal
+    uv = np.zeros((n, 2), np.float32) if uv is None else uv
+    v[:, 0:3] = pos
+    v[:, 3] = color[:, 0]
+    v[:, 4:6] = color[:, 1:3]
+    v[:, 6:8] = normal[:, 0:2]
+    v[:, 8] = normal[:, 2]
+    v[:, 9:12] = 0.0          # tangent.xyz
+    v[:, 12] = 1.0            # tangent.w
+    v[:, 17:19] = uv
+    return v
+
+
+def unit_cube(inward=False):
+    """[-0.5, 0.5]^3, 24 vertices (flat faces), 12 triangles."""
+    faces = []
+    for axis in range(3):
+        for s in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = s
+            u = np.zeros(3)
+            u[(axis + 1) % 3] = 1.0
+            w = np.cross(n, u)
+            corners = [0.5 * n + 0.5 * (a * u + b * w) for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1))]
+            faces.append((np.array(corners), -n if inward else n))
+    pos, nrm, idx = [], [], []
+    for f, (c, n) in enumerate(faces):
+        base = 4 * f
+        pos.extend(c)
+        nrm.extend([n] * 4)
+        tri = [(0, 1, 2), (0, 2, 3)]
+        if inward:
+            tri = [(0, 2, 1), (0, 3, 2)]
+        for t in tri:
+            idx.extend(base + k for k in t)
+    return np.array(pos, np.float32), np.array(nrm, np.float32), np.array(idx, np.uint32)
+
+
+def icosphere(subdiv=1, radius=0.5):
+    """Icosahedron subdivided ``subdiv`` times (80 triangles for subdiv=1)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+             (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.array(verts)
+    return (v * radius).astype(np.float32), v.astype(np.float32), np.array(faces, np.uint32).ravel()
+
+
+def _hash2(seed, ix, iz):
+    """Lattice value in [-1, 1]: one PCG32 output keyed by (seed, ix, iz)."""
+    key = (ix.astype(np.int64) & 0xFFFFF) * 1048576 + (iz.astype(np.int64) & 0xFFFFF)
+    g = PCG32(seed, key.astype(np.uint64).ravel())
+    return (g.unit() * 2.0 - 1.0).reshape(ix.shape)
+
+
+def value_noise(seed, x, z, octaves=4, base_period=64.0):
+    """Fractal value noise in [-1, 1] (smoothstep-interpolated lattice values)."""
+    total = np.zeros_like(x, dtype=np.float64)
+    amp, norm, period = 1.0, 0.0, base_period
+    for o in range(octaves):
+        fx, fz = x / period, z / period
+        ix, iz = np.floor(fx), np.floor(fz)
+        tx, tz = fx - ix, fz - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        ix, iz = ix.astype(np.int64), iz.astype(np.int64)
+        s = seed * 16 + o
+        v00, v10 = _hash2(s, ix, iz), _hash2(s, ix + 1, iz)
+        v01, v11 = _hash2(s, ix, iz + 1), _hash2(s, ix + 1, iz + 1)
+        total += amp * ((v00 * (1 - sx) + v10 * sx) * (1 - sz) + (v01 * (1 - sx) + v11 * sx) * sz)
+        norm += amp
+        amp *= 0.5
+        period *= 0.5
+    return total / norm
+
+
+def terrain_mesh(seed, cells, spacing=1.0, amplitude=8.0):
+    """(cells+1)^2 vertex heightfield centred on the origin, 2*cells^2 triangles
+    (two per cell, split like TerrainSystem.cpp:201-220)."""
+    g = cells + 1
+    half = cells * spacing / 2.0
+    xs = np.arange(g) * spacing - half
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    H = amplitude * value_noise(seed, X, Z)
+    e = spacing
+    Hx = amplitude * (value_noise(seed, X + e, Z) - value_noise(seed, X - e, Z)) / (2 * e)
+    Hz = amplitude * (value_noise(seed, X, Z + e) - value_noise(seed, X, Z - e)) / (2 * e)
+    nrm = np.stack([-Hx, np.ones_like(H), -Hz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    pos = np.stack([X, H, Z], -1).reshape(-1, 3).astype(np.float32)
+    # sRGB vertex colour: grass to rock by height
+    t = np.clip((H.ravel() + amplitude) / (2 * amplitude), 0.0, 1.0)[:, None]
+    col = ((1 - t) * np.array([0.25, 0.45, 0.15]) + t * np.array([0.55, 0.5, 0.45])).astype(np.float32)
+    uv = np.stack([(X.ravel() + half) / (2 * half), (Z.ravel() + half) / (2 * half)], -1).astype(np.float32)
+    i = np.arange(cells)
+    I, J = np.meshgrid(i, i, indexing="xy")
+    v00 = (J * g + I).ravel()
+    v10, v01, v11 = v00 + 1, v00 + g, v00 + g + 1
+    idx = np.stack([v00, v01, v10, v10, v01, v11], -1).ravel().astype(np.uint32)
+    return pos, nrm.reshape(-1, 3).astype(np.float32), col, uv, idx, H
+
+
+# ---------------------------------------------------------------------------
+# scene container
+# ---------------------------------------------------------------------------
+@dataclass
+class Scene:
+    vertices: np.ndarray
+    indices: np.ndarray
+    meshes: np.ndarray
+    materials: np.ndarray
+    instances: np.ndarray
+    info: dict = field(default_factory=dict)
+
+    @property
+    def num_tris(self):
+        mesh = (self.instances["custom_index_and_mask"] >> 10) & 0x3FFF
+        return int((self.meshes["index_count"][mesh] // 3).sum())
+
+
+class _MeshBuilder:
+    def __init__(self):
+        self.verts, self.idx, self.meshes = [], [], []
+        self.nv = self.ni = 0
+
+    def add(self, pos, nrm, idx, color=None, uv=None):
+        m = np.zeros((), MESH_DTYPE)
+        m["min_pos"][:3] = pos.min(0)
+        m["max_pos"][:3] = pos.max(0)
+        m["vertex_offset"] = self.nv
+        m["index_offset"] = self.ni
+        m["index_count"] = len(idx)
+        self.verts.append(pack_vertices(pos, color, nrm, uv))
+        self.idx.append(idx.astype(np.uint32))
+        self.meshes.append(m)
+        self.nv += len(pos)
+        self.ni += len(idx)
+        return len(self.meshes) - 1
+
+    def arrays(self):
+        return (np.ascontiguousarray(np.concatenate(self.verts)),
+                np.ascontiguousarray(np.concatenate(self.idx)),
+                np.ascontiguousarray(np.array(self.meshes, MESH_DTYPE)))
+
+
+def random_materials(g, n, emissive_p=0.05, emissive_intensity=5.0):
+    """Albedo U[0.1, 0.9]^3, roughness U[0.2, 1], metallic 1 w.p. 0.2, emissive w.p. 0.05."""
+    m = np.zeros(n, MATERIAL_DTYPE)
+    alb = np.stack([g.uniform(0.1, 0.9) for _ in range(3)], -1)
+    rough = g.uniform(0.2, 1.0)
+    metal = (g.unit() < 0.2).astype(np.float64)
+    emis = g.unit() < emissive_p
+    m["base_col_fac"][:, :3] = alb
+    m["base_col_fac"][:, 3] = 1.0
+    m["met_rough"][:, 0] = metal
+    m["met_rough"][:, 1] = rough
+    m["emissive"][:, :3] = np.where(emis[:, None], alb, 0.0)
+    m["emissive"][:, 3] = np.where(emis, emissive_intensity, 0.0)
+    m["bindless_indices"] = -1
+    return m
+
+
+def make_instance(mesh, mat, rot, scale, trans):
+    """3x4 row-major object-to-world = [R * diag(scale) | t] (tlas_update.comp:70-84 layout)."""
+    inst = np.zeros((), INSTANCE_DTYPE)
+    M = np.asarray(rot, np.float64) * np.asarray(scale, np.float64)[None, :]
+    T = np.concatenate([M, np.asarray(trans, np.float64)[:, None]], 1)
+    inst["transform"] = T.astype(np.float32).ravel()
+    inst["custom_index_and_mask"] = (mat & 0x3FF) | ((mesh & 0x3FFF) << 10) | (0xFF << 24)
+    inst["sbt_offset_and_flags"] = 1 << 24
+    return inst
+
+
+def _random_rotations(g):
+    """Uniform random rotation matrices (Shoemake 1992 quaternion), one per stream."""
+    u1, u2, u3 = g.unit(), g.unit(), g.unit()
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], -1)
+    x, y, z, w = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+    return R
+
+
+# ---------------------------------------------------------------------------
+# scene families
+# ---------------------------------------------------------------------------
+def room_scene(seed=1, num_boxes=64):
+    """10 x 4 x 10 m closed room (floor at y = -0.5) plus ``num_boxes`` boxes with
+    sides U[0.2, 1.5] m; about 12 + 12 * num_boxes triangles."""
+    mb = _MeshBuilder()
+    p, n, i = unit_cube(inward=True)
+    room_mesh = mb.add(p, n, i)
+    p, n, i = unit_cube()
+    box_mesh = mb.add(p, n, i)
+    g = PCG32(seed, np.arange(num_boxes + 1) + 1000)
+    mats = random_materials(g, num_boxes + 1)
+    mats["emissive"][0] = 0.0          # the room itself does not glow
+    gb = PCG32(seed, np.arange(num_boxes) + 5000)
+    size = np.stack([gb.uniform(0.2, 1.5) for _ in range(3)], -1)
+    cx = gb.uniform(-5.0 + size[:, 0] / 2, 5.0 - size[:, 0] / 2)
+    cy = gb.uniform(-0.5 + size[:, 1] / 2, 3.5 - size[:, 1] / 2)
+    cz = gb.uniform(-5.0 + size[:, 2] / 2, 5.0 - size[:, 2] / 2)
+    inst = [make_instance(room_mesh, 0, np.eye(3), (10.0, 4.0, 10.0), (0.0, 1.5, 0.0))]
+    for k in range(num_boxes):
+        inst.append(make_instance(box_mesh, k + 1, np.eye(3), size[k], (cx[k], cy[k], cz[k])))
+    v, idx, meshes = mb.arrays()
+    return Scene(v, idx, meshes, mats, np.array(inst, INSTANCE_DTYPE),
+                 {"family": "room", "seed": seed, "boxes": num_boxes})
+
+
+def terrain_scene(seed=2, cells=512, num_instances=10000, amplitude=8.0, spread=None,
+                  ico_subdiv=1):
+    """Value-noise heightfield of ``cells``^2 quads (2 triangles each, 1 m spacing,
+    centred on the origin, amplitude ``amplitude`` m) plus ``num_instances``
+    instances, half unit boxes (12 tris) and half icospheres (80 tris at
+    subdiv 1), randomly rotated, scaled U[0.5, 3] and placed on the terrain."""
+    mb = _MeshBuilder()
+    tp, tn, tc, tuv, ti, H = terrain_mesh(seed, cells, 1.0, amplitude)
+    terrain = mb.add(tp, tn, ti, color=tc, uv=tuv)
+    p, n, i = unit_cube()
+    box = mb.add(p, n, i)
+    p, n, i = icosphere(ico_subdiv)
+    ico = mb.add(p, n, i)
+    nmat = 1024
+    g = PCG32(seed, np.arange(nmat) + 1000)
+    mats = random_materials(g, nmat)
+    mats["base_col_fac"][0] = 0.0      # terrain: vertex colour (bindless.glsl:205-209)
+    mats["met_rough"][0] = (0.0, 0.9, 0.0, 0.0)
+    mats["emissive"][0] = 0.0
+    inst = np.zeros(num_instances + 1, INSTANCE_DTYPE)
+    inst[0] = make_instance(terrain, 0, np.eye(3), (1, 1, 1), (0, 0, 0))
+    if num_instances:
+        gi = PCG32(seed, np.arange(num_instances) + 100000)
+        R = _random_rotations(gi)
+        s = np.stack([gi.uniform(0.5, 3.0) for _ in range(3)], -1)
+        half = cells / 2.0 - 4.0 if spread is None else spread
+        x, z = gi.uniform(-half, half), gi.uniform(-half, half)
+        matid = 1 + (gi.next_u32() % np.uint32(nmat - 1)).astype(np.int64)
+        lift = gi.uniform(-0.5, 2.0)
+        hx = amplitude * value_noise(seed, x, z)
+        k = np.arange(num_instances)
+        is_box = (k % 2) == 0
+        s = np.where(is_box[:, None], s, s[:, :1])      # icospheres: uniform scale
+        M = R * s[:, None, :]
+        T = np.concatenate([M, np.stack([x, hx + lift, z], -1)[:, :, None]], 2)
+        inst["transform"][1:] = T.reshape(num_instances, 12).astype(np.float32)
+        mesh = np.where(is_box, box, ico).astype(np.uint32)
+        inst["custom_index_and_mask"][1:] = (matid.astype(np.uint32) & 0x3FF) | (mesh << 10) | (0xFF << 24)
+        inst["sbt_offset_and_flags"][1:] = 1 << 24
+    v, idx, meshes = mb.arrays()
+    return Scene(v, idx, meshes, mats, inst,
+                 {"family": "terrain", "seed": seed, "cells": cells, "instances": num_instances})
+
+
+# ---------------------------------------------------------------------------
+# frames
+# ---------------------------------------------------------------------------
+def rotation_matrix(angle_deg, axis):
+    """glm::rotate(mat4(1), radians(angle), axis) in binary32, column-major 16 floats
+    (IrradianceProbeRayTracingPass.cpp:114-133 draws angle U[0, 360) and axis in {x, y, z})."""
+    a = np.float32(np.float32(angle_deg) * np.float32(math.pi / 180.0))
+    c, s = np.float32(math.cos(a)), np.float32(math.sin(a))
+    ax = np.zeros(3, np.float32)
+    ax[axis] = 1.0
+    temp = (np.float32(1.0) - c) * ax
+    R = np.zeros((4, 4), np.float32)   # R[col][row]
+    R[0][0] = c + temp[0] * ax[0]
+    R[0][1] = temp[0] * ax[1] + s * ax[2]
+    R[0][2] = temp[0] * ax[2] - s * ax[1]
+    R[1][0] = temp[1] * ax[0] - s * ax[2]
+    R[1][1] = c + temp[1] * ax[1]
+    R[1][2] = temp[1] * ax[2] + s * ax[0]
+    R[2][0] = temp[2] * ax[0] + s * ax[1]
+    R[2][1] = temp[2] * ax[1] - s * ax[0]
+    R[2][2] = c + temp[2] * ax[2]
+    R[3][3] = 1.0
+    return R.ravel()
+
+
+def sky_intensity_for(radiance):
+    """skyIntensity whose toLinear() is ``radiance`` (rmiss:20 applies toLinear after scaling)."""
+    return 1.055 * radiance ** (1.0 / 2.4) - 0.055
+
+
+def make_frame(seed, camera=(0.5, 1.5, 0.5), sun_intensity=3.0, sky_radiance=0.3, layer_begin=0,
+               layer_count=None, config=None, iteration=0, flags=FLAG_DDGI | FLAG_DDGI_MULTI):
+    """Frame constants: sun direction uniform on the upper hemisphere (light travels along
+    -sun), the per-dispatch random rotation drawn from stream ``iteration``."""
+    g = PCG32(seed, [777])
+    z1, phi = g.unit()[0], 2 * math.pi * g.unit()[0]
+    r = math.sqrt(max(0.0, 1 - z1 * z1))
+    sun = (r * math.cos(phi), z1, r * math.sin(phi))
+    gr = PCG32(seed, [9000 + iteration])
+    angle = float(gr.uniform(0.0, 360.0)[0])
+    axis = int(gr.next_u32()[0] % 3)
+    f = Frame()
+    f.camera_pos[:] = (camera[0], camera[1], camera[2], 1.0)
+    f.light_dir[:] = (-sun[0], -sun[1], -sun[2], 0.0)
+    f.sky_color[:] = (1.0, 1.0, 1.0, 1.0)
+    f.sun_intensity = sun_intensity
+    f.sky_intensity = sky_intensity_for(sky_radiance)
+    f.flags = flags
+    f.layer_begin = layer_begin
+    f.layer_count = layer_count if layer_count is not None else (config.probes_y if config else 1)
+    f.rotation[:] = rotation_matrix(angle, axis).tolist()
+    f.bake_tile[:] = (0, 0, 32, 0)
+    return f
+
+
+# ---------------------------------------------------------------------------
+# named workloads
+# ---------------------------------------------------------------------------
+def small_room_config():
+    """8^3 probes at 1 m, 256 rays, hysteresis 0.97 (the CPU-runnable case)."""
+    return Config.make((8, 8, 8), step=(1.0, 1.0, 1.0), rays=256, hysteresis=0.97)
+
+
+def production_config(precision=0):
+    """32 x 16 x 32 probes (step 1, 2, 1 m), 512 rays."""
+    return Config.make((32, 16, 32), step=(1.0, 2.0, 1.0), rays=512, hysteresis=0.96,
+                       precision=precision)
+
+
+def workload(name):
+    """(scene, config, frame) for a named workload."""
+    if name == "room8":
+        cfg = small_room_config()
+        return room_scene(1), cfg, make_frame(1, camera=(0.5, 1.5, 0.5), config=cfg)
+    if name == "terrain1m":
+        cfg = production_config()
+        return terrain_scene(2), cfg, make_frame(2, camera=(0.5, 12.0, 0.5), config=cfg)
+    raise KeyError(name)

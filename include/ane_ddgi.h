@@ -1,0 +1,240 @@
+/*
+ * ane_ddgi.h -- C-ABI of the MI355X (gfx950) DDGI probe-update engine.
+ *
+ * Drop-in for AneRend's probe update (paths relative to the reference checkout):
+ *
+ *   reference pass / call                                     replaced by
+ *   --------------------------------------------------------  -------------------------
+ *   IrradianceProbeTrans exe: 2x vkCmdCopyImage               ane_ddgi_translate
+ *     (src/anerend/render/passes/IrradianceProbeTranslationPass.cpp:56-186)
+ *   IrradianceProbeRT exe: vkCmdTraceRaysKHR                  ane_ddgi_trace
+ *     (IrradianceProbeRayTracingPass.cpp:95-165, shaders irradiance_probe_update.*)
+ *   IrradianceProbeConvolve exe: vkCmdDispatch                ane_ddgi_convolve
+ *     (IrradianceProbeConvolvePass.cpp:109-143, shader probe_conv.comp)
+ *   BLASUpdate / TLASUpdate: vkCmdBuildAccelerationStructures ane_scene_create / ane_scene_refit
+ *     (VulkanRenderer.cpp:1267-1417, UpdateBlasPass.cpp:16-209, UpdateTLASPass.cpp:35-249)
+ *   ProbeRaysConvTex resource (RGBA16F atlas)                 ane_ddgi_atlas_get / _set / _device
+ *     (IrradianceProbeConvolvePass.cpp:82-99)
+ *
+ * Conventions
+ *   - Every function returns an ane_status; nothing throws across the ABI.
+ *     The message of the last failure is ane_last_error() (thread-local).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).  All
+ *     device work of a call is ordered on it; host-buffer calls synchronize it.
+ *   - Host arrays are copied during the call; no pointer is retained.
+ *   - One ane_ddgi_ctx per probe grid; a context is not thread-safe (the
+ *     reference's pass objects are not either).  Scenes are read-only during
+ *     traces and may be shared by several contexts.
+ *
+ * Data layouts
+ *   vertices   [nvert][24] words, the 96-byte PackedVertex (bindless.glsl:60-68)
+ *   atlas      [10*probes_z*probes_y][10*probes_x][4] FP32 RGBA, the reference's
+ *              ProbeRaysConvTex texel layout (probe (x,y,z) interior at
+ *              (10x+1, 10*probes_z*y + 10z + 1), 1-texel borders)
+ *   probes     p = (y*probes_z + z)*probes_x + x  (layer-major)
+ *   radiance   [probe][3][ray] (channel-planar per probe)
+ */
+#ifndef ANE_DDGI_H
+#define ANE_DDGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANE_DDGI_ABI_VERSION 1
+
+typedef enum ane_status {
+  ANE_OK = 0,
+  ANE_ERR_INVALID_ARG = 1,
+  ANE_ERR_OUT_OF_MEMORY = 2,
+  ANE_ERR_ADMISSION_REFUSED = 3,
+  ANE_ERR_DEVICE = 4,
+  ANE_ERR_NOT_READY = 5,
+  ANE_ERR_UNSUPPORTED = 6,
+  ANE_ERR_CANCELLED = 7
+} ane_status;
+
+/* scene_ubo.glsl:1-12 flag bits honoured by the probe update */
+#define ANE_FLAG_DDGI (1u << 4)
+#define ANE_FLAG_DDGI_MULTI (1u << 5) /* indirect light from the previous atlas (rchit:98) */
+#define ANE_FLAG_BAKE (1u << 11)      /* bake placement + accumulate blend (rgen:61, probe_conv.comp:67) */
+
+typedef enum ane_precision {
+  ANE_PRECISION_FP32 = 0, /* plain path: FP32 radiance, FP32 convolution */
+  ANE_PRECISION_BF16 = 1, /* BF16 radiance and weights, MFMA, FP32 accumulate */
+  ANE_PRECISION_FP8 = 2   /* FP8-e4m3 radiance (per-probe scale) and weights, MFMA, FP32 accumulate */
+} ane_precision;
+
+/* Probe grid constants: probe_helpers.glsl:1-7 (compile-time in the reference),
+ * probe_conv.comp:85 (hysteresis), irradiance_probe_update.rgen:91-92 (t range).
+ * The octahedral tile is fixed at 8x8 texels + 1-texel border. */
+typedef struct ane_ddgi_config {
+  int32_t probes_x, probes_y, probes_z; /* NUM_PROBES_PER_PLANE (x, z), NUM_PROBE_PLANES (y) */
+  float probe_step[3];                  /* PROBE_STEP */
+  int32_t rays_per_probe;               /* NUM_RAYS_PER_PROBE, 1..4096 (need not be square) */
+  float hysteresis;                     /* 0.96 in the reference */
+  float t_min, t_max;                   /* 0.1, 50.0 in the reference */
+  int32_t precision;                    /* ane_precision */
+  uint32_t quirks;                      /* reserved, must be 0 */
+} ane_ddgi_config;
+
+/* Per-update constants: the GPUSceneData fields the path reads
+ * (GpuBuffers.h:125-148, filled at VulkanRenderer.cpp:1201-1243) and the
+ * ray-generation push constants (IrradianceProbeRayTracingPass.cpp:14-17). */
+typedef struct ane_ddgi_frame {
+  float camera_pos[4];
+  float light_dir[4]; /* direction the sun light travels; the sun is at -light_dir */
+  float sky_color[4];
+  float sun_intensity;
+  float sky_intensity;
+  uint32_t flags;      /* ANE_FLAG_* */
+  int32_t layer_begin; /* first probe layer (y) to update (push constant probeLayer) */
+  int32_t layer_count; /* number of layers; probes_y = full update */
+  float rotation[16];  /* randomRot, column-major mat4; the upper 3x3 rotates ray directions */
+  int32_t bake_tile[4];/* bakeTileInfo: tile x, tile z, tile size (bake mode) */
+} ane_ddgi_frame;
+
+typedef struct ane_mesh { /* GPUMeshInfo (GpuBuffers.h:38-44), index count in place of blasRef */
+  float min_pos[4];
+  float max_pos[4];
+  uint32_t vertex_offset;
+  uint32_t index_offset;
+  uint32_t index_count;
+  uint32_t reserved;
+} ane_mesh;
+
+typedef struct ane_material { /* GPUMaterialInfo (GpuBuffers.h:29-36); textures unsupported (-1) */
+  float base_col_fac[4];
+  float emissive[4];
+  int32_t bindless_indices[4];
+  float met_rough[4];
+} ane_material;
+
+typedef struct ane_instance { /* VkAccelerationStructureInstanceKHR as written by tlas_update.comp:94-97 */
+  float transform[12];            /* 3x4 row-major object-to-world */
+  uint32_t custom_index_and_mask; /* material bits 0-9, mesh bits 10-23, mask bits 24-31 */
+  uint32_t sbt_offset_and_flags;
+  uint64_t reserved;
+} ane_instance;
+
+typedef struct ane_hit {
+  int32_t tri;   /* global triangle id (instance-major input order), -1 = miss */
+  float t, u, v; /* hit distance; barycentric weights of vertices 1 and 2 */
+} ane_hit;
+
+typedef struct ane_scene_info {
+  int64_t num_triangles;
+  int32_t num_instances;
+  int32_t bvh_depth;      /* max root-to-leaf edge count of the LBVH */
+  int64_t device_bytes;   /* HBM held by the scene */
+  float build_ms;         /* last build or refit, device time */
+} ane_scene_info;
+
+typedef struct ane_ddgi_stats {
+  /* device time per stage from hipEvent pairs, ms: the last call and the mean
+   * over all calls since the last ane_ddgi_stats_reset.  Stage order:
+   * setup, trace (primary rays), shadow, shade, convolve, translate. */
+  float last_ms[6];
+  float mean_ms[6];
+  uint64_t calls[6];
+  /* traversal work of the last trace (only while counting is enabled) */
+  uint64_t primary_nodes, primary_tris, shadow_nodes, shadow_tris;
+  uint64_t rays;          /* primary rays of the last trace */
+  int64_t device_bytes;   /* HBM held by the context */
+} ane_ddgi_stats;
+
+typedef struct ane_ddgi_ctx ane_ddgi_ctx;
+typedef struct ane_scene ane_scene;
+
+/* ---- library ---- */
+int ane_abi_version(void);
+const char* ane_last_error(void);
+const char* ane_status_string(int status);
+/* Number of the HIP device used by this thread (hipGetDevice). */
+int ane_device(int32_t* device);
+/* Wait for all work queued on `stream` (hipStreamSynchronize). */
+int ane_stream_synchronize(void* stream);
+
+/* ---- scene: replaces BLAS/TLAS build (VulkanRenderer.cpp:1267-1417, UpdateTLASPass.cpp:184-228) ----
+ * Uploads the geometry, flattens every instance to world-space triangles and
+ * builds the LBVH on the GPU (Morton codes, radix sort, Karras hierarchy,
+ * bottom-up boxes). */
+int ane_scene_create(const float* vertices, int64_t num_vertices, const uint32_t* indices,
+                     int64_t num_indices, const ane_mesh* meshes, int32_t num_meshes,
+                     const ane_material* materials, int32_t num_materials,
+                     const ane_instance* instances, int32_t num_instances, void* stream,
+                     ane_scene** out);
+/* New instance transforms, same topology (UpdateBlasPass.cpp:131-138 MODE_UPDATE).
+ * Instances must keep their mesh; `instances` is a host array. */
+int ane_scene_refit(ane_scene* scene, const ane_instance* instances, int32_t num_instances,
+                    void* stream);
+/* Full rebuild (sort + hierarchy) after new transforms (UpdateTLASPass.cpp:184-228 BUILD). */
+int ane_scene_rebuild(ane_scene* scene, const ane_instance* instances, int32_t num_instances,
+                      void* stream);
+int ane_scene_info_get(const ane_scene* scene, ane_scene_info* out);
+int ane_scene_destroy(ane_scene* scene);
+
+/* ---- probe grid context ---- */
+int ane_ddgi_create(const ane_ddgi_config* config, ane_ddgi_ctx** out);
+int ane_ddgi_destroy(ane_ddgi_ctx* ctx);
+
+/* IrradianceProbeTrans (IrradianceProbeTranslationPass.cpp:56-186): if the camera
+ * cell (trunc(cam.x / step.x), trunc(cam.z / step.z)) differs from the last one
+ * the atlas tiles shift by one probe toward the motion (entry edge keeps its old
+ * texels).  *shifted (nullable) receives 1 if the atlas moved.  The last cell
+ * starts at (0, 0) like _lastProbeIdx. */
+int ane_ddgi_translate(ane_ddgi_ctx* ctx, const float* camera_pos3, int32_t* shifted,
+                       void* stream);
+/* IrradianceProbeRT: trace probe rays of layers [layer_begin, +layer_count),
+ * shade hits (shadow ray, direct + indirect + emissive) and store radiance. */
+int ane_ddgi_trace(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_frame* frame,
+                   void* stream);
+/* IrradianceProbeConvolve: cosine-weighted octahedral convolution of the stored
+ * radiance and hysteresis blend (or bake accumulation) into the atlas. */
+int ane_ddgi_convolve(ane_ddgi_ctx* ctx, const ane_ddgi_frame* frame, void* stream);
+/* trace then convolve. */
+int ane_ddgi_update(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_frame* frame,
+                    void* stream);
+
+/* ---- probe state (ProbeRaysConvTex) ---- */
+int ane_ddgi_atlas_get(ane_ddgi_ctx* ctx, float* host_rgba, void* stream);
+int ane_ddgi_atlas_set(ane_ddgi_ctx* ctx, const float* host_rgba, void* stream);
+int ane_ddgi_atlas_device(ane_ddgi_ctx* ctx, float** device_rgba, int64_t* width,
+                          int64_t* height);
+int ane_ddgi_translation_state(ane_ddgi_ctx* ctx, int32_t* last_cell2, int32_t set);
+
+/* ---- radiance (ProbeRaysIrrTex) in the context's precision ----
+ * set: host FP32 [probe][3][ray] for the layers of `frame`, quantized on upload. */
+int ane_ddgi_radiance_set(ane_ddgi_ctx* ctx, const ane_ddgi_frame* frame, const float* host,
+                          void* stream);
+/* get: FP32 [probe][3][ray] (dequantized) for the layers of the last trace. */
+int ane_ddgi_radiance_get(ane_ddgi_ctx* ctx, float* host, void* stream);
+
+/* ---- parity / diagnostics ---- */
+/* Per-ray hits [probe][ray] and shadow flags (0 lit, 1 shadowed, 255 miss) of the last trace. */
+int ane_ddgi_debug_rays(ane_ddgi_ctx* ctx, ane_hit* hits, uint8_t* shadowed, void* stream);
+/* Count traversal work (nodes/triangles) in the next traces (slower kernels). */
+int ane_ddgi_set_counting(ane_ddgi_ctx* ctx, int32_t enable);
+int ane_ddgi_stats_get(const ane_ddgi_ctx* ctx, ane_ddgi_stats* out);
+int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
+/* LBVH export in sorted order: keys, triangle ids, internal children (>= 0
+ * internal, < 0 leaf ~position), parents, internal boxes [n-1][6] (lo, hi). */
+int ane_scene_debug_bvh(const ane_scene* scene, uint64_t* keys, uint32_t* ids, int32_t* left,
+                        int32_t* right, int32_t* parent, float* boxes);
+int ane_scene_debug_world_tris(const ane_scene* scene, float* tris9);
+/* Trace arbitrary rays (host arrays) against a scene: closest or any hit. */
+int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const float* dirs,
+                         int64_t num_rays, float t_min, float t_max, int32_t any_hit,
+                         ane_hit* hits, uint64_t* counters2, void* stream);
+
+/* ---- host-only helpers (no device work) ---- */
+/* Fibonacci-lattice ray directions rotated by mat3(rot16) (rgen:35-46,86-88). */
+int ane_ddgi_ray_directions(int32_t num_rays, const float* rot16, float* out_xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANE_DDGI_H */

@@ -1,0 +1,71 @@
+"""GPU parity on the small room workload (8^3 probes x 256 rays, ~780 triangles,
+4 updates): hit ids, t/u/v, shadow decisions, traversal counters and BVH
+topology bit-exact with the oracle; radiance and atlas within 1e-4 relative."""
+import numpy as np
+import pytest
+
+from anerend_amd import native, scenegen
+from oracle import oracle as O
+from tests.conftest import bits, rel_err
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4  # FP32 radiance / irradiance tolerance (relative, floor 1e-3)
+
+
+@pytest.fixture(scope="module")
+def room():
+    scene, cfg, _ = scenegen.workload("room8")
+    return scene, cfg, native.DeviceScene(scene), O.OracleScene(scene)
+
+
+def test_bvh_topology_bit_exact(room):
+    scene, cfg, ds, osc = room
+    g, o = ds.bvh(), osc.bvh()
+    for k in ("keys", "ids", "left", "right", "parent"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["boxes"], o["boxes"])
+    assert ds.info()["bvh_depth"] == osc.max_depth
+    assert np.array_equal(bits(ds.world_tris()), bits(osc.world_tris()))
+
+
+def test_trace_rays_bit_exact(room):
+    scene, cfg, ds, osc = room
+    rng = np.random.default_rng(7)
+    n = 20000
+    org = rng.uniform([-4.5, -0.4, -4.5], [4.5, 3.4, 4.5], (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    for any_hit in (False, True):
+        hg, cg = ds.trace_rays(org, d, any_hit=any_hit, count=True)
+        ho, co = osc.trace(org, d, any_hit=any_hit)
+        assert np.array_equal(hg["tri"], ho["tri"])
+        for f in ("t", "u", "v"):
+            assert np.array_equal(bits(hg[f]), bits(ho[f])), f
+        assert np.array_equal(cg, co)
+
+
+def test_room_updates_parity(room):
+    scene, cfg, ds, osc = room
+    grid = native.ProbeGrid(cfg)
+    grid.set_counting(True)
+    atlas = O.new_atlas(cfg)
+    for it in range(4):
+        fr = scenegen.make_frame(1, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        print(f"room update {it}", flush=True)
+        ho, sho, rado, ctro = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), f"hit ids differ at update {it}"
+        for f in ("t", "u", "v"):
+            assert np.array_equal(bits(hg[f]), bits(ho[f])), f
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        st = grid.stats()
+        assert [st["primary_nodes"], st["primary_tris"], st["shadow_nodes"], st["shadow_tris"]] == \
+            [int(x) for x in ctro]
+        radg = grid.radiance(fr)
+        e_rad = rel_err(radg, rado)
+        e_atl = rel_err(grid.atlas, atlas)
+        print(f"  radiance rel err {e_rad:.3e}  atlas rel err {e_atl:.3e}", flush=True)
+        assert e_rad <= RTOL
+        assert e_atl <= RTOL
