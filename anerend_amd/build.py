@@ -28,10 +28,11 @@ def hipcc():
     return "hipcc"
 
 
-def build(quiet=False):
+def build(quiet=False, out=None, defines=()):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-o", tmp] + srcs
+    lib = out or LIB
+    tmp = lib + ".tmp"
+    cmd = [hipcc()] + FLAGS + [f"-D{d}" for d in defines] + ["-o", tmp] + srcs
     res = subprocess.run(cmd, capture_output=True, text=True)
     out = res.stdout + res.stderr
     # the host pass does not know the packed-fp32 feature; that warning is expected
@@ -41,8 +42,8 @@ def build(quiet=False):
         raise RuntimeError("hipcc failed building libane_ddgi.so")
     if lines and not quiet:
         sys.stderr.write("\n".join(lines) + "\n")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

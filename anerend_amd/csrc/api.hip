@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -98,6 +100,28 @@ static void fibonacci_dirs(int n, float* out)
     out[3 * i + 1] = (float)(y / l);
     out[3 * i + 2] = (float)(z / l);
   }
+}
+
+// Processing order of the N ray indices: sorted by the 2-D Morton code of the
+// octahedral encoding of their (unrotated) direction, so that the 64 lanes of a
+// wave trace a compact cone of directions from one probe.  Every probe shares
+// the directions and a rotation preserves their neighbourhoods.  Only the
+// order in which rays are processed changes; results are per ray.
+static void ray_order(int n, const float* dirs, std::vector<uint32_t>* out)
+{
+  std::vector<std::pair<uint32_t, uint32_t>> k((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    float ox, oy;
+    oct_encode(ld3(dirs + 3 * i), ox, oy);
+    uint32_t qx = (uint32_t)fminf(fmaxf((ox + 1.0f) * 512.0f, 0.0f), 1023.0f);
+    uint32_t qy = (uint32_t)fminf(fmaxf((oy + 1.0f) * 512.0f, 0.0f), 1023.0f);
+    uint32_t m = 0;
+    for (int b = 0; b < 10; ++b) m |= (((qx >> b) & 1u) << (2 * b + 1)) | (((qy >> b) & 1u) << (2 * b));
+    k[i] = {m, (uint32_t)i};
+  }
+  std::sort(k.begin(), k.end());
+  out->resize((size_t)n);
+  for (int i = 0; i < n; ++i) (*out)[i] = k[i].second;
 }
 
 static float to_linear_host(float c)
@@ -213,6 +237,9 @@ struct ane_scene {
   uint32_t* d_cbounds = nullptr;
   uint32_t* d_depth = nullptr;
   uint32_t* d_scratch = nullptr;
+  int32_t* d_order = nullptr;   // internal nodes bucketed by depth
+  uint32_t* d_hist = nullptr;
+  std::vector<int32_t> level_begin;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t depth = 0;
   float build_ms = 0.0f;
@@ -227,6 +254,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_depth); dfree(s->d_scratch);
+  dfree(s->d_order); dfree(s->d_hist);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
 }
@@ -270,7 +298,9 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
   launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  launch_fit(s->d_tris, s->d_leaf_parent, n, s->d_nodes, s->d_arrive, s->d_node_box, st);
+  if (launch_levels(s->d_nodes, n, s->d_arrive, s->d_hist, s->d_order, &s->level_begin, st) != 0)
+    return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
   launch_depth(s->d_leaf_parent, s->d_nodes, n, s->d_depth, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
@@ -290,7 +320,7 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
   launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_cbounds, false, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  launch_fit(s->d_tris, s->d_leaf_parent, n, s->d_nodes, s->d_arrive, s->d_node_box, st);
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
   ANE_CHECK_HIP(hipEventSynchronize(s->ev1));
@@ -367,6 +397,8 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_cbounds, 6);
   ANE_A(d_depth, 1);
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
+  ANE_A(d_order, n);
+  ANE_A(d_hist, 128);
 #undef ANE_A
   auto bail = [&](int code) { scene_free(s); delete s; return code; };
   hipError_t e = hipSuccess;
@@ -603,8 +635,8 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   }
   std::vector<float> dirs((size_t)c->N * 3);
   fibonacci_dirs((int)c->N, dirs.data());
-  std::vector<uint32_t> order((size_t)c->N);
-  for (int64_t i = 0; i < c->N; ++i) order[i] = (uint32_t)i;
+  std::vector<uint32_t> order;
+  ray_order((int)c->N, dirs.data(), &order);
   if ((e = hipMemcpy(c->d_atlas[0], clear.data(), clear.size() * 4, hipMemcpyHostToDevice)) ||
       (e = hipMemcpy(c->d_atlas[1], clear.data(), clear.size() * 4, hipMemcpyHostToDevice)) ||
       (e = hipMemcpy(c->d_base_dirs, dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice)) ||

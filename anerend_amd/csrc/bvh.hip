@@ -13,8 +13,12 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <vector>
+
 namespace ane {
 
+static constexpr int kMaxLevels = 128;  // > the 96-level bound of a 63-bit-key LBVH
+static constexpr int kTopLevels = 10;   // depth < 10 holds at most 512 nodes per level
 static constexpr int kBlock = 256;
 static constexpr int kItems = 8;
 static constexpr int kTile = kBlock * kItems;
@@ -395,35 +399,90 @@ __device__ __forceinline__ void write_child_box(NodeBuild* nd, int slot, const f
   }
 }
 
-__global__ __launch_bounds__(256) void k_fit(const float4* tris, const int32_t* leaf_parent, int n,
-                                             NodeBuild* nodes, uint32_t* arrive, float* node_box)
+// Box of one child: a leaf's triangle box or an internal node's stored box.
+__device__ __forceinline__ void child_box(const float4* tris, const float* node_box, int c, float* lo,
+                                          float* hi)
 {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  float4 v0 = tris[3 * (int64_t)p], v1 = tris[3 * (int64_t)p + 1], v2 = tris[3 * (int64_t)p + 2];
-  float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y),
-                 fminf(fminf(v0.z, v1.z), v2.z)};
-  float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y),
-                 fmaxf(fmaxf(v0.z, v1.z), v2.z)};
-  int child = ~p;
-  int node = leaf_parent[p];
-  for (int level = 0; node >= 0 && node < n - 1 && level < 128; ++level) {
-    NodeBuild* nd = &nodes[node];
-    int slot = (nd->d.x == child) ? 0 : 1;
-    write_child_box(nd, slot, lo, hi);
-    __threadfence();
-    uint32_t old = atomicAdd(&arrive[node], 1u);
-    if (old == 0) return;
-    __threadfence();
-    volatile NodeBuild* vn = nd;
-    float l0x = vn->a.x, h0x = vn->a.y, l0y = vn->a.z, h0y = vn->a.w, l0z = vn->c.x, h0z = vn->c.y;
-    float l1x = vn->b.x, h1x = vn->b.y, l1y = vn->b.z, h1y = vn->b.w, l1z = vn->c.z, h1z = vn->c.w;
-    lo[0] = fminf(l0x, l1x); lo[1] = fminf(l0y, l1y); lo[2] = fminf(l0z, l1z);
-    hi[0] = fmaxf(h0x, h1x); hi[1] = fmaxf(h0y, h1y); hi[2] = fmaxf(h0z, h1z);
-    float* nb = node_box + 6 * (int64_t)node;
-    nb[0] = lo[0]; nb[1] = lo[1]; nb[2] = lo[2]; nb[3] = hi[0]; nb[4] = hi[1]; nb[5] = hi[2];
-    child = node;
-    node = vn->d.z;
+  if (c < 0) {
+    const int64_t p = ~c;
+    float4 v0 = tris[3 * p], v1 = tris[3 * p + 1], v2 = tris[3 * p + 2];
+    lo[0] = fminf(fminf(v0.x, v1.x), v2.x); lo[1] = fminf(fminf(v0.y, v1.y), v2.y);
+    lo[2] = fminf(fminf(v0.z, v1.z), v2.z);
+    hi[0] = fmaxf(fmaxf(v0.x, v1.x), v2.x); hi[1] = fmaxf(fmaxf(v0.y, v1.y), v2.y);
+    hi[2] = fmaxf(fmaxf(v0.z, v1.z), v2.z);
+  } else {
+    const float* b = node_box + 6 * (int64_t)c;
+    lo[0] = b[0]; lo[1] = b[1]; lo[2] = b[2]; hi[0] = b[3]; hi[1] = b[4]; hi[2] = b[5];
+  }
+}
+
+// Fit one internal node from its (already final) children: child boxes into
+// the traversal record, the union into node_box.
+__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, int i)
+{
+  NodeBuild* nd = &nodes[i];
+  const int L = nd->d.x, R = nd->d.y;
+  float l0[3], h0[3], l1[3], h1[3];
+  child_box(tris, node_box, L, l0, h0);
+  child_box(tris, node_box, R, l1, h1);
+  nd->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
+  nd->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
+  nd->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+  float* nb = node_box + 6 * (int64_t)i;
+  nb[0] = fminf(l0[0], l1[0]); nb[1] = fminf(l0[1], l1[1]); nb[2] = fminf(l0[2], l1[2]);
+  nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
+}
+
+// Depth (root = 0) of every internal node and the per-depth histogram.
+__global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int n, uint32_t* ndepth,
+                                                    uint32_t* hist)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint32_t d = 0;
+  int node = nodes[i].d.z;
+  while (node >= 0 && node < n - 1 && d < (uint32_t)kMaxLevels - 1) {
+    ++d;
+    node = nodes[node].d.z;
+  }
+  ndepth[i] = d;
+  atomicAdd(&hist[d], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_bucket(const uint32_t* ndepth, int n, uint32_t* cursor,
+                                                int32_t* order)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint32_t slot = atomicAdd(&cursor[ndepth[i]], 1u);
+  if (slot < (uint32_t)(n - 1)) order[slot] = i;
+}
+
+// One depth level: every child is a leaf or belongs to a deeper, finished level.
+__global__ __launch_bounds__(256) void k_fit_level(const float4* tris, NodeBuild* nodes,
+                                                   float* node_box, const int32_t* order, int begin,
+                                                   int count)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  fit_node(tris, nodes, node_box, order[begin + k]);
+}
+
+// The top levels (at most 2^d nodes at depth d) in one workgroup, deepest first.
+struct TopLevels {
+  int32_t begin[kTopLevels];
+  int32_t count[kTopLevels];
+  int32_t num;
+};
+
+__global__ __launch_bounds__(1024) void k_fit_top(const float4* tris, NodeBuild* nodes,
+                                                  float* node_box, const int32_t* order, TopLevels t)
+{
+  for (int l = t.num - 1; l >= 0; --l) {
+    for (int k = threadIdx.x; k < t.count[l]; k += blockDim.x)
+      fit_node(tris, nodes, node_box, order[t.begin[l] + k]);
+    __threadfence_block();
+    __syncthreads();
   }
 }
 
@@ -476,13 +535,52 @@ void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, ids, n, tris);
 }
 
-void launch_fit(const float4* tris, const int32_t* leaf_parent, int n, NodeBuild* nodes,
-                uint32_t* arrive, float* node_box, hipStream_t s)
+int launch_levels(const NodeBuild* nodes, int n, uint32_t* ndepth, uint32_t* hist,
+                  int32_t* order, std::vector<int32_t>* level_begin, hipStream_t s)
 {
-  if (n < 2) return;
-  fill_u32(arrive, 0u, n - 1, s);
-  hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, tris, leaf_parent, n, nodes,
-                     arrive, node_box);
+  level_begin->clear();
+  if (n < 2) return 0;
+  fill_u32(hist, 0u, kMaxLevels, s);
+  hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, nodes, n, ndepth,
+                     hist);
+  uint32_t h[kMaxLevels];
+  if (hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  int levels = 0;
+  for (int d = 0; d < kMaxLevels; ++d)
+    if (h[d]) levels = d + 1;
+  uint32_t off[kMaxLevels];
+  uint32_t acc = 0;
+  for (int d = 0; d < kMaxLevels; ++d) {
+    off[d] = acc;
+    if (d < levels) level_begin->push_back((int32_t)acc);
+    acc += h[d];
+  }
+  level_begin->push_back((int32_t)acc);
+  if (hipMemcpyAsync(hist, off, sizeof(off), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+  hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ndepth, n, hist, order);
+  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  return 0;
+}
+
+void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
+                const std::vector<int32_t>& level_begin, hipStream_t s)
+{
+  if (n < 2 || level_begin.size() < 2) return;
+  const int levels = (int)level_begin.size() - 1;
+  for (int d = levels - 1; d >= kTopLevels; --d) {
+    const int cnt = level_begin[d + 1] - level_begin[d];
+    if (cnt > 0)
+      hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, tris, nodes,
+                         node_box, order, level_begin[d], cnt);
+  }
+  TopLevels t{};
+  t.num = levels < kTopLevels ? levels : kTopLevels;
+  for (int d = 0; d < t.num; ++d) {
+    t.begin[d] = level_begin[d];
+    t.count[d] = level_begin[d + 1] - level_begin[d];
+  }
+  hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, tris, nodes, node_box, order, t);
 }
 
 void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,

@@ -1,6 +1,8 @@
 // kernels.hpp -- host-side launchers of the gfx950 kernels (internal).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace ane {
@@ -19,8 +21,13 @@ void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t*
 void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
                    hipStream_t s);
 void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s);
-void launch_fit(const float4* tris, const int32_t* leaf_parent, int n, NodeBuild* nodes,
-                uint32_t* arrive, float* node_box, hipStream_t s);
+// Buckets internal nodes by depth (syncs the stream once); level_begin gets
+// the offset of every depth in `order` plus a final end offset.
+int launch_levels(const NodeBuild* nodes, int n, uint32_t* ndepth, uint32_t* hist, int32_t* order,
+                  std::vector<int32_t>* level_begin, hipStream_t s);
+// Level-synchronous bottom-up boxes (no atomics, no inter-workgroup fences).
+void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
+                const std::vector<int32_t>& level_begin, hipStream_t s);
 void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,
                   hipStream_t s);
 

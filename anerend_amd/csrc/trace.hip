@@ -16,10 +16,22 @@
 
 namespace ane {
 
-static constexpr int kTB = 128;     // threads per traversal block (2 waves)
-static constexpr int kSS = 16;      // LDS stack entries per lane (ring)
-static constexpr int kSteps = 32;   // traversal steps between refill checks
-static constexpr int kRefill = 16;  // idle lanes that trigger a refill
+#ifndef ANE_TRACE_BLOCK
+#define ANE_TRACE_BLOCK 128
+#endif
+#ifndef ANE_TRACE_STACK
+#define ANE_TRACE_STACK 16
+#endif
+#ifndef ANE_TRACE_STEPS
+#define ANE_TRACE_STEPS 64
+#endif
+#ifndef ANE_TRACE_REFILL
+#define ANE_TRACE_REFILL 16
+#endif
+static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
+static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring)
+static constexpr int kSteps = ANE_TRACE_STEPS;   // traversal steps between refill checks
+static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 
 int trace_stack_lds() { return kSS; }
 

@@ -11,7 +11,7 @@ import numpy as np
 from .types import HIT_DTYPE, Config, Frame, ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libane_ddgi.so")
+LIB_PATH = os.environ.get("ANE_DDGI_LIB") or os.path.join(HERE, "libane_ddgi.so")
 
 ANE_OK = 0
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "admission refused",

@@ -176,7 +176,7 @@ def terrain_mesh(seed, cells, spacing=1.0, amplitude=8.0):
     nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
     pos = np.stack([X, H, Z], -1).reshape(-1, 3).astype(np.float32)
     # sRGB vertex colour: grass to rock by height
-    t = np.clip((H.ravel() + amplitude) / (2 * amplitude), 0.0, 1.0)[:, None]
+    t = np.clip((H.ravel() + amplitude) / (2 * max(amplitude, 1e-9)), 0.0, 1.0)[:, None]
     col = ((1 - t) * np.array([0.25, 0.45, 0.15]) + t * np.array([0.55, 0.5, 0.45])).astype(np.float32)
     uv = np.stack([(X.ravel() + half) / (2 * half), (Z.ravel() + half) / (2 * half)], -1).astype(np.float32)
     i = np.arange(cells)

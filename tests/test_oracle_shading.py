@@ -1,0 +1,145 @@
+"""Oracle pins for the hit shading (rchit:24-110, pbr_light.glsl:10-388) by
+closed forms: N = L = V direct light, constant-atlas indirect light
+(E = c * pi / 2), shadowed and sun-below-horizon cases, emissive, sky."""
+import math
+
+import numpy as np
+import pytest
+
+from anerend_amd import scenegen
+from anerend_amd.types import FLAG_DDGI, FLAG_DDGI_MULTI, HIT_DTYPE, INSTANCE_DTYPE, MATERIAL_DTYPE, Config
+from oracle import oracle as O
+
+
+def quad_scene(albedo, rough, metal, emissive=(0, 0, 0, 0), occluder=False):
+    """A 40 x 40 m quad facing +y at y = 0 (+ optional occluder quad at y = 2)."""
+    mb = scenegen._MeshBuilder()
+    p = np.array([[-20, 0, -20], [20, 0, -20], [20, 0, 20], [-20, 0, 20]], np.float32)
+    n = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
+    m = mb.add(p, n, np.array([0, 2, 1, 0, 3, 2], np.uint32))
+    mats = np.zeros(2, MATERIAL_DTYPE)
+    mats["bindless_indices"] = -1
+    mats["base_col_fac"][0] = (*albedo, 1.0)
+    mats["met_rough"][0] = (metal, rough, 0, 0)
+    mats["emissive"][0] = emissive
+    mats["base_col_fac"][1] = (0.5, 0.5, 0.5, 1.0)
+    mats["met_rough"][1] = (0, 1, 0, 0)
+    inst = [scenegen.make_instance(m, 0, np.eye(3), (1, 1, 1), (0, 0, 0))]
+    if occluder:
+        inst.append(scenegen.make_instance(m, 1, np.eye(3), (0.05, 1, 0.05), (0, 2, 0)))
+    v, i, meshes = mb.arrays()
+    return scenegen.Scene(v, i, meshes, mats, np.array(inst, INSTANCE_DTYPE))
+
+
+def hit_at(osc, origin, x, z):
+    o = np.array([[x, origin[1], z]], np.float32)
+    d = np.array([[0, -1, 0]], np.float32)
+    h, _ = osc.trace(o, d)
+    assert h["tri"][0] >= 0
+    return h[0]
+
+
+CFG = Config.make((8, 8, 8), step=(1, 1, 1), rays=64)
+
+
+def frame(light_dir, sun=3.0, flags=FLAG_DDGI):
+    f = scenegen.make_frame(1, camera=(0.5, 1.5, 0.5), config=CFG, flags=flags, sun_intensity=sun)
+    f.light_dir[:3] = light_dir
+    return f
+
+
+def direct_closed_form(albedo, rough, metal, sun):
+    """Cook-Torrance with N = L = V (so N.H = N.V = N.L = H.V = 1), float64."""
+    a2 = (rough * rough) ** 2
+    D = (a2 + 0.001) / (math.pi * a2 * a2)
+    G = 1.0
+    F0 = np.array([0.04 * (1 - metal) + c * metal for c in albedo])
+    F = F0
+    spec = np.clip(D * G * F / (4.0 + 1e-4), 0.0, 1.0)
+    kD = (1 - F) * (1 - metal)
+    return (kD * np.array(albedo) / math.pi + spec) * sun
+
+
+@pytest.mark.parametrize("albedo,rough,metal", [((0.8, 0.5, 0.2), 0.6, 0.0),
+                                                ((0.3, 0.7, 0.9), 0.9, 1.0),
+                                                ((0.5, 0.5, 0.5), 0.35, 0.0)])
+def test_direct_light_n_eq_l_eq_v(albedo, rough, metal):
+    osc = O.OracleScene(quad_scene(albedo, rough, metal))
+    origin = np.array([1.25, 3.0, -2.5], np.float32)
+    h = hit_at(osc, origin, origin[0], origin[2])
+    atlas = O.new_atlas(CFG)
+    out, shadowed = osc.shade_hit(CFG, frame((0, -1, 0)), atlas, origin, h)
+    assert shadowed == 0
+    assert np.allclose(out, direct_closed_form(albedo, rough, metal, 3.0), rtol=2e-5)
+
+
+def test_indirect_constant_atlas():
+    albedo, rough, metal = (0.6, 0.4, 0.3), 0.5, 0.0
+    osc = O.OracleScene(quad_scene(albedo, rough, metal))
+    origin = np.array([0.3, 2.0, 0.7], np.float32)
+    h = hit_at(osc, origin, origin[0], origin[2])
+    c = np.array([0.7, 0.2, 1.3], np.float32)
+    atlas = O.new_atlas(CFG)
+    atlas[..., :3] = c
+    # sun below the horizon: N.L = 0 so only the indirect term remains
+    out, _ = osc.shade_hit(CFG, frame((0, 1, 0), flags=FLAG_DDGI | FLAG_DDGI_MULTI), atlas, origin, h)
+    F0 = 0.04
+    kd = (1 - F0) * (1 - metal)      # N.V = 1 -> Fresnel-roughness term is F0
+    expect = kd * (c.astype(np.float64) * 0.5 * 3.1415926) * np.array(albedo)
+    assert np.allclose(out, expect, rtol=2e-5)
+    # multi-bounce off: nothing
+    out0, _ = osc.shade_hit(CFG, frame((0, 1, 0)), atlas, origin, h)
+    assert np.all(out0 == 0)
+
+
+def test_sample_probe_constant_and_repeat_wrap():
+    cfg = Config.make((4, 3, 4), step=(1, 2, 1), rays=64)
+    fr = scenegen.make_frame(1, camera=(0.5, 0, 0.5), config=cfg)
+    atlas = O.new_atlas(cfg)
+    atlas[..., :3] = 0.25
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        p = rng.uniform([-2, 0, -2], [2, 5, 2]).astype(np.float32)
+        n = rng.normal(size=3)
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        assert np.allclose(O.sample_probe(cfg, fr, atlas, p, n), 0.25 * 0.5 * 3.1415926, rtol=1e-5)
+    # top layer: the +1 corner probes wrap to layer 0 (REPEAT sampler, Q2).  Give
+    # layer 0 a distinct value: a point at the top layer sees it, one below does not.
+    atlas[..., :3] = 0.25
+    atlas[: 10 * cfg.probes_z, :, :3] = 4.0
+    top = O.sample_probe(cfg, fr, atlas, np.array([0.2, 4.6, 0.3], np.float32), np.array([0, 1, 0], np.float32))
+    mid = O.sample_probe(cfg, fr, atlas, np.array([0.2, 2.6, 0.3], np.float32), np.array([0, 1, 0], np.float32))
+    assert top[0] > mid[0] * 1.5
+    assert np.allclose(mid, 0.25 * 0.5 * 3.1415926, rtol=1e-5)
+
+
+def test_shadowed_and_sun_below():
+    albedo = (0.5, 0.5, 0.5)
+    osc = O.OracleScene(quad_scene(albedo, 0.5, 0.0, occluder=True))
+    atlas = O.new_atlas(CFG)
+    origin = np.array([0.0, 1.0, 0.0], np.float32)
+    # under the occluder (y = 2, 2 m wide): shadow ray hits it
+    h = hit_at(osc, origin, 0.1, 0.1)
+    out, sh = osc.shade_hit(CFG, frame((0, -1, 0)), atlas, origin, h)
+    assert sh == 1 and np.all(out == 0)
+    # away from it: lit
+    o2 = np.array([5.0, 1.0, 5.0], np.float32)
+    h2 = hit_at(osc, o2, 5.0, 5.0)
+    out, sh = osc.shade_hit(CFG, frame((0, -1, 0)), atlas, o2, h2)
+    assert sh == 0 and np.all(out > 0)
+    # grazing sun from below the surface: lit (no occluder below) but N.L = 0
+    out, sh = osc.shade_hit(CFG, frame((0, 1, 0)), atlas, o2, h2)
+    assert sh == 0 and np.all(out == 0)
+
+
+def test_emissive_and_miss():
+    osc = O.OracleScene(quad_scene((0.5, 0.5, 0.5), 0.5, 0.0, emissive=(0.2, 0.4, 0.8, 5.0)))
+    atlas = O.new_atlas(CFG)
+    origin = np.array([0.0, 1.0, 0.0], np.float32)
+    h = hit_at(osc, origin, 0.0, 0.0)
+    out_sun_below, _ = osc.shade_hit(CFG, frame((0, 1, 0)), atlas, origin, h)
+    assert np.allclose(out_sun_below, np.array([0.2, 0.4, 0.8]) * 5.0, rtol=1e-6)
+    miss = np.zeros(1, HIT_DTYPE)
+    miss["tri"] = -1
+    out, sh = osc.shade_hit(CFG, frame((0, -1, 0)), atlas, origin, miss[0])
+    assert sh == -1 and np.allclose(out, 0.3, rtol=1e-5)

@@ -1,0 +1,133 @@
+"""Host-side checks with no GPU: the PCG32 generator against its published
+known answers, scene determinism and sizes, the C-ABI library (exports, record
+layouts, argument validation) and host-side parity of the ray-direction table."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from anerend_amd import native, scenegen
+from anerend_amd.types import (INSTANCE_DTYPE, MATERIAL_DTYPE, MESH_DTYPE, Config, Frame,
+                               PRECISION_BF16, ptr)
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ane_ddgi.h")
+
+
+def test_pcg32_known_answers():
+    # pcg32-demo (O'Neill, pcg-c-basic): pcg32_srandom_r(&rng, 42u, 54u)
+    g = scenegen.PCG32(42, [54])
+    got = [int(g.next_u32()[0]) for _ in range(6)]
+    assert got == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_scenes_are_deterministic_and_sized():
+    a, b = scenegen.room_scene(1), scenegen.room_scene(1)
+    for f in ("vertices", "indices", "meshes", "materials", "instances"):
+        assert getattr(a, f).tobytes() == getattr(b, f).tobytes()
+    assert a.num_tris == 12 + 64 * 12
+    assert scenegen.room_scene(2).instances.tobytes() != a.instances.tobytes()
+    t = scenegen.terrain_scene(2, cells=32, num_instances=100)
+    assert t.num_tris == 2 * 32 * 32 + 50 * 12 + 50 * 80
+    mats = a.materials
+    assert np.all((mats["base_col_fac"][:, :3] >= 0.1) & (mats["base_col_fac"][:, :3] <= 0.9))
+    assert np.all(np.isin(mats["met_rough"][:, 0], [0.0, 1.0]))
+
+
+def test_library_exports_every_declared_symbol():
+    L = native.load()
+    text = open(HEADER).read()
+    declared = sorted(set(re.findall(r"^\w[\w\s\*]*?\b(ane_\w+)\(", text, re.M)))
+    assert len(declared) >= 25
+    missing = [d for d in declared if not hasattr(L, d)]
+    assert not missing, missing
+    assert set(native.EXPORTED) <= set(declared)
+    assert L.ane_abi_version() == 1
+
+
+def test_record_layouts_match_header(tmp_path):
+    """Compile a probe against include/ane_ddgi.h with gcc and compare sizes/offsets."""
+    src = tmp_path / "probe.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "ane_ddgi.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(ane_ddgi_config), sizeof(ane_ddgi_frame),
+         sizeof(ane_mesh), sizeof(ane_material), sizeof(ane_instance), sizeof(ane_hit));
+  printf("%zu %zu %zu %zu\n", offsetof(ane_ddgi_frame, rotation), offsetof(ane_ddgi_frame, bake_tile),
+         offsetof(ane_instance, custom_index_and_mask), offsetof(ane_ddgi_config, precision));
+  return 0;
+}''')
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    sizes = [int(x) for x in out]
+    assert sizes[:6] == [ctypes.sizeof(Config), ctypes.sizeof(Frame), MESH_DTYPE.itemsize,
+                         MATERIAL_DTYPE.itemsize, INSTANCE_DTYPE.itemsize, 16]
+    assert sizes[6:] == [Frame.rotation.offset, Frame.bake_tile.offset,
+                         INSTANCE_DTYPE.fields["custom_index_and_mask"][1], Config.precision.offset]
+
+
+@pytest.mark.parametrize("n,angle,axis", [(256, 0.0, 0), (512, 123.4, 1), (100, 359.0, 2)])
+def test_host_ray_directions_bit_exact_with_oracle(n, angle, axis):
+    rot = scenegen.rotation_matrix(angle, axis)
+    lib = native.ray_directions(n, rot)
+    ora = O.rotate_dirs(rot, O.fibonacci_dirs(n))
+    assert np.array_equal(lib.view(np.uint32), ora.view(np.uint32))
+
+
+def _create(cfg):
+    h = ctypes.c_void_p()
+    return native.load().ane_ddgi_create(ctypes.byref(cfg), ctypes.byref(h))
+
+
+def test_config_validation_without_device():
+    L = native.load()
+    bad = [Config.make((0, 8, 8)), Config.make((8, 8, 8), rays=0), Config.make((8, 8, 8), rays=5000),
+           Config.make((8, 8, 8), step=(0, 1, 1)), Config.make((8, 8, 8), hysteresis=1.5),
+           Config.make((8, 8, 8), t_min=1.0, t_max=0.5)]
+    for c in bad:
+        assert _create(c) == 1, L.ane_last_error()
+    assert _create(Config.make((8, 8, 8), rays=100, precision=PRECISION_BF16)) == 6
+    assert _create(Config.make((8, 8, 8), quirks=1)) == 6
+    assert L.ane_ddgi_create(ctypes.byref(Config.make((8, 8, 8))), None) == 1
+    assert b"out is NULL" in L.ane_last_error()
+
+
+def test_scene_validation_without_device():
+    L = native.load()
+    s = scenegen.room_scene(1, num_boxes=2)
+
+    def create(scene):
+        h = ctypes.c_void_p()
+        arrs = [np.ascontiguousarray(a) for a in
+                (scene.vertices, scene.indices, scene.meshes, scene.materials, scene.instances)]
+        v, i, m, mt, inst = arrs
+        return L.ane_scene_create(ptr(v), len(v), ptr(i), len(i), ptr(m), len(m), ptr(mt), len(mt),
+                                  ptr(inst), len(inst), None, ctypes.byref(h))
+
+    bad_idx = scenegen.Scene(s.vertices, s.indices.copy(), s.meshes, s.materials, s.instances)
+    bad_idx.indices[5] = 10_000
+    assert create(bad_idx) == 1 and b"out of the vertex array" in L.ane_last_error()
+    tex = scenegen.Scene(s.vertices, s.indices, s.meshes, s.materials.copy(), s.instances)
+    tex.materials["bindless_indices"][0, 1] = 3
+    assert create(tex) == 6
+    bad_mesh = scenegen.Scene(s.vertices, s.indices, s.meshes, s.materials, s.instances.copy())
+    bad_mesh.instances["custom_index_and_mask"][1] |= (9 << 10)
+    assert create(bad_mesh) == 1 and b"references mesh" in L.ane_last_error()
+    bad_x = scenegen.Scene(s.vertices, s.indices, s.meshes, s.materials, s.instances.copy())
+    bad_x.instances["transform"][0, 0] = np.nan
+    assert create(bad_x) == 1
+
+
+def test_status_strings():
+    L = native.load()
+    assert L.ane_status_string(0) == b"ok"
+    assert L.ane_status_string(3) == b"admission refused"
+    err = native.AneError(1, "x")
+    assert "invalid argument" in str(err)
