@@ -516,6 +516,12 @@ struct ane_ddgi_ctx {
   int cur = 0;
   float* d_base_dirs = nullptr;
   float* d_dirs = nullptr;
+  float4* d_dirpre = nullptr;
+  float4* d_hitpos = nullptr;
+  uint4* d_hitgeom = nullptr;
+  uint32_t* d_list = nullptr;
+  uint32_t* d_flag = nullptr;      // hit flags, scanned in place into list offsets
+  uint32_t* d_scan = nullptr;      // scan scratch
   float* d_w32 = nullptr;
   float* d_wsum = nullptr;
   uint16_t* d_wbf16 = nullptr;
@@ -545,6 +551,7 @@ struct ane_ddgi_ctx {
 static void ctx_free(ane_ddgi_ctx* c)
 {
   dfree(c->d_atlas[0]); dfree(c->d_atlas[1]); dfree(c->d_base_dirs); dfree(c->d_dirs);
+  dfree(c->d_dirpre); dfree(c->d_hitpos); dfree(c->d_hitgeom); dfree(c->d_list); dfree(c->d_flag); dfree(c->d_scan);
   dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
@@ -604,6 +611,12 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   ANE_A(d_atlas[1], c->texels * 4);
   ANE_A(d_base_dirs, c->N * 3);
   ANE_A(d_dirs, c->N * 3);
+  ANE_A(d_dirpre, (c->N + 1) * 3);
+  ANE_A(d_hitpos, PN);
+  ANE_A(d_hitgeom, PN);
+  ANE_A(d_list, PN);
+  ANE_A(d_flag, PN);
+  ANE_A(d_scan, scan_scratch_words(PN));
   ANE_A(d_w32, 64 * c->N);
   ANE_A(d_wsum, 64);
   ANE_A(d_wbf16, 64 * c->N);
@@ -704,7 +717,11 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
   a.g = c->g;
   a.f = fk;
   a.dirs = c->d_dirs;
+  a.dirpre = c->d_dirpre;
   a.order = c->d_order;
+  a.hitpos = c->d_hitpos;
+  a.list = c->d_list;
+  a.list_count = c->d_work + 2;
   a.hits = c->d_hits;
   a.shadow = c->d_shadow;
   a.stats = c->d_stats;
@@ -721,8 +738,8 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
 
 static int do_frame_setup(ane_ddgi_ctx* c, const FrameK& fk, bool bake, hipStream_t st)
 {
-  launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_dirs, c->d_w32, c->d_wsum, c->d_wbf16,
-                     c->d_wfp8, c->d_wfp8_scale, bake, st);
+  launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_dirs, c->d_dirpre, c->d_w32, c->d_wsum,
+                     c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, bake, st);
   ANE_CHECK_HIP(hipGetLastError());
   return ANE_OK;
 }
@@ -750,11 +767,14 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   b.work = c->d_work + 1;
   b.stats = c->d_stats + 2;
   ANE_TRY(c->timers[kStShadow].begin(st));
+  launch_hit_compact(s->view, c->d_hits, c->d_order, (int)c->N, a.nrays, c->d_hitpos,
+                     c->d_hitgeom, c->d_shadow, c->d_list, c->d_work + 2, c->d_flag, c->d_scan, st);
   launch_trace(b, true, false, c->counting, st);
   ANE_TRY(c->timers[kStShadow].end(st));
   ANE_TRY(c->timers[kStShade].begin(st));
-  launch_shade(s->view, c->g, fk, c->d_hits, c->d_shadow, c->d_atlas[c->cur], c->cfg.precision,
-               c->d_rad, c->d_rad_scale, nprobes, a.probe_base, st);
+  launch_shade(s->view, c->g, fk, c->d_hits, c->d_hitpos, c->d_hitgeom, c->d_shadow,
+               c->d_atlas[c->cur], c->cfg.precision, c->d_rad, c->d_rad_scale, nprobes,
+               a.probe_base, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_TRY(c->timers[kStShade].end(st));
   c->rad_begin = fk.layer_begin;

@@ -185,11 +185,12 @@ ANE_HD bool tri_test(const RayPre& r, f3 v0, f3 v1, f3 v2, float tmin, float tma
   if (det == 0.0f) return false;
   float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
   float T = U * Az + V * Bz + W * Cz;
-  float t = T / det;
+  const float rcp = 1.0f / det;  // one correctly rounded reciprocal, then products
+  float t = T * rcp;
   if (!(t >= tmin && t <= tmax)) return false;
   t_out = t;
-  u_out = V / det;
-  v_out = W / det;
+  u_out = V * rcp;
+  v_out = W * rcp;
   return true;
 }
 

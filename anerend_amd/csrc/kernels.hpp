@@ -37,7 +37,11 @@ struct TraceArgs {
   GridK g;
   FrameK f;
   const float* dirs;      // rotated directions [N][3]
+  const float4* dirpre;   // direction ray setup [N + 1][3] (entry N: the sun)
   const uint32_t* order;  // processing order of ray indices [N]
+  const float4* hitpos;   // shadow: hit positions (shadow-ray origins) per ray
+  const uint32_t* list;   // shadow: compacted hit rays
+  const uint32_t* list_count;
   ane_hit* hits;          // closest: out; shadow: in
   uint8_t* shadow;        // shadow: out (0 lit, 1 shadowed, 255 miss)
   uint32_t* work;         // dynamic ray counter (zeroed before launch)
@@ -59,11 +63,16 @@ int trace_lanes_per_launch();
 int trace_stack_lds();
 void launch_trace(const TraceArgs& a, bool shadow, bool any_hit_rays, bool count, hipStream_t s);
 void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
-                        float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8, float* wfp8_scale,
-                        bool bake, hipStream_t s);
+                        float4* dirpre, float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8,
+                        float* wfp8_scale, bool bake, hipStream_t s);
+// flag: nrays words; scratch: scan_scratch_words(nrays) words
+void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
+                        int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
+                        uint32_t* count, uint32_t* flag, uint32_t* scratch, hipStream_t s);
 void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
-                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
-                  float* rad_scale, int num_probes, int probe_base, hipStream_t s);
+                  const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
+                  const float* atlas, int precision, void* rad, float* rad_scale, int num_probes,
+                  int probe_base, hipStream_t s);
 
 // conv.hip
 void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,

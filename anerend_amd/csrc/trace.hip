@@ -2,15 +2,20 @@
 //
 // Replaces vkCmdTraceRaysKHR of IrradianceProbeRT (IrradianceProbeRayTracingPass.cpp:95-165)
 // and its shaders irradiance_probe_update.{rgen,rchit,rmiss}:
-//   k_frame_setup  rotated Fibonacci directions (rgen:35-46,86-88) and the 64 x N
-//                  cosine-lobe weight matrix of the convolution (probe_conv.comp:37-61)
-//   k_trace        persistent wave-wide BVH traversal, one ray per lane, ballot-based
-//                  refill of finished lanes, LDS ring stack with a global spill area;
-//                  closest hit (rgen:90-106) or any hit toward the sun (rchit:44-66)
-//   k_shade        one block per probe: attribute fetch, Cook-Torrance direct light,
-//                  indirect light from the previous atlas, emissive, sky on miss
-//                  (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
-//                  context precision (FP8: per-probe scale from the block maximum)
+//   k_frame_setup    rotated Fibonacci directions (rgen:35-46,86-88), their ray setup
+//                    (reciprocals, watertight shear), the sun ray setup and the 64 x N
+//                    cosine-lobe weight matrix of the convolution (probe_conv.comp:37-61)
+//   k_trace          persistent wave-wide BVH traversal, one ray per lane: each wave
+//                    claims chunks of consecutive (direction-coherent) rays and refills
+//                    finished lanes by ballot between traversal steps; short stack in
+//                    LDS with a global spill area; closest hit (rgen:90-106) or any hit
+//                    toward the sun (rchit:44-66)
+//   k_hit_compact    hit position (rchit:30-42) of every hit and ballot compaction of
+//                    the hit rays into the shadow-ray list; misses marked
+//   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
+//                    indirect light from the previous atlas, emissive, sky on miss
+//                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
+//                    context precision (FP8: per-probe scale from the block maximum)
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -22,25 +27,50 @@ namespace ane {
 #ifndef ANE_TRACE_STACK
 #define ANE_TRACE_STACK 16
 #endif
-#ifndef ANE_TRACE_STEPS
-#define ANE_TRACE_STEPS 64
+#ifndef ANE_TRACE_CHUNK
+#define ANE_TRACE_CHUNK 256
 #endif
 #ifndef ANE_TRACE_REFILL
-#define ANE_TRACE_REFILL 16
+#define ANE_TRACE_REFILL 32
 #endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring)
-static constexpr int kSteps = ANE_TRACE_STEPS;   // traversal steps between refill checks
+static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 
 int trace_stack_lds() { return kSS; }
 
+// Direction-only part of the ray setup (RayPre without the origin), 48 bytes:
+//   a = (d.x, d.y, d.z, inv.x)  b = (inv.y, inv.z, sx, sy)  c = (sz, kx | ky << 2 | kz << 4, 0, 0)
+__device__ __forceinline__ void store_dirpre(float4* out, const RayPre& r)
+{
+  out[0] = make_float4(r.d.x, r.d.y, r.d.z, r.inv.x);
+  out[1] = make_float4(r.inv.y, r.inv.z, r.sx, r.sy);
+  out[2] = make_float4(r.sz, __int_as_float(r.kx | (r.ky << 2) | (r.kz << 4)), 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ void load_dirpre(RayPre& r, const float4* p, f3 o)
+{
+  const float4 a = p[0], b = p[1], c = p[2];
+  r.o = o;
+  r.d = mk(a.x, a.y, a.z);
+  r.inv = mk(a.w, b.x, b.y);
+  r.sx = b.z;
+  r.sy = b.w;
+  r.sz = c.x;
+  const int k = __float_as_int(c.y);
+  r.kx = k & 3;
+  r.ky = (k >> 2) & 3;
+  r.kz = (k >> 4) & 3;
+}
+
 // ---------------------------------------------------------------------------
-// frame setup: rotated directions and convolution weights
+// frame setup: rotated directions, their ray setup, the sun ray setup and the
+// convolution weights (one block per texel)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, const float* base,
-                                                     float* dirs, float* w32, float* wsum,
-                                                     uint16_t* wbf16, uint8_t* wfp8,
+__global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, f3 sun, const float* base,
+                                                     float* dirs, float4* dirpre, float* w32,
+                                                     float* wsum, uint16_t* wbf16, uint8_t* wfp8,
                                                      float* wfp8_scale, int32_t bake)
 {
   const int t = blockIdx.x;  // texel 0..63
@@ -54,11 +84,19 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, const fl
       dirs[3 * r + 0] = d.x;
       dirs[3 * r + 1] = d.y;
       dirs[3 * r + 2] = d.z;
+      RayPre rp;
+      ray_setup(rp, mk(0.0f, 0.0f, 0.0f), d);
+      store_dirpre(dirpre + 3 * r, rp);
     }
     float w = fmaxf(0.0f, dot(d, o));
     if (!(w > 1e-6f)) w = 0.0f;
     w32[(int64_t)t * n + r] = w;
     s += w;
+  }
+  if (t == 1 && threadIdx.x == 0) {
+    RayPre rp;
+    ray_setup(rp, mk(0.0f, 0.0f, 0.0f), sun);
+    store_dirpre(dirpre + 3 * n, rp);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -94,13 +132,14 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, const fl
 }
 
 void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
-                        float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8, float* wfp8_scale,
-                        bool bake, hipStream_t s)
+                        float4* dirpre, float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8,
+                        float* wfp8_scale, bool bake, hipStream_t s)
 {
   Mat3 rot;
   for (int i = 0; i < 16; ++i) rot.m[i] = f.rot[i];
-  hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, rot, base_dirs, dirs, w32, wsum,
-                     wbf16, wfp8, wfp8_scale, (int32_t)bake);
+  const f3 sun = normalize(mk(-f.light_dir[0], -f.light_dir[1], -f.light_dir[2]));
+  hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, rot, sun, base_dirs, dirs, dirpre,
+                     w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,9 +190,12 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
   const float tmin = RAYS ? a.t_min : a.g.t_min;
   const float tmax0 = RAYS ? a.t_max : a.g.t_max;
   const int n = a.g.rays;
+  // shadow rays: the compacted hit list, its length lives in device memory
+  const uint32_t nrays = (SHADOW && !RAYS) ? *a.list_count : (uint32_t)a.nrays;
 
-  int32_t ray = -1;  // index into [0, nrays) of the active ray, -1 idle
+  int32_t ray = -1;  // active ray (processing index), -1 idle
   bool exhausted = false;
+  uint32_t cbeg = 0, cend = 0;  // wave-uniform: remaining part of the claimed chunk
   RayPre r;
   float tbest = 0.0f, bu = 0.0f, bv = 0.0f;
   int32_t bid = -1, node = 0, sp = 0, out = 0;
@@ -161,87 +203,74 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
   unsigned long long cn = 0, ct = 0;
 
   for (;;) {
-    // ---- refill idle lanes with new rays (one atomic per wave) ----
+    // ---- refill finished lanes (between steps) ----
     const bool idle = (ray < 0) && !exhausted;
     const uint64_t im = __ballot(idle);
     const uint64_t am = __ballot(ray >= 0);
-    if (im != 0ull && (am == 0ull || __popcll(im) >= kRefill)) {
-      const int leader = __ffsll((unsigned long long)im) - 1;
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(a.work, (uint32_t)__popcll(im));
-      base = __shfl(base, leader, 64);
+    const uint32_t need = (uint32_t)__popcll(im);
+    if (need != 0u && (am == 0ull || need >= (uint32_t)kRefill)) {
+      if (cbeg >= cend) {  // claim the next chunk (one atomic per wave and chunk)
+        uint32_t base = 0;
+        if (lane == __ffsll((unsigned long long)im) - 1) base = atomicAdd(a.work, (uint32_t)kChunk);
+        base = __shfl(base, __ffsll((unsigned long long)im) - 1, 64);
+        cbeg = base;
+        cend = base + (uint32_t)kChunk < nrays ? base + (uint32_t)kChunk : nrays;
+      }
+      const uint32_t avail = cend > cbeg ? cend - cbeg : 0u;
+      const uint32_t rank = (uint32_t)__popcll(im & lt);
       if (idle) {
-        const uint32_t gidx = base + (uint32_t)__popcll(im & lt);
-        if (gidx >= (uint32_t)a.nrays) {
+        if (avail == 0u) {
           exhausted = true;
-        } else {
-          f3 o, d;
-          bool skip = false;
+        } else if (rank < avail) {
+          const uint32_t gidx = cbeg + rank;
+          f3 o;
           if (RAYS) {
             out = (int32_t)gidx;
             o = ld3(a.ray_orig + 3 * (int64_t)gidx);
-            d = ld3(a.ray_dir + 3 * (int64_t)gidx);
+            ray_setup(r, o, ld3(a.ray_dir + 3 * (int64_t)gidx));
+          } else if (SHADOW) {
+            out = (int32_t)a.list[gidx];
+            const float4 p = a.hitpos[out];
+            load_dirpre(r, a.dirpre + 3 * n, mk(p.x, p.y, p.z));
           } else {
             const int ql = (int)(gidx / (uint32_t)n);
             const int j = (int)(gidx - (uint32_t)ql * (uint32_t)n);
             const int i = (int)a.order[j];
             out = ql * n + i;
-            if (SHADOW) {
-              const ane_hit h = a.hits[out];
-              if (h.tri < 0) {
-                a.shadow[out] = 255;
-                skip = true;
-              } else {
-                HitGeom hg = hit_geom(sc, h.tri);
-                o = hit_pos(sc, hg, h.u, h.v);
-                d = mk(a.sun[0], a.sun[1], a.sun[2]);
-              }
-            } else {
-              const int q = a.probe_base + ql;
-              const int px = q % a.g.nx;
-              const int pz = (q / a.g.nx) % a.g.nz;
-              const int py = q / (a.g.nx * a.g.nz);
-              o = probe_origin(a.g, a.f, px, py, pz);
-              d = ld3(a.dirs + 3 * i);
-            }
+            const int q = a.probe_base + ql;
+            const int px = q % a.g.nx;
+            const int pz = (q / a.g.nx) % a.g.nz;
+            const int py = q / (a.g.nx * a.g.nz);
+            load_dirpre(r, a.dirpre + 3 * i, probe_origin(a.g, a.f, px, py, pz));
           }
-          if (!skip) {
-            ray_setup(r, o, d);
-            tbest = tmax0;
-            bid = -1;
-            bu = bv = 0.0f;
-            sp = 0;
-            steps_left = 2u * (uint32_t)sc.ntri + 8u;
-            ray = (int32_t)gidx;
-            if (sc.ntri >= 2) {
-              node = 0;
-            } else if (sc.ntri == 1) {
-              node = ~0;
-            } else {
-              node = 0x7FFFFFFF;  // empty scene: finish immediately
-            }
-          }
+          tbest = tmax0;
+          bid = -1;
+          bu = bv = 0.0f;
+          sp = 0;
+          steps_left = 2u * (uint32_t)sc.ntri + 8u;
+          ray = (int32_t)gidx;
+          node = (sc.ntri >= 2) ? 0 : (sc.ntri == 1 ? ~0 : 0x7FFFFFFF);
         }
       }
+      cbeg += need < avail ? need : avail;
     }
-    if (__ballot(ray >= 0) == 0ull) {
+    if (am == 0ull && __ballot(ray >= 0) == 0ull) {
       if (__ballot(!exhausted) == 0ull) break;
       continue;
     }
-    // ---- advance the active rays by up to kSteps traversal steps ----
+    // ---- one traversal step ----
     if (ray >= 0) {
       bool done = (node == 0x7FFFFFFF);
-      for (int step = 0; step < kSteps && !done; ++step) {
+      if (!done) {
         bool pop = true;
         // guards: a malformed hierarchy ends the ray and raises the error flag
         // instead of reading out of bounds or looping forever
         if (steps_left == 0u || (node >= 0 && node >= sc.ntri - 1) || (node < 0 && ~node >= sc.ntri)) {
           *a.overflow = 1u;
           done = true;
-          break;
-        }
-        --steps_left;
-        if (node >= 0) {
+          pop = false;
+        } else if (node >= 0) {
+          --steps_left;
           if (COUNT) ++cn;
           const float4* nd = sc.nodes + 4 * (int64_t)node;
           const float4 na = nd[0], nb = nd[1], nc = nd[2], nn = nd[3];
@@ -255,30 +284,34 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
               const bool sw = tn1 < tn0;
               const int32_t nearc = sw ? R : L;
               const int32_t farc = sw ? L : R;
-              // push far child (LDS ring, spill the oldest entry when full)
+              // push the far child (LDS ring; the oldest entry spills when full)
               const int slot = sp & (kSS - 1);
+              bool ok = true;
               if (sp >= kSS) {
                 const int si = sp - kSS;
                 if (si < a.spill_depth) {
                   a.spill[(int64_t)si * a.total_lanes + gl] = stk[slot][tid];
                 } else {
                   *a.overflow = 1u;
-                  done = true;
-                  break;
+                  ok = false;
                 }
               }
-              stk[slot][tid] = farc;
-              ++sp;
-              node = nearc;
+              if (ok) {
+                stk[slot][tid] = farc;
+                ++sp;
+                node = nearc;
+              } else {
+                done = true;
+              }
             } else {
               node = b0 ? L : R;
             }
           }
         } else {
+          --steps_left;
           if (COUNT) ++ct;
-          const int p = ~node;
-          const float4 v0 = sc.tris[3 * (int64_t)p], v1 = sc.tris[3 * (int64_t)p + 1],
-                       v2 = sc.tris[3 * (int64_t)p + 2];
+          const int64_t p = ~node;
+          const float4 v0 = sc.tris[3 * p], v1 = sc.tris[3 * p + 1], v2 = sc.tris[3 * p + 2];
           float t, u, v;
           if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin,
                        tbest, t, u, v)) {
@@ -289,9 +322,8 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
               bu = u;
               bv = v;
               done = true;
-              break;
-            }
-            if (t < tbest || (t == tbest && id < bid)) {
+              pop = false;
+            } else if (t < tbest || (t == tbest && id < bid)) {
               bid = id;
               tbest = t;
               bu = u;
@@ -299,15 +331,15 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
             }
           }
         }
-        if (pop) {
+        if (pop && !done) {
           if (sp == 0) {
             done = true;
-            break;
+          } else {
+            --sp;
+            const int slot = sp & (kSS - 1);
+            node = stk[slot][tid];
+            if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
           }
-          --sp;
-          const int slot = sp & (kSS - 1);
-          node = stk[slot][tid];
-          if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
         }
       }
       if (done) {
@@ -360,6 +392,59 @@ void launch_trace(const TraceArgs& a, bool shadow, bool rays, bool count, hipStr
     else { if (count) ANE_TR(false, false, true); else ANE_TR(false, false, false); }
   }
 #undef ANE_TR
+}
+
+// ---------------------------------------------------------------------------
+// hit positions + order-preserving compaction of the shadow-ray list: the list
+// keeps the processing order (probe-major, direction-sorted), so consecutive
+// shadow rays start from neighbouring hit points
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hit_pos(SceneK sc, const ane_hit* hits,
+                                                 const uint32_t* order, int n, int nrays,
+                                                 float4* hitpos, uint4* hitgeom, uint8_t* shadow,
+                                                 uint32_t* flag)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrays) return;
+  const int ql = g / n;
+  const int out = ql * n + (int)order[g - ql * n];
+  const ane_hit h = hits[out];
+  const bool hit = h.tri >= 0;
+  if (hit) {
+    HitGeom hg = hit_geom(sc, h.tri);
+    const f3 p = hit_pos(sc, hg, h.u, h.v);
+    hitpos[out] = make_float4(p.x, p.y, p.z, 1.0f);
+    hitgeom[out] = make_uint4(hg.inst, hg.v[0], hg.v[1], hg.v[2]);
+  } else {
+    shadow[out] = 255;
+  }
+  flag[g] = hit ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_hit_scatter(const ane_hit* hits, const uint32_t* order, int n,
+                                                     int nrays, const uint32_t* offs, uint32_t* list,
+                                                     uint32_t* count)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrays) return;
+  const int ql = g / n;
+  const int out = ql * n + (int)order[g - ql * n];
+  const bool hit = hits[out].tri >= 0;
+  if (hit) list[offs[g]] = (uint32_t)out;
+  if (g == nrays - 1) *count = offs[g] + (hit ? 1u : 0u);
+}
+
+void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
+                        int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
+                        uint32_t* count, uint32_t* flag, uint32_t* scratch, hipStream_t s)
+{
+  if (nrays <= 0) return;
+  const unsigned blocks = (unsigned)((nrays + 255) / 256);
+  hipLaunchKernelGGL(k_hit_pos, dim3(blocks), dim3(256), 0, s, sc, hits, order, n, nrays, hitpos,
+                     hitgeom, shadow, flag);
+  exclusive_scan(flag, nrays, scratch, s);
+  hipLaunchKernelGGL(k_hit_scatter, dim3(blocks), dim3(256), 0, s, hits, order, n, nrays,
+                     (const uint32_t*)flag, list, count);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,13 +623,17 @@ __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas
 
 // rchit:24-110 (+ rmiss:17-21 for misses)
 __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const float4* atlas,
-                        f3 origin, ane_hit h, uint8_t shadowed)
+                        f3 origin, ane_hit h, float4 hp, uint4 geom, uint8_t shadowed)
 {
   if (h.tri < 0) return mk(f.sky[0], f.sky[1], f.sky[2]);
-  HitGeom hg = hit_geom(sc, h.tri);
+  HitGeom hg;  // resolved by k_hit_pos (instance, absolute vertex indices)
+  hg.inst = geom.x;
+  hg.v[0] = geom.y;
+  hg.v[1] = geom.z;
+  hg.v[2] = geom.w;
   const ane_instance& in = sc.inst[hg.inst];
   const ane_material& mat = sc.mats[mat_of(in.custom_index_and_mask)];
-  f3 pos = hit_pos(sc, hg, h.u, h.v);
+  const f3 pos = mk(hp.x, hp.y, hp.z);  // k_hit_compact: rchit:40-42
   const bool lit = shadowed == 0;
   float bw[3] = {1.0f - h.u - h.v, h.u, h.v};
   f3 nrm = mk(0.0f, 0.0f, 0.0f), col = mk(0.0f, 0.0f, 0.0f);
@@ -578,22 +667,32 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   return add(acc, emissive);
 }
 
-template <int PREC>
-__global__ __launch_bounds__(256) void k_shade(SceneK sc, GridK g, FrameK f, const ane_hit* hits,
-                                               const uint8_t* shadow, const float4* atlas,
-                                               void* rad, float* rad_scale, int probe_base)
+template <int PREC, int TB>
+__global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, const ane_hit* hits,
+                                              const float4* hitpos, const uint4* hitgeom,
+                                              const uint8_t* shadow, const float4* atlas, void* rad,
+                                              float* rad_scale, int probe_base)
 {
   extern __shared__ float sL[];  // FP8: [3][N] staging for the per-probe scale
-  __shared__ float red[256];
+  __shared__ float red[TB];
   const int ql = blockIdx.x;
   const int q = probe_base + ql;
   const int n = g.rays;
   const int px = q % g.nx, pz = (q / g.nx) % g.nz, py = q / (g.nx * g.nz);
   const f3 origin = probe_origin(g, f, px, py, pz);
   float mx = 0.0f;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = threadIdx.x; i < n; i += TB) {
     const int64_t idx = (int64_t)ql * n + i;
-    const f3 L = shade_ray(sc, g, f, atlas, origin, hits[idx], shadow[idx]);
+    const ane_hit h = hits[idx];
+    float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint4 geom = make_uint4(0u, 0u, 0u, 0u);
+    uint8_t sh = 255;
+    if (h.tri >= 0) {
+      hp = hitpos[idx];
+      geom = hitgeom[idx];
+      sh = shadow[idx];
+    }
+    const f3 L = shade_ray(sc, g, f, atlas, origin, h, hp, geom, sh);
     if (PREC == ANE_PRECISION_FP32) {
       float* o = (float*)rad + (int64_t)ql * 3 * n;
       o[i] = L.x;
@@ -614,33 +713,48 @@ __global__ __launch_bounds__(256) void k_shade(SceneK sc, GridK g, FrameK f, con
   if (PREC == ANE_PRECISION_FP8) {
     red[threadIdx.x] = mx;
     __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
+    for (int k = TB / 2; k > 0; k >>= 1) {
       if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
       __syncthreads();
     }
     const float sc8 = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
     uint8_t* o = (uint8_t*)rad + (int64_t)ql * 3 * n;
-    for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) o[i] = f32_to_e4m3(sL[i] / sc8);
+    for (int i = threadIdx.x; i < 3 * n; i += TB) o[i] = f32_to_e4m3(sL[i] / sc8);
     if (threadIdx.x == 0) rad_scale[ql] = sc8;
   }
 }
 
+template <int TB>
+static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
+                         const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
+                         const float4* a4, int precision, void* rad, float* rad_scale,
+                         int num_probes, int probe_base, hipStream_t s)
+{
+  if (precision == ANE_PRECISION_FP32)
+    hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP32, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
+                       hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+  else if (precision == ANE_PRECISION_BF16)
+    hipLaunchKernelGGL((k_shade<ANE_PRECISION_BF16, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
+                       hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+  else
+    hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP8, TB>), dim3(num_probes), dim3(TB),
+                       (size_t)12 * g.rays, s, sc, g, f, hits, hitpos, hitgeom, shadow, a4, rad,
+                       rad_scale, probe_base);
+}
+
 void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
-                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
-                  float* rad_scale, int num_probes, int probe_base, hipStream_t s)
+                  const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
+                  const float* atlas, int precision, void* rad, float* rad_scale, int num_probes,
+                  int probe_base, hipStream_t s)
 {
   if (num_probes <= 0) return;
   const float4* a4 = reinterpret_cast<const float4*>(atlas);
-  if (precision == ANE_PRECISION_FP32)
-    hipLaunchKernelGGL(k_shade<ANE_PRECISION_FP32>, dim3(num_probes), dim3(256), 0, s, sc, g, f,
-                       hits, shadow, a4, rad, rad_scale, probe_base);
-  else if (precision == ANE_PRECISION_BF16)
-    hipLaunchKernelGGL(k_shade<ANE_PRECISION_BF16>, dim3(num_probes), dim3(256), 0, s, sc, g, f,
-                       hits, shadow, a4, rad, rad_scale, probe_base);
+  if (g.rays >= 512)
+    shade_launch<512>(sc, g, f, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+                      num_probes, probe_base, s);
   else
-    hipLaunchKernelGGL(k_shade<ANE_PRECISION_FP8>, dim3(num_probes), dim3(256),
-                       (size_t)12 * g.rays, s, sc, g, f, hits, shadow, a4, rad, rad_scale,
-                       probe_base);
+    shade_launch<256>(sc, g, f, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+                      num_probes, probe_base, s);
 }
 
 }  // namespace ane

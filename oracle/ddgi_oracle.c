@@ -552,7 +552,8 @@ static inline int box_test(const ray_pre* r, const float* lo, const float* hi, f
 
 /* Watertight test (Woop, Benthin, Wald 2013), no backface culling
  * (tlas_update.comp:86-87 sets the cull-disable flag).  Returns 1 and t, u, v
- * (u, v: barycentrics of vertices 1 and 2) if tmin <= t <= tmax. */
+ * (u, v: barycentrics of vertices 1 and 2) if tmin <= t <= tmax, where
+ * t = T * (1 / det), u = V * (1 / det), v = W * (1 / det). */
 static inline int tri_test(const ray_pre* r, const float* w, float tmin, float tmax, float* t_out,
                            float* u_out, float* v_out)
 {
@@ -577,11 +578,13 @@ static inline int tri_test(const ray_pre* r, const float* w, float tmin, float t
   if (det == 0.0f) return 0;
   float Az = r->sz * Akz, Bz = r->sz * Bkz, Cz = r->sz * Ckz;
   float T = U * Az + V * Bz + W * Cz;
-  float t = T / det;
+  /* contract: one correctly rounded reciprocal, then products */
+  float rcp = 1.0f / det;
+  float t = T * rcp;
   if (!(t >= tmin && t <= tmax)) return 0;
   *t_out = t;
-  *u_out = V / det;
-  *v_out = W / det;
+  *u_out = V * rcp;
+  *v_out = W * rcp;
   return 1;
 }
 
