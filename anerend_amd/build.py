@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libane_ddgi.so")
-SOURCES = ["api.hip", "bvh.hip", "trace.hip", "conv.hip", "translate.hip"]
+SOURCES = ["api.hip", "bvh.hip", "trace.hip", "conv.hip", "translate.hip", "serve.hip"]
 
 # Float contract: no contraction, correctly rounded divide/sqrt, denormals kept.
 FLAGS = [
@@ -46,5 +46,28 @@ def build(quiet=False, out=None, defines=()):
     return lib
 
 
+HOST = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HOST, "ddgi_frames")
+HOST_SOURCES = ["ddgi_passes.cpp", "ddgi_frames.cpp"]
+
+
+def build_host(quiet=False):
+    """C++ frame-graph driver over the C-ABI (plain g++, links libane_ddgi.so)."""
+    cxx = os.environ.get("CXX", "g++")
+    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES]
+    tmp = HOST_BIN + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", tmp] + srcs + [
+        "-L" + HERE, "-lane_ddgi", "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building ddgi_frames")
+    if (res.stdout or res.stderr) and not quiet:
+        sys.stderr.write(res.stdout + res.stderr)
+    os.replace(tmp, HOST_BIN)
+    return HOST_BIN
+
+
 if __name__ == "__main__":
     print(build(quiet="--quiet" in sys.argv))
+    print(build_host(quiet="--quiet" in sys.argv))

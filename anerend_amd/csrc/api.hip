@@ -223,6 +223,7 @@ struct ane_scene {
   ane_instance* d_inst = nullptr;
   float* d_inv = nullptr;
   uint32_t* d_tri_inst = nullptr;
+  float4* d_tri_rec = nullptr;
   uint32_t* d_inst_off = nullptr;
   float* d_world = nullptr;
   uint64_t* d_keys = nullptr;
@@ -238,11 +239,20 @@ struct ane_scene {
   uint32_t* d_depth = nullptr;
   uint32_t* d_scratch = nullptr;
   int32_t* d_order = nullptr;   // internal nodes bucketed by depth
+  uint64_t* d_lvl_key = nullptr;
   uint32_t* d_hist = nullptr;
   std::vector<int32_t> level_begin;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t depth = 0;
-  float build_ms = 0.0f;
+  mutable float build_ms = 0.0f;
+  mutable bool build_ms_pending = false;  // refit timing not yet read back
+  // pinned double-buffered staging of instance records and inverse transforms:
+  // an upload waits only for the copy that used its slot two uploads ago
+  ane_instance* h_inst[2] = {nullptr, nullptr};
+  float* h_inv[2] = {nullptr, nullptr};
+  hipEvent_t ev_up[2] = {nullptr, nullptr};
+  bool up_pending[2] = {false, false};
+  int up_slot = 0;
   int64_t bytes = 0;
   SceneK view{};
 };
@@ -250,13 +260,19 @@ struct ane_scene {
 static void scene_free(ane_scene* s)
 {
   dfree(s->d_vertices); dfree(s->d_indices); dfree(s->d_meshes); dfree(s->d_mats);
-  dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_inst_off);
+  dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_tri_rec); dfree(s->d_inst_off);
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_depth); dfree(s->d_scratch);
-  dfree(s->d_order); dfree(s->d_hist);
+  dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (int k = 0; k < 2; ++k) {
+    if (s->up_pending[k]) (void)hipEventSynchronize(s->ev_up[k]);
+    if (s->ev_up[k]) (void)hipEventDestroy(s->ev_up[k]);
+    if (s->h_inst[k]) (void)hipHostFree(s->h_inst[k]);
+    if (s->h_inv[k]) (void)hipHostFree(s->h_inv[k]);
+  }
 }
 
 static int validate_instances(const ane_scene* s, const ane_instance* inst, int32_t ninst)
@@ -278,11 +294,26 @@ static int validate_instances(const ane_scene* s, const ane_instance* inst, int3
 static int upload_instances(ane_scene* s, const ane_instance* inst, hipStream_t st)
 {
   s->inst.assign(inst, inst + s->ninst);
-  std::vector<float> inv((size_t)12 * (s->ninst > 0 ? s->ninst : 1));
-  for (int32_t k = 0; k < s->ninst; ++k) invert_3x4(inst[k].transform, inv.data() + 12 * k);
-  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inst, inst, sizeof(ane_instance) * s->ninst, hipMemcpyHostToDevice, st));
-  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inv, inv.data(), sizeof(float) * 12 * s->ninst, hipMemcpyHostToDevice, st));
-  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  const size_t cap = (size_t)(s->ninst > 0 ? s->ninst : 1);
+  const int k = s->up_slot;
+  if (!s->h_inst[k]) {
+    ANE_CHECK_HIP(hipHostMalloc((void**)&s->h_inst[k], sizeof(ane_instance) * cap, hipHostMallocDefault));
+    ANE_CHECK_HIP(hipHostMalloc((void**)&s->h_inv[k], sizeof(float) * 12 * cap, hipHostMallocDefault));
+    ANE_CHECK_HIP(hipEventCreateWithFlags(&s->ev_up[k], hipEventDisableTiming));
+  }
+  if (s->up_pending[k]) {
+    ANE_CHECK_HIP(hipEventSynchronize(s->ev_up[k]));
+    s->up_pending[k] = false;
+  }
+  std::memcpy(s->h_inst[k], inst, sizeof(ane_instance) * s->ninst);
+  for (int32_t i = 0; i < s->ninst; ++i) invert_3x4(inst[i].transform, s->h_inv[k] + 12 * i);
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inst, s->h_inst[k], sizeof(ane_instance) * s->ninst,
+                               hipMemcpyHostToDevice, st));
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_inv, s->h_inv[k], sizeof(float) * 12 * s->ninst,
+                               hipMemcpyHostToDevice, st));
+  ANE_CHECK_HIP(hipEventRecord(s->ev_up[k], st));
+  s->up_pending[k] = true;
+  s->up_slot = k ^ 1;
   return ANE_OK;
 }
 
@@ -293,12 +324,13 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   const int n = s->ntri;
   const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
   ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
-  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_cbounds, write_inst, st);
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cbounds, write_inst, st);
   launch_morton(s->d_world, n, s->d_cbounds, s->d_keys, s->d_ids, st);
   radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
   launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  if (launch_levels(s->d_nodes, n, s->d_arrive, s->d_hist, s->d_order, &s->level_begin, st) != 0)
+  if (launch_levels(s->d_nodes, n, s->d_lvl_key, s->d_keys2, s->d_ids2, s->d_order, s->d_scratch,
+                    s->d_hist, &s->level_begin, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
   launch_depth(s->d_leaf_parent, s->d_nodes, n, s->d_depth, st);
@@ -309,6 +341,7 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   ANE_CHECK_HIP(hipStreamSynchronize(st));
   s->depth = (int32_t)depth;
   ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
+  s->build_ms_pending = false;
   return ANE_OK;
 }
 
@@ -318,13 +351,13 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   const int n = s->ntri;
   const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
   ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
-  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_cbounds, false, st);
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cbounds, false, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
-  ANE_CHECK_HIP(hipEventSynchronize(s->ev1));
-  ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
+  // the refit stays asynchronous; its device time is read when it is asked for
+  s->build_ms_pending = true;
   return ANE_OK;
 }
 
@@ -383,6 +416,7 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_inst, num_instances);
   ANE_A(d_inv, (int64_t)num_instances * 12);
   ANE_A(d_tri_inst, n);
+  ANE_A(d_tri_rec, (int64_t)n * 4);
   ANE_A(d_inst_off, (int64_t)num_instances + 1);
   ANE_A(d_world, n * 9);
   ANE_A(d_keys, n);
@@ -398,6 +432,7 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_depth, 1);
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
   ANE_A(d_order, n);
+  ANE_A(d_lvl_key, n);
   ANE_A(d_hist, 128);
 #undef ANE_A
   auto bail = [&](int code) { scene_free(s); delete s; return code; };
@@ -422,7 +457,7 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   if (st) return bail(st);
   SceneK& v = s->view;
   v.vertices = s->d_vertices; v.indices = s->d_indices; v.meshes = s->d_meshes; v.mats = s->d_mats;
-  v.inst = s->d_inst; v.inv_xform = s->d_inv; v.tri_inst = s->d_tri_inst; v.inst_tri_off = s->d_inst_off;
+  v.inst = s->d_inst; v.inv_xform = s->d_inv; v.tri_inst = s->d_tri_inst; v.tri_rec = s->d_tri_rec; v.inst_tri_off = s->d_inst_off;
   v.nodes = reinterpret_cast<const float4*>(s->d_nodes); v.tris = s->d_tris;
   v.ntri = s->ntri; v.ninst = s->ninst;
   st = scene_build(s, sm, true);
@@ -463,6 +498,11 @@ extern "C" int ane_scene_info_get(const ane_scene* s, ane_scene_info* out)
   out->num_instances = s->ninst;
   out->bvh_depth = s->depth;
   out->device_bytes = s->bytes;
+  if (s->build_ms_pending) {
+    ANE_CHECK_HIP(hipEventSynchronize(s->ev1));
+    ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
+    s->build_ms_pending = false;
+  }
   out->build_ms = s->build_ms;
   return ANE_OK;
 }
@@ -517,11 +557,10 @@ struct ane_ddgi_ctx {
   float* d_base_dirs = nullptr;
   float* d_dirs = nullptr;
   float4* d_dirpre = nullptr;
+  float4* d_probe_org = nullptr;
   float4* d_hitpos = nullptr;
   uint4* d_hitgeom = nullptr;
   uint32_t* d_list = nullptr;
-  uint32_t* d_flag = nullptr;      // hit flags, scanned in place into list offsets
-  uint32_t* d_scan = nullptr;      // scan scratch
   float* d_w32 = nullptr;
   float* d_wsum = nullptr;
   uint16_t* d_wbf16 = nullptr;
@@ -551,7 +590,7 @@ struct ane_ddgi_ctx {
 static void ctx_free(ane_ddgi_ctx* c)
 {
   dfree(c->d_atlas[0]); dfree(c->d_atlas[1]); dfree(c->d_base_dirs); dfree(c->d_dirs);
-  dfree(c->d_dirpre); dfree(c->d_hitpos); dfree(c->d_hitgeom); dfree(c->d_list); dfree(c->d_flag); dfree(c->d_scan);
+  dfree(c->d_dirpre); dfree(c->d_probe_org); dfree(c->d_hitpos); dfree(c->d_hitgeom); dfree(c->d_list);
   dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
@@ -612,11 +651,10 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   ANE_A(d_base_dirs, c->N * 3);
   ANE_A(d_dirs, c->N * 3);
   ANE_A(d_dirpre, (c->N + 1) * 3);
+  ANE_A(d_probe_org, c->P);
   ANE_A(d_hitpos, PN);
   ANE_A(d_hitgeom, PN);
   ANE_A(d_list, PN);
-  ANE_A(d_flag, PN);
-  ANE_A(d_scan, scan_scratch_words(PN));
   ANE_A(d_w32, 64 * c->N);
   ANE_A(d_wsum, 64);
   ANE_A(d_wbf16, 64 * c->N);
@@ -718,6 +756,10 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
   a.f = fk;
   a.dirs = c->d_dirs;
   a.dirpre = c->d_dirpre;
+  a.probe_org = c->d_probe_org;
+  const RayDiv rd = ray_div_magic(c->P * c->N, (int)c->N);
+  a.div_magic = rd.magic;
+  a.div_shift = rd.shift;
   a.order = c->d_order;
   a.hitpos = c->d_hitpos;
   a.list = c->d_list;
@@ -738,8 +780,8 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
 
 static int do_frame_setup(ane_ddgi_ctx* c, const FrameK& fk, bool bake, hipStream_t st)
 {
-  launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_dirs, c->d_dirpre, c->d_w32, c->d_wsum,
-                     c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, bake, st);
+  launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_order, c->d_dirs, c->d_dirpre, c->d_probe_org, c->d_w32,
+                     c->d_wsum, c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, bake, st);
   ANE_CHECK_HIP(hipGetLastError());
   return ANE_OK;
 }
@@ -768,11 +810,11 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   b.stats = c->d_stats + 2;
   ANE_TRY(c->timers[kStShadow].begin(st));
   launch_hit_compact(s->view, c->d_hits, c->d_order, (int)c->N, a.nrays, c->d_hitpos,
-                     c->d_hitgeom, c->d_shadow, c->d_list, c->d_work + 2, c->d_flag, c->d_scan, st);
+                     c->d_hitgeom, c->d_shadow, c->d_list, c->d_work + 2, st);
   launch_trace(b, true, false, c->counting, st);
   ANE_TRY(c->timers[kStShadow].end(st));
   ANE_TRY(c->timers[kStShade].begin(st));
-  launch_shade(s->view, c->g, fk, c->d_hits, c->d_hitpos, c->d_hitgeom, c->d_shadow,
+  launch_shade(s->view, c->g, fk, c->d_probe_org, c->d_hits, c->d_hitpos, c->d_hitgeom, c->d_shadow,
                c->d_atlas[c->cur], c->cfg.precision, c->d_rad, c->d_rad_scale, nprobes,
                a.probe_base, st);
   ANE_CHECK_HIP(hipGetLastError());

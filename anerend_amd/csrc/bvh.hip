@@ -55,7 +55,8 @@ void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
 // flatten: world-space triangles in input order + centroid bounds
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32_t* tri_inst,
-                                                 uint32_t* cbounds, int32_t write_inst)
+                                                 float4* tri_rec, uint32_t* cbounds,
+                                                 int32_t write_inst)
 {
   __shared__ uint32_t sb[6];
   if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
@@ -72,10 +73,14 @@ __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32
     const ane_mesh& m = sc.meshes[mesh_of(in.custom_index_and_mask)];
     uint32_t prim = (uint32_t)t - sc.inst_tri_off[lo];
     float w[9];
+    uint32_t vi[3];
+    f3 po[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       uint32_t idx = sc.indices[m.index_offset + 3 * prim + j];
+      vi[j] = m.vertex_offset + idx;
       f3 p = ld3(sc.vertices + (int64_t)(m.vertex_offset + idx) * 24);
+      po[j] = p;
       f3 q = xform_point(in.transform, p);
       w[3 * j + 0] = q.x;
       w[3 * j + 1] = q.y;
@@ -84,7 +89,14 @@ __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32
     float* dst = world + (int64_t)9 * t;
 #pragma unroll
     for (int j = 0; j < 9; ++j) dst[j] = w[j];
-    if (write_inst) tri_inst[t] = (uint32_t)lo;
+    if (write_inst) {
+      tri_inst[t] = (uint32_t)lo;
+      // one 64-byte record per triangle: what a hit needs, in one cache line
+      float4* r = tri_rec + 4 * (int64_t)t;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r[j] = make_float4(po[j].x, po[j].y, po[j].z, __uint_as_float(vi[j]));
+      r[3] = make_float4(__uint_as_float((uint32_t)lo), 0.0f, 0.0f, 0.0f);
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       float l = fminf(fminf(w[a], w[3 + a]), w[6 + a]);
@@ -433,9 +445,9 @@ __device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, f
   nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
 }
 
-// Depth (root = 0) of every internal node and the per-depth histogram.
-__global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int n, uint32_t* ndepth,
-                                                    uint32_t* hist)
+// Depth (root = 0) of every internal node as a radix key, node id as value.
+__global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int n, uint64_t* key,
+                                                    uint32_t* val)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
@@ -445,17 +457,16 @@ __global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int 
     ++d;
     node = nodes[node].d.z;
   }
-  ndepth[i] = d;
-  atomicAdd(&hist[d], 1u);
+  key[i] = d;
+  val[i] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(256) void k_bucket(const uint32_t* ndepth, int n, uint32_t* cursor,
-                                                int32_t* order)
+// After the exclusive scan of the digit-major tile histograms, the offset of
+// digit d in tile 0 is the start of depth level d in the sorted order.
+__global__ void k_level_offsets(const uint32_t* offs, int64_t ntiles, uint32_t* begin)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  uint32_t slot = atomicAdd(&cursor[ndepth[i]], 1u);
-  if (slot < (uint32_t)(n - 1)) order[slot] = i;
+  const int d = threadIdx.x;
+  if (d < kMaxLevels) begin[d] = offs[(int64_t)d * ntiles];
 }
 
 // One depth level: every child is a leaf or belongs to a deeper, finished level.
@@ -505,12 +516,12 @@ __global__ __launch_bounds__(256) void k_depth(const int32_t* leaf_parent, const
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, uint32_t* cbounds,
-                    bool write_inst, hipStream_t s)
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec,
+                    uint32_t* cbounds, bool write_inst, hipStream_t s)
 {
   if (sc.ntri <= 0) return;
   hipLaunchKernelGGL(k_flatten, dim3(blocks_for(sc.ntri, 256)), dim3(256), 0, s, sc, world,
-                     tri_inst, cbounds, (int32_t)write_inst);
+                     tri_inst, tri_rec, cbounds, (int32_t)write_inst);
 }
 
 void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
@@ -535,31 +546,36 @@ void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, ids, n, tris);
 }
 
-int launch_levels(const NodeBuild* nodes, int n, uint32_t* ndepth, uint32_t* hist,
-                  int32_t* order, std::vector<int32_t>* level_begin, hipStream_t s)
+int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, uint32_t* val,
+                  int32_t* order, uint32_t* scratch, uint32_t* begin,
+                  std::vector<int32_t>* level_begin, hipStream_t s)
 {
   level_begin->clear();
   if (n < 2) return 0;
-  fill_u32(hist, 0u, kMaxLevels, s);
-  hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, nodes, n, ndepth,
-                     hist);
+  const int64_t m = n - 1;
+  hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(m, 256)), dim3(256), 0, s, nodes, n, key, val);
+  // one stable counting pass on the 8-bit depth digit (no global atomics)
+  const int64_t ntiles = (m + kTile - 1) / kTile;
+  uint32_t* hist = scratch;
+  uint32_t* sc = scratch + 256 * ntiles;
+  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s, (const uint64_t*)key, m,
+                     0, hist, ntiles);
+  exclusive_scan(hist, 256 * ntiles, sc, s);
+  hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
+                     (const uint64_t*)key, (const uint32_t*)val, key2,
+                     reinterpret_cast<uint32_t*>(order), m, 0, (const uint32_t*)hist, ntiles);
+  hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(kMaxLevels), 0, s, (const uint32_t*)hist, ntiles,
+                     begin);
   uint32_t h[kMaxLevels];
-  if (hipMemcpyAsync(h, hist, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(h, begin, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   int levels = 0;
-  for (int d = 0; d < kMaxLevels; ++d)
-    if (h[d]) levels = d + 1;
-  uint32_t off[kMaxLevels];
-  uint32_t acc = 0;
   for (int d = 0; d < kMaxLevels; ++d) {
-    off[d] = acc;
-    if (d < levels) level_begin->push_back((int32_t)acc);
-    acc += h[d];
+    const uint32_t next = (d + 1 < kMaxLevels) ? h[d + 1] : (uint32_t)m;
+    if (next > h[d]) levels = d + 1;
   }
-  level_begin->push_back((int32_t)acc);
-  if (hipMemcpyAsync(hist, off, sizeof(off), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
-  hipLaunchKernelGGL(k_bucket, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ndepth, n, hist, order);
-  if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  for (int d = 0; d < levels; ++d) level_begin->push_back((int32_t)h[d]);
+  level_begin->push_back(levels < kMaxLevels ? (int32_t)h[levels] : (int32_t)m);
   return 0;
 }
 

@@ -84,6 +84,7 @@ struct SceneK {
   const ane_instance* inst;
   const float* inv_xform;       // [ninst][12] world-to-object 3x4
   const uint32_t* tri_inst;     // [ntri] instance of triangle id
+  const float4* tri_rec;        // [ntri][4] of triangle id: (object-space p_j, vertex index j) j = 0..2, (instance)
   const uint32_t* inst_tri_off; // [ninst + 1]
   const float4* nodes;          // [ntri - 1][4] traversal nodes
   const float4* tris;           // [ntri][3] sorted world triangles (v0.w = id bits)

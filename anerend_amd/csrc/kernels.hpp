@@ -14,8 +14,8 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t s)
 int64_t radix_scratch_words(int64_t n);
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2, int64_t n,
                       uint32_t* scratch, hipStream_t s);
-void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, uint32_t* cbounds,
-                    bool write_inst, hipStream_t s);
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec,
+                    uint32_t* cbounds, bool write_inst, hipStream_t s);
 void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
                    uint32_t* ids, hipStream_t s);
 void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
@@ -23,7 +23,10 @@ void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_
 void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s);
 // Buckets internal nodes by depth (syncs the stream once); level_begin gets
 // the offset of every depth in `order` plus a final end offset.
-int launch_levels(const NodeBuild* nodes, int n, uint32_t* ndepth, uint32_t* hist, int32_t* order,
+// key/key2: n words of 64 bits, val: n words, scratch: radix_scratch_words(n),
+// begin: 128 words (the per-depth offsets read back to the host)
+int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, uint32_t* val,
+                  int32_t* order, uint32_t* scratch, uint32_t* begin,
                   std::vector<int32_t>* level_begin, hipStream_t s);
 // Level-synchronous bottom-up boxes (no atomics, no inter-workgroup fences).
 void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
@@ -37,8 +40,11 @@ struct TraceArgs {
   GridK g;
   FrameK f;
   const float* dirs;      // rotated directions [N][3]
-  const float4* dirpre;   // direction ray setup [N + 1][3] (entry N: the sun)
+  const float4* dirpre;   // ray setup in processing order [N + 1][3] (entry N: the sun)
   const uint32_t* order;  // processing order of ray indices [N]
+  const float4* probe_org; // probe origins of the grid
+  uint64_t div_magic;      // ray index -> probe: (gidx * div_magic) >> div_shift == gidx / N
+  int32_t div_shift;
   const float4* hitpos;   // shadow: hit positions (shadow-ray origins) per ray
   const uint32_t* list;   // shadow: compacted hit rays
   const uint32_t* list_count;
@@ -62,17 +68,22 @@ struct TraceArgs {
 int trace_lanes_per_launch();
 int trace_stack_lds();
 void launch_trace(const TraceArgs& a, bool shadow, bool any_hit_rays, bool count, hipStream_t s);
-void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
-                        float4* dirpre, float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8,
-                        float* wfp8_scale, bool bake, hipStream_t s);
-// flag: nrays words; scratch: scan_scratch_words(nrays) words
+void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
+                        const uint32_t* order, float* dirs, float4* dirpre, float4* probe_org, float* w32, float* wsum, uint16_t* wbf16,
+                        uint8_t* wfp8, float* wfp8_scale, bool bake, hipStream_t s);
+struct RayDiv {
+  uint64_t magic;
+  int32_t shift;
+};
+// exact x / n for 0 <= x < nrays (nrays < 2^31, 1 <= n <= 4096)
+RayDiv ray_div_magic(int64_t nrays, int n);
 void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
                         int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
-                        uint32_t* count, uint32_t* flag, uint32_t* scratch, hipStream_t s);
-void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
-                  const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
-                  const float* atlas, int precision, void* rad, float* rad_scale, int num_probes,
-                  int probe_base, hipStream_t s);
+                        uint32_t* count, hipStream_t s);
+void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
+                  const ane_hit* hits, const float4* hitpos, const uint4* hitgeom,
+                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
+                  float* rad_scale, int num_probes, int probe_base, hipStream_t s);
 
 // conv.hip
 void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,

@@ -1,0 +1,416 @@
+// serve.hip -- persistent request loop for probe updates of many resident scenes.
+//
+// No counterpart in the reference (it updates one scene per frame from its
+// render loop, IrradianceProbeRayTracingPass.cpp:95-165).  A session is a
+// resident scene (its LBVH) plus a probe-grid context (its atlas and buffers);
+// sessions are admitted against an HBM budget.  Requests (one probe update of
+// one session with its frame constants) are queued from any host thread; a
+// dispatcher thread admits them continuously: whenever a slot is free and the
+// request's session has no update on the device, the request is launched on
+// the least-loaded of the server's HIP streams, so work of different sessions
+// overlaps on the device while each session's updates stay in submission order.
+// Completion is observed through one event per request.  A session's results
+// depend only on its own sequence of frames, so a request gives the same atlas
+// alone and in a mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ane_serve.h"
+
+namespace ane {
+int fail(int status, const char* fmt, ...);
+}
+using ane::fail;
+
+namespace {
+
+double now_ms()
+{
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Session {
+  const ane_scene* scene = nullptr;
+  ane_ddgi_ctx* ctx = nullptr;
+  int64_t bytes = 0;        // the context's device bytes (the scene is charged once, per scene)
+  int64_t rays = 0;         // primary rays of one full update
+  int32_t probes_y = 0;
+  bool busy = false;        // an update of this session is on the device
+  uint64_t submitted = 0, finished = 0;
+};
+
+struct Request {
+  int32_t session = -1;
+  ane_ddgi_frame frame{};
+  double t_submit = 0.0, t_launch = 0.0, t_done = 0.0;
+  int status = ANE_OK;
+  int stream = -1;
+  hipEvent_t done = nullptr;
+  bool launched = false, finished = false;
+  int64_t rays = 0;
+};
+
+}  // namespace
+
+struct ane_serve {
+  ane_serve_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::map<int32_t, Session> sessions;
+  struct SceneUse {
+    int refs = 0;
+    int64_t bytes = 0;
+  };
+  std::unordered_map<const ane_scene*, SceneUse> scenes;
+  std::unordered_map<uint64_t, Request> reqs;
+  std::deque<uint64_t> queue;        // submitted, not launched (FIFO)
+  std::vector<uint64_t> inflight;    // launched, not finished
+  std::vector<hipStream_t> streams;
+  std::vector<int> stream_load;
+  std::vector<hipEvent_t> event_pool;
+  int64_t used_bytes = 0;
+  int32_t next_session = 0;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  int device = 0;
+  std::thread worker;
+  // counters
+  uint64_t completed = 0, failed = 0;
+  int64_t rays_done = 0;
+  int32_t peak_inflight = 0;
+
+  void run();
+  int retire_locked(double t);
+};
+
+int ane_serve::retire_locked(double t)
+{
+  int n = 0;
+  for (size_t i = 0; i < inflight.size();) {
+    Request& r = reqs[inflight[i]];
+    const hipError_t q = hipEventQuery(r.done);
+    if (q == hipErrorNotReady) {
+      ++i;
+      continue;
+    }
+    r.finished = true;
+    r.t_done = t;
+    if (q != hipSuccess) {
+      r.status = ANE_ERR_DEVICE;
+      ++failed;
+    } else {
+      ++completed;
+      rays_done += r.rays;
+    }
+    event_pool.push_back(r.done);
+    r.done = nullptr;
+    stream_load[r.stream] -= 1;
+    Session& s = sessions[r.session];
+    s.busy = false;
+    s.finished += 1;
+    inflight[i] = inflight.back();
+    inflight.pop_back();
+    ++n;
+  }
+  return n;
+}
+
+void ane_serve::run()
+{
+  (void)hipSetDevice(device);
+  std::unique_lock<std::mutex> lk(mu);
+  struct Launch {
+    uint64_t ticket;
+    ane_ddgi_ctx* ctx;
+    const ane_scene* scene;
+    ane_ddgi_frame frame;
+    hipStream_t stream;
+    hipEvent_t done;
+  };
+  std::vector<Launch> batch;
+  for (;;) {
+    if (retire_locked(now_ms()) > 0) cv_done.notify_all();
+    // admit: oldest first, one update per session on the device at a time
+    batch.clear();
+    const int slots = cfg.max_inflight - (int)inflight.size();
+    if (slots > 0 && !queue.empty()) {
+      std::vector<int32_t> taken;
+      for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < slots;) {
+        Request& r = reqs[*it];
+        Session& s = sessions[r.session];
+        if (s.busy || std::find(taken.begin(), taken.end(), r.session) != taken.end()) {
+          ++it;
+          continue;
+        }
+        int best = 0;
+        for (int k = 1; k < (int)streams.size(); ++k)
+          if (stream_load[k] < stream_load[best]) best = k;
+        if (event_pool.empty()) {
+          hipEvent_t e = nullptr;
+          if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) break;
+          event_pool.push_back(e);
+        }
+        r.done = event_pool.back();
+        event_pool.pop_back();
+        r.stream = best;
+        stream_load[best] += 1;
+        s.busy = true;
+        taken.push_back(r.session);
+        batch.push_back({*it, s.ctx, s.scene, r.frame, streams[best], r.done});
+        it = queue.erase(it);
+      }
+    }
+    if (!batch.empty()) {
+      // launch outside the lock: submitters keep queueing meanwhile
+      lk.unlock();
+      std::vector<std::pair<uint64_t, int>> launched;
+      launched.reserve(batch.size());
+      for (Launch& b : batch) {
+        int st = ane_ddgi_update(b.ctx, b.scene, &b.frame, b.stream);
+        if (st == ANE_OK && hipEventRecord(b.done, b.stream) != hipSuccess) st = ANE_ERR_DEVICE;
+        launched.push_back({b.ticket, st});
+      }
+      lk.lock();
+      const double t = now_ms();
+      for (auto& l : launched) {
+        Request& r = reqs[l.first];
+        r.launched = true;
+        r.t_launch = t;
+        r.status = l.second;
+        if (l.second == ANE_OK) {
+          inflight.push_back(l.first);
+        } else {
+          // failed before reaching the device: finish now, free the slot
+          r.finished = true;
+          r.t_done = t;
+          ++failed;
+          event_pool.push_back(r.done);
+          r.done = nullptr;
+          stream_load[r.stream] -= 1;
+          Session& s = sessions[r.session];
+          s.busy = false;
+          s.finished += 1;
+        }
+      }
+      peak_inflight = std::max(peak_inflight, (int32_t)inflight.size());
+      cv_done.notify_all();
+      continue;
+    }
+    if (stop && queue.empty() && inflight.empty()) break;
+    if (inflight.empty()) {
+      cv_work.wait(lk);  // nothing on the device: sleep until a submission or close
+    } else {
+      // completions are polled; a submission wakes the loop at once
+      cv_work.wait_for(lk, std::chrono::microseconds(20));
+    }
+  }
+  cv_done.notify_all();
+}
+
+extern "C" int ane_serve_open(const ane_serve_config* cfg, ane_serve** out)
+{
+  if (!cfg || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  if (cfg->hbm_budget_bytes <= 0) return fail(ANE_ERR_INVALID_ARG, "hbm_budget_bytes must be positive");
+  if (cfg->max_inflight < 1 || cfg->max_inflight > 256)
+    return fail(ANE_ERR_INVALID_ARG, "max_inflight must be in 1..256");
+  if (cfg->num_streams < 1 || cfg->num_streams > 16)
+    return fail(ANE_ERR_INVALID_ARG, "num_streams must be in 1..16");
+  ane_serve* s = new ane_serve();
+  s->cfg = *cfg;
+  (void)hipGetDevice(&s->device);
+  for (int k = 0; k < cfg->num_streams; ++k) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
+      delete s;
+      return fail(ANE_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    s->streams.push_back(st);
+    s->stream_load.push_back(0);
+  }
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return ANE_OK;
+}
+
+extern "C" int ane_serve_add_session(ane_serve* s, const ane_scene* scene,
+                                     const ane_ddgi_config* config, int32_t* session)
+{
+  if (!s || !scene || !config || !session) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  *session = -1;
+  ane_scene_info si{};
+  int st = ane_scene_info_get(scene, &si);
+  if (st != ANE_OK) return st;
+  ane_ddgi_ctx* ctx = nullptr;
+  {
+    // refuse before allocating when even the scene alone does not fit
+    std::lock_guard<std::mutex> g(s->mu);
+    const int64_t scene_bytes = s->scenes.count(scene) ? 0 : si.device_bytes;
+    if (s->used_bytes + scene_bytes > s->cfg.hbm_budget_bytes)
+      return fail(ANE_ERR_ADMISSION_REFUSED, "HBM budget: %lld of %lld bytes in use, scene needs %lld",
+                  (long long)s->used_bytes, (long long)s->cfg.hbm_budget_bytes, (long long)scene_bytes);
+  }
+  st = ane_ddgi_create(config, &ctx);
+  if (st != ANE_OK) return st;
+  ane_ddgi_stats ds{};
+  ane_ddgi_stats_get(ctx, &ds);
+  std::lock_guard<std::mutex> g(s->mu);
+  const int64_t scene_bytes = s->scenes.count(scene) ? 0 : si.device_bytes;
+  const int64_t need = ds.device_bytes + scene_bytes;
+  if (s->used_bytes + need > s->cfg.hbm_budget_bytes) {
+    ane_ddgi_destroy(ctx);
+    return fail(ANE_ERR_ADMISSION_REFUSED, "HBM budget: %lld of %lld bytes in use, session needs %lld",
+                (long long)s->used_bytes, (long long)s->cfg.hbm_budget_bytes, (long long)need);
+  }
+  Session ses;
+  ses.scene = scene;
+  ses.ctx = ctx;
+  ses.bytes = ds.device_bytes;
+  ses.rays = (int64_t)config->probes_x * config->probes_y * config->probes_z * config->rays_per_probe;
+  ses.probes_y = config->probes_y;
+  const int32_t id = s->next_session++;
+  s->sessions[id] = ses;
+  ane_serve::SceneUse& su = s->scenes[scene];
+  su.refs += 1;
+  su.bytes = si.device_bytes;
+  s->used_bytes += need;
+  *session = id;
+  return ANE_OK;
+}
+
+// Waits until every submitted request of the session has finished.
+static void wait_session_idle(ane_serve* s, std::unique_lock<std::mutex>& lk, int32_t id)
+{
+  s->cv_done.wait(lk, [&] {
+    auto it = s->sessions.find(id);
+    return it == s->sessions.end() || it->second.finished == it->second.submitted;
+  });
+}
+
+extern "C" int ane_serve_remove_session(ane_serve* s, int32_t session)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (!s->sessions.count(session)) return fail(ANE_ERR_INVALID_ARG, "unknown session %d", session);
+  wait_session_idle(s, lk, session);
+  const Session ses = s->sessions[session];
+  s->sessions.erase(session);
+  s->used_bytes -= ses.bytes;
+  auto sc = s->scenes.find(ses.scene);
+  if (--sc->second.refs == 0) {
+    s->used_bytes -= sc->second.bytes;
+    s->scenes.erase(sc);
+  }
+  lk.unlock();
+  return ane_ddgi_destroy(ses.ctx);
+}
+
+extern "C" int ane_serve_submit(ane_serve* s, int32_t session, const ane_ddgi_frame* frame,
+                                uint64_t* ticket)
+{
+  if (!s || !frame || !ticket) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  if (s->stop) return fail(ANE_ERR_CANCELLED, "server is closing");
+  auto it = s->sessions.find(session);
+  if (it == s->sessions.end()) return fail(ANE_ERR_INVALID_ARG, "unknown session %d", session);
+  if (frame->layer_begin < 0 || frame->layer_count < 1 ||
+      frame->layer_begin + frame->layer_count > it->second.probes_y)
+    return fail(ANE_ERR_INVALID_ARG, "layers [%d, +%d) outside the grid of session %d",
+                frame->layer_begin, frame->layer_count, session);
+  const uint64_t t = s->next_ticket++;
+  Request r;
+  r.session = session;
+  r.frame = *frame;
+  r.t_submit = now_ms();
+  r.rays = it->second.rays / it->second.probes_y * frame->layer_count;
+  s->reqs[t] = r;
+  s->queue.push_back(t);
+  it->second.submitted += 1;
+  *ticket = t;
+  s->cv_work.notify_one();
+  return ANE_OK;
+}
+
+extern "C" int ane_serve_poll(ane_serve* s, uint64_t ticket, int32_t timeout_ms,
+                              ane_serve_result* out)
+{
+  if (!s || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [&] {
+    auto it = s->reqs.find(ticket);
+    return it == s->reqs.end() || it->second.finished;
+  };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lk, ready);
+  else if (timeout_ms > 0)
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(ANE_ERR_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  const Request& r = it->second;
+  if (!r.finished) return ANE_ERR_NOT_READY;
+  out->status = r.status;
+  out->session = r.session;
+  out->queue_ms = r.t_launch - r.t_submit;
+  out->latency_ms = r.t_done - r.t_submit;
+  out->rays = r.rays;
+  s->reqs.erase(it);
+  return ANE_OK;
+}
+
+extern "C" int ane_serve_session_atlas(ane_serve* s, int32_t session, float* host_rgba)
+{
+  if (!s || !host_rgba) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->sessions.find(session);
+  if (it == s->sessions.end()) return fail(ANE_ERR_INVALID_ARG, "unknown session %d", session);
+  wait_session_idle(s, lk, session);
+  ane_ddgi_ctx* ctx = s->sessions[session].ctx;
+  lk.unlock();
+  return ane_ddgi_atlas_get(ctx, host_rgba, nullptr);
+}
+
+extern "C" int ane_serve_stats_get(ane_serve* s, ane_serve_stats* out)
+{
+  if (!s || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  out->sessions = (int32_t)s->sessions.size();
+  out->queued = (int32_t)s->queue.size();
+  out->inflight = (int32_t)s->inflight.size();
+  out->peak_inflight = s->peak_inflight;
+  out->completed = s->completed;
+  out->failed = s->failed;
+  out->rays_done = s->rays_done;
+  out->used_bytes = s->used_bytes;
+  out->budget_bytes = s->cfg.hbm_budget_bytes;
+  return ANE_OK;
+}
+
+extern "C" int ane_serve_close(ane_serve* s)
+{
+  if (!s) return ANE_OK;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();  // drains the queue and the device work
+  for (auto& kv : s->sessions) ane_ddgi_destroy(kv.second.ctx);
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  for (auto& kv : s->reqs)
+    if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+  for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  delete s;
+  return ANE_OK;
+}

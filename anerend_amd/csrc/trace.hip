@@ -68,25 +68,39 @@ __device__ __forceinline__ void load_dirpre(RayPre& r, const float4* p, f3 o)
 // frame setup: rotated directions, their ray setup, the sun ray setup and the
 // convolution weights (one block per texel)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, f3 sun, const float* base,
-                                                     float* dirs, float4* dirpre, float* w32,
-                                                     float* wsum, uint16_t* wbf16, uint8_t* wfp8,
+__global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot, f3 sun,
+                                                     const float* base, const uint32_t* order,
+                                                     float* dirs, float4* dirpre,
+                                                     float4* probe_org, float* w32, float* wsum,
+                                                     uint16_t* wbf16, uint8_t* wfp8,
                                                      float* wfp8_scale, int32_t bake)
 {
   const int t = blockIdx.x;  // texel 0..63
+  // probe origins of the whole grid (rgen:60-84), read by ray generation and shading
+  const int P = g.nx * g.ny * g.nz;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < P; q += gridDim.x * blockDim.x) {
+    const int px = q % g.nx, pz = (q / g.nx) % g.nz, py = q / (g.nx * g.nz);
+    const f3 o = probe_origin(g, f, px, py, pz);
+    probe_org[q] = make_float4(o.x, o.y, o.z, 0.0f);
+  }
   const int n = g.rays;
   const f3 o = texel_dir(t);
   __shared__ float red[256];
   float s = 0.0f, mx = 0.0f;
+  if (t == 0) {
+    // ray setup in processing order: entry j belongs to ray order[j]
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      RayPre rp;
+      ray_setup(rp, mk(0.0f, 0.0f, 0.0f), rot3(rot.m, ld3(base + 3 * order[j])));
+      store_dirpre(dirpre + 3 * j, rp);
+    }
+  }
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
     f3 d = rot3(rot.m, ld3(base + 3 * r));
     if (t == 0) {
       dirs[3 * r + 0] = d.x;
       dirs[3 * r + 1] = d.y;
       dirs[3 * r + 2] = d.z;
-      RayPre rp;
-      ray_setup(rp, mk(0.0f, 0.0f, 0.0f), d);
-      store_dirpre(dirpre + 3 * r, rp);
     }
     float w = fmaxf(0.0f, dot(d, o));
     if (!(w > 1e-6f)) w = 0.0f;
@@ -131,15 +145,33 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, Mat3 rot, f3 sun, 
   }
 }
 
-void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs, float* dirs,
-                        float4* dirpre, float* w32, float* wsum, uint16_t* wbf16, uint8_t* wfp8,
-                        float* wfp8_scale, bool bake, hipStream_t s)
+void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
+                        const uint32_t* order, float* dirs, float4* dirpre, float4* probe_org, float* w32, float* wsum, uint16_t* wbf16,
+                        uint8_t* wfp8, float* wfp8_scale, bool bake, hipStream_t s)
 {
   Mat3 rot;
   for (int i = 0; i < 16; ++i) rot.m[i] = f.rot[i];
   const f3 sun = normalize(mk(-f.light_dir[0], -f.light_dir[1], -f.light_dir[2]));
-  hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, rot, sun, base_dirs, dirs, dirpre,
-                     w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake);
+  hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, f, rot, sun, base_dirs, order,
+                     dirs, dirpre, probe_org, w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake);
+}
+
+// gidx / n for gidx < 2^31 and 1 <= n <= 4096: floor(gidx * ceil(2^44 / n) / 2^44) is exact
+// (the multiplier error times gidx stays below 2^-13 < 1/n)
+// Division of a ray index x < nrays by n as (x * m) >> k, m = ceil(2^k / n).
+// With e = m n - 2^k < n, x m / 2^k = x / n + x e / (n 2^k) and the floor is
+// exact when x e / 2^k < 1, which k = bit_length(nrays * n) guarantees
+// (x e < nrays n <= 2^k).  No overflow: x m < nrays (2^k / n + 1)
+// <= 2 nrays^2 + nrays < 2^63 for nrays < 2^31.
+RayDiv ray_div_magic(int64_t nrays, int n)
+{
+  const uint64_t bound = (uint64_t)nrays * (uint64_t)n;
+  int k = 0;
+  while (k < 63 && (1ull << k) <= bound) ++k;
+  RayDiv d;
+  d.shift = k;
+  d.magic = ((1ull << k) + (uint64_t)n - 1) / (uint64_t)n;
+  return d;
 }
 
 // ---------------------------------------------------------------------------
@@ -148,30 +180,23 @@ void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
 // ---------------------------------------------------------------------------
 struct HitGeom {
   uint32_t inst;
-  uint32_t mesh;
   uint32_t v[3];  // absolute vertex indices
 };
 
-__device__ __forceinline__ HitGeom hit_geom(const SceneK& sc, int32_t id)
+// Instance, absolute vertex indices (rchit:27-38) and world position of a hit
+// on triangle id, from the triangle's 64-byte record written by k_flatten.
+__device__ __forceinline__ f3 hit_geom_pos(const SceneK& sc, int32_t id, float u, float v,
+                                           HitGeom& h)
 {
-  HitGeom h;
-  h.inst = sc.tri_inst[id];
-  const ane_instance& in = sc.inst[h.inst];
-  h.mesh = mesh_of(in.custom_index_and_mask);
-  const ane_mesh& m = sc.meshes[h.mesh];
-  uint32_t prim = (uint32_t)id - sc.inst_tri_off[h.inst];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) h.v[j] = m.vertex_offset + sc.indices[m.index_offset + 3 * prim + j];
-  return h;
-}
-
-__device__ __forceinline__ f3 hit_pos(const SceneK& sc, const HitGeom& h, float u, float v)
-{
-  f3 b = mk(1.0f - u - v, u, v);
-  f3 p0 = ld3(sc.vertices + (int64_t)h.v[0] * 24);
-  f3 p1 = ld3(sc.vertices + (int64_t)h.v[1] * 24);
-  f3 p2 = ld3(sc.vertices + (int64_t)h.v[2] * 24);
-  f3 pos = add(add(muls(p0, b.x), muls(p1, b.y)), muls(p2, b.z));
+  const float4* r = sc.tri_rec + 4 * (int64_t)id;
+  const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+  h.inst = __float_as_uint(d.x);
+  h.v[0] = __float_as_uint(a.w);
+  h.v[1] = __float_as_uint(b.w);
+  h.v[2] = __float_as_uint(c.w);
+  const f3 bw = mk(1.0f - u - v, u, v);
+  const f3 pos = add(add(muls(mk(a.x, a.y, a.z), bw.x), muls(mk(b.x, b.y, b.z), bw.y)),
+                     muls(mk(c.x, c.y, c.z), bw.z));
   return xform_point(sc.inst[h.inst].transform, pos);
 }
 
@@ -233,15 +258,12 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
             const float4 p = a.hitpos[out];
             load_dirpre(r, a.dirpre + 3 * n, mk(p.x, p.y, p.z));
           } else {
-            const int ql = (int)(gidx / (uint32_t)n);
-            const int j = (int)(gidx - (uint32_t)ql * (uint32_t)n);
-            const int i = (int)a.order[j];
-            out = ql * n + i;
-            const int q = a.probe_base + ql;
-            const int px = q % a.g.nx;
-            const int pz = (q / a.g.nx) % a.g.nz;
-            const int py = q / (a.g.nx * a.g.nz);
-            load_dirpre(r, a.dirpre + 3 * i, probe_origin(a.g, a.f, px, py, pz));
+            // independent loads: processing-order ray setup, probe origin, output slot
+            const uint32_t ql = (uint32_t)(((uint64_t)gidx * a.div_magic) >> a.div_shift);
+            const uint32_t j = gidx - ql * (uint32_t)n;
+            const float4 po = a.probe_org[a.probe_base + (int)ql];
+            load_dirpre(r, a.dirpre + 3 * j, mk(po.x, po.y, po.z));
+            out = (int32_t)ql * n + (int)a.order[j];
           }
           tbest = tmax0;
           bid = -1;
@@ -395,56 +417,57 @@ void launch_trace(const TraceArgs& a, bool shadow, bool rays, bool count, hipStr
 }
 
 // ---------------------------------------------------------------------------
-// hit positions + order-preserving compaction of the shadow-ray list: the list
-// keeps the processing order (probe-major, direction-sorted), so consecutive
-// shadow rays start from neighbouring hit points
+// hit positions + compaction of the shadow-ray list: one block compacts 256
+// consecutive rays of the processing order (probe-major, direction-sorted)
+// with a block-local prefix sum and reserves its list range with one atomic,
+// so each 256-ray group stays together and in order
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hit_pos(SceneK sc, const ane_hit* hits,
-                                                 const uint32_t* order, int n, int nrays,
-                                                 float4* hitpos, uint4* hitgeom, uint8_t* shadow,
-                                                 uint32_t* flag)
+__global__ __launch_bounds__(256) void k_hit_compact(SceneK sc, const ane_hit* hits,
+                                                     const uint32_t* order, int n, int nrays,
+                                                     float4* hitpos, uint4* hitgeom, uint8_t* shadow,
+                                                     uint32_t* list, uint32_t* count)
 {
+  __shared__ uint32_t wcount[4];
+  __shared__ uint32_t base;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nrays) return;
-  const int ql = g / n;
-  const int out = ql * n + (int)order[g - ql * n];
-  const ane_hit h = hits[out];
-  const bool hit = h.tri >= 0;
-  if (hit) {
-    HitGeom hg = hit_geom(sc, h.tri);
-    const f3 p = hit_pos(sc, hg, h.u, h.v);
-    hitpos[out] = make_float4(p.x, p.y, p.z, 1.0f);
-    hitgeom[out] = make_uint4(hg.inst, hg.v[0], hg.v[1], hg.v[2]);
-  } else {
-    shadow[out] = 255;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  bool hit = false;
+  int out = 0;
+  if (g < nrays) {
+    const int ql = g / n;
+    out = ql * n + (int)order[g - ql * n];
+    const ane_hit h = hits[out];
+    hit = h.tri >= 0;
+    if (hit) {
+      HitGeom hg;
+      const f3 p = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
+      hitpos[out] = make_float4(p.x, p.y, p.z, 1.0f);
+      hitgeom[out] = make_uint4(hg.inst, hg.v[0], hg.v[1], hg.v[2]);
+    } else {
+      shadow[out] = 255;
+    }
   }
-  flag[g] = hit ? 1u : 0u;
-}
-
-__global__ __launch_bounds__(256) void k_hit_scatter(const ane_hit* hits, const uint32_t* order, int n,
-                                                     int nrays, const uint32_t* offs, uint32_t* list,
-                                                     uint32_t* count)
-{
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nrays) return;
-  const int ql = g / n;
-  const int out = ql * n + (int)order[g - ql * n];
-  const bool hit = hits[out].tri >= 0;
-  if (hit) list[offs[g]] = (uint32_t)out;
-  if (g == nrays - 1) *count = offs[g] + (hit ? 1u : 0u);
+  const uint64_t hm = __ballot(hit);
+  if (lane == 0) wcount[wave] = (uint32_t)__popcll(hm);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    base = total ? atomicAdd(count, total) : 0u;
+  }
+  __syncthreads();
+  uint32_t pre = base;
+  for (int w = 0; w < wave; ++w) pre += wcount[w];
+  if (hit) list[pre + (uint32_t)__popcll(hm & lt)] = (uint32_t)out;
 }
 
 void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
                         int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
-                        uint32_t* count, uint32_t* flag, uint32_t* scratch, hipStream_t s)
+                        uint32_t* count, hipStream_t s)
 {
   if (nrays <= 0) return;
-  const unsigned blocks = (unsigned)((nrays + 255) / 256);
-  hipLaunchKernelGGL(k_hit_pos, dim3(blocks), dim3(256), 0, s, sc, hits, order, n, nrays, hitpos,
-                     hitgeom, shadow, flag);
-  exclusive_scan(flag, nrays, scratch, s);
-  hipLaunchKernelGGL(k_hit_scatter, dim3(blocks), dim3(256), 0, s, hits, order, n, nrays,
-                     (const uint32_t*)flag, list, count);
+  hipLaunchKernelGGL(k_hit_compact, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, sc, hits,
+                     order, n, nrays, hitpos, hitgeom, shadow, list, count);
 }
 
 // ---------------------------------------------------------------------------
@@ -668,7 +691,8 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
 }
 
 template <int PREC, int TB>
-__global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, const ane_hit* hits,
+__global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
+                                              const ane_hit* hits,
                                               const float4* hitpos, const uint4* hitgeom,
                                               const uint8_t* shadow, const float4* atlas, void* rad,
                                               float* rad_scale, int probe_base)
@@ -678,8 +702,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
   const int ql = blockIdx.x;
   const int q = probe_base + ql;
   const int n = g.rays;
-  const int px = q % g.nx, pz = (q / g.nx) % g.nz, py = q / (g.nx * g.nz);
-  const f3 origin = probe_origin(g, f, px, py, pz);
+  const float4 po = probe_org[q];
+  const f3 origin = mk(po.x, po.y, po.z);
   float mx = 0.0f;
   for (int i = threadIdx.x; i < n; i += TB) {
     const int64_t idx = (int64_t)ql * n + i;
@@ -725,35 +749,36 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
 }
 
 template <int TB>
-static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
+static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
+                         const ane_hit* hits,
                          const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
                          const float4* a4, int precision, void* rad, float* rad_scale,
                          int num_probes, int probe_base, hipStream_t s)
 {
   if (precision == ANE_PRECISION_FP32)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP32, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
-                       hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+                       probe_org, hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
   else if (precision == ANE_PRECISION_BF16)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_BF16, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
-                       hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+                       probe_org, hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
   else
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP8, TB>), dim3(num_probes), dim3(TB),
-                       (size_t)12 * g.rays, s, sc, g, f, hits, hitpos, hitgeom, shadow, a4, rad,
+                       (size_t)12 * g.rays, s, sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, rad,
                        rad_scale, probe_base);
 }
 
-void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const ane_hit* hits,
-                  const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
-                  const float* atlas, int precision, void* rad, float* rad_scale, int num_probes,
-                  int probe_base, hipStream_t s)
+void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
+                  const ane_hit* hits, const float4* hitpos, const uint4* hitgeom,
+                  const uint8_t* shadow, const float* atlas, int precision, void* rad,
+                  float* rad_scale, int num_probes, int probe_base, hipStream_t s)
 {
   if (num_probes <= 0) return;
   const float4* a4 = reinterpret_cast<const float4*>(atlas);
   if (g.rays >= 512)
-    shade_launch<512>(sc, g, f, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+    shade_launch<512>(sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
   else
-    shade_launch<256>(sc, g, f, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+    shade_launch<256>(sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
 }
 

@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .types import (FLAG_DDGI, FLAG_DDGI_MULTI, INSTANCE_DTYPE, MATERIAL_DTYPE, MESH_DTYPE,
-                    VERTEX_WORDS, Config, Frame)
+                    PRECISION_FP8, VERTEX_WORDS, Config, Frame)
 
 _MULT = np.uint64(6364136223846793005)
 
@@ -420,3 +420,105 @@ def workload(name):
         cfg = production_config()
         return terrain_scene(2), cfg, make_frame(2, camera=(0.5, 12.0, 0.5), config=cfg)
     raise KeyError(name)
+
+
+SCENE_MAGIC = b"ANESCN01"
+
+
+def save_scene(scene, path):
+    """Write a scene file: magic, int64 vertex and index counts, int32 mesh,
+    material and instance counts and a zero word, then the five arrays in the
+    byte layouts of the C-ABI (read by the C++ frame driver)."""
+    arrs = [np.ascontiguousarray(a) for a in
+            (scene.vertices, scene.indices, scene.meshes, scene.materials, scene.instances)]
+    with open(path, "wb") as fp:
+        fp.write(SCENE_MAGIC)
+        fp.write(np.array([len(arrs[0]), len(arrs[1])], "<i8").tobytes())
+        fp.write(np.array([len(arrs[2]), len(arrs[3]), len(arrs[4]), 0], "<i4").tobytes())
+        for a in arrs:
+            fp.write(a.tobytes())
+
+
+# ---------------------------------------------------------------------------
+# dynamic large scene: moving instances, camera crossing a cell every 3 frames
+# ---------------------------------------------------------------------------
+DYNAMIC_DT = 1.0 / 30.0   # 30 Hz update cadence
+
+
+def dynamic_config(precision=PRECISION_FP8):
+    """64 x 32 x 64 probes (step 1, 2, 1 m), 256 rays, FP8 convolution."""
+    return Config.make((64, 32, 64), step=(1.0, 2.0, 1.0), rays=256, hysteresis=0.96,
+                       precision=precision)
+
+
+def dynamic_scene(seed=5, cells=600, num_instances=20000, spread=60.0):
+    """About 4M triangles: a 600^2-cell terrain (720k triangles) plus 20,000
+    instances, half boxes (12 tris) and half subdivision-2 icospheres (320 tris),
+    within +-60 m of the origin.  Returns (scene, velocities): every instance but
+    the terrain moves along a seeded straight line, U[-1, 1] m/s in x and z."""
+    scene = terrain_scene(seed, cells=cells, num_instances=num_instances, ico_subdiv=2,
+                          spread=spread)
+    g = PCG32(seed, np.arange(num_instances) + 300000)
+    vel = np.zeros((num_instances + 1, 3), np.float64)
+    vel[1:, 0] = g.uniform(-1.0, 1.0)
+    vel[1:, 2] = g.uniform(-1.0, 1.0)
+    return scene, vel
+
+
+def dynamic_instances(scene, vel, frame):
+    """Instance records at ``frame``: translation = start + vel * frame * dt
+    (binary64, rounded once to float)."""
+    inst = scene.instances.copy()
+    T = scene.instances["transform"].astype(np.float64).reshape(-1, 3, 4)
+    T[:, :, 3] += vel * (frame * DYNAMIC_DT)
+    inst["transform"] = T.reshape(-1, 12).astype(np.float32)
+    return inst
+
+
+def dynamic_camera(frame, start_x=-50.5, height=12.0, z=0.5):
+    """Camera moving +1/3 m per frame along x: it crosses an integer cell every
+    3 frames (6 across the 2 m-wide cell around 0, where int() truncates)."""
+    return (float(np.float32(start_x + frame / 3.0)), height, z)
+
+
+def dynamic_frame(seed, frame, config):
+    return make_frame(seed, camera=dynamic_camera(frame), config=config, iteration=frame)
+
+
+# ---------------------------------------------------------------------------
+# serving mix: many resident scenes of uneven sizes
+# ---------------------------------------------------------------------------
+SERVE_GRIDS = [(4, 4, 4), (8, 8, 8), (16, 8, 16), (32, 16, 32)]
+SERVE_RAYS = [128, 256, 512]
+
+
+def serving_specs(seed=4, num_scenes=64, min_tris=1e4, max_tris=2e6):
+    """Per scene: (target triangles, terrain cells, instances, grid, rays).
+    Triangle counts log-uniform on [min_tris, max_tris], half terrain and half
+    box / icosphere instances (46 triangles on average); grid and rays uniform
+    over SERVE_GRIDS and SERVE_RAYS."""
+    g = PCG32(seed, np.arange(num_scenes) + 700000)
+    target = g.log_uniform(min_tris, max_tris)
+    gi = (g.next_u32() % np.uint32(len(SERVE_GRIDS))).astype(int)
+    gr = (g.next_u32() % np.uint32(len(SERVE_RAYS))).astype(int)
+    out = []
+    for k in range(num_scenes):
+        t = float(target[k])
+        cells = max(8, int(round(math.sqrt(t / 4.0))))
+        ninst = max(0, int(round((t - 2.0 * cells * cells) / 46.0)))
+        out.append((t, cells, ninst, SERVE_GRIDS[gi[k]], SERVE_RAYS[gr[k]]))
+    return out
+
+
+def serving_scene(seed, k, spec):
+    _, cells, ninst, _, _ = spec
+    return terrain_scene(seed * 1000 + k, cells=cells, num_instances=ninst)
+
+
+def serving_config(spec, precision):
+    grid, rays = spec[3], spec[4]
+    return Config.make(grid, step=(1.0, 2.0, 1.0), rays=rays, hysteresis=0.96, precision=precision)
+
+
+def serving_frame(seed, k, iteration, config):
+    return make_frame(seed * 1000 + k, camera=(0.5, 12.0, 0.5), config=config, iteration=iteration)

@@ -36,34 +36,321 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp8"])
-    p.add_argument("--workload", default="terrain1m")
-    p.add_argument("--cpu-layers", type=int, default=2,
-                   help="probe layers of the same workload traced by the CPU oracle baseline")
+    p.add_argument("--workload", default="terrain1m",
+                   choices=["terrain1m", "room8", "dynamic4m", "serve64"])
+    p.add_argument("--requests", type=int, default=2000, help="serve64: open-loop requests")
+    p.add_argument("--burst", type=int, default=256, help="serve64: requests of the saturation burst")
+    p.add_argument("--inflight", type=int, default=8, help="serve64: requests on the device at once")
+    p.add_argument("--streams", type=int, default=4, help="serve64: HIP streams of the server")
+    p.add_argument("--frames", type=int, default=300, help="frames of the dynamic4m workload")
+    p.add_argument("--rebuild-every", type=int, default=1,
+                   help="dynamic4m: full LBVH rebuild every K frames, refit in between (0 = refit only)")
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="CPU oracle baseline: repeat whole updates for at least this long")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-conv-study", action="store_true",
+                   help="skip the FP32/BF16/FP8 convolution timing and FP64 error study")
     return p.parse_args()
 
 
-def cpu_baseline(scene, cfg, frame, layers):
-    """The oracle (C, OpenMP) on a bounded sample of the same workload."""
+def cpu_baseline(scene, cfg, frame_at, min_seconds):
+    """The oracle (C, OpenMP over rays and probes) on a bounded sample of the same
+    workload: whole probe updates, repeated until ``min_seconds`` have passed."""
     from oracle import oracle as O
     t0 = time.time()
     osc = O.OracleScene(scene)
     build_s = time.time() - t0
-    fr = frame.copy()
-    fr.layer_begin, fr.layer_count = 0, layers
     atlas = O.new_atlas(cfg)
+    per = cfg.probes_x * cfg.probes_y * cfg.probes_z * cfg.rays_per_probe
+    n = 0
     t0 = time.time()
-    osc.update(cfg, fr, atlas, want_rays=False)
-    dt = time.time() - t0
-    rays = layers * cfg.probes_x * cfg.probes_z * cfg.rays_per_probe
-    return {"value": rays / dt, "unit": "probe rays/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{layers} of {cfg.probes_y} probe layers ({rays} rays: trace, shadow, shade, "
-                      f"convolve, blend) of the same workload in {dt:.2f} s; CPU LBVH build "
-                      f"{build_s:.2f} s not counted"}
+    while True:
+        osc.update(cfg, frame_at(n), atlas, want_rays=False)
+        n += 1
+        dt = time.time() - t0
+        if dt >= min_seconds or n >= 100:
+            break
+    return {"value": n * per / dt, "unit": "probe rays/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{n} full probe updates of the same workload ({n * per} rays: trace, shadow, "
+                      f"shade, convolve, blend) in {dt:.2f} s; CPU LBVH build {build_s:.2f} s not "
+                      f"counted"}
+
+
+def texel_dirs64():
+    """Octahedral directions of the 8x8 tile texel centres in float64, t = ty * 8 + tx
+    (probe_conv.comp:37-46, octahedron_helpers.glsl:25-34: y and z swapped)."""
+    u = (2.0 * np.arange(8) + 1.0) / 16.0 * 2.0 - 1.0
+    ox, oy = (a.ravel() for a in np.meshgrid(u, u))
+    z = 1.0 - np.abs(ox) - np.abs(oy)
+    nx = np.where(z < 0, (1.0 - np.abs(oy)) * np.where(ox >= 0, 1.0, -1.0), ox)
+    ny = np.where(z < 0, (1.0 - np.abs(ox)) * np.where(oy >= 0, 1.0, -1.0), oy)
+    v = np.stack([nx, z, ny], 1)
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def conv_study(cfg0, frame, reps=20):
+    """Convolution alone at the production shape (16,384 probes x 512 rays x RGB) on
+    seeded radiance, log-uniform on [1e-3, 50] (seed 3): FP32 / BF16 (MFMA) / FP8
+    (MFMA) kernel times and their error against a float64 GEMM of the same
+    weights.  Hysteresis 0, so the atlas interior is the convolution itself."""
+    from anerend_amd import native
+    from anerend_amd.types import Config, PRECISION_BF16, PRECISION_FP8, PRECISION_FP32
+    P = cfg0.probes_x * cfg0.probes_y * cfg0.probes_z
+    N = cfg0.rays_per_probe
+    rng = np.random.default_rng(3)
+    rad = np.exp(rng.uniform(np.log(1e-3), np.log(50.0), (P, 3, N))).astype(np.float32)
+    fr = frame.copy()
+    fr.layer_begin, fr.layer_count = 0, cfg0.probes_y
+    d = native.ray_directions(N, list(fr.rotation)).astype(np.float64)
+    W = np.maximum(0.0, texel_dirs64() @ d.T)
+    W[W <= 1e-6] = 0.0
+    Wn = W / W.sum(1, keepdims=True)
+    ref = (rad.reshape(P * 3, N).astype(np.float64) @ Wn.T).reshape(P, 3, 64)
+    nx, ny, nz = cfg0.probes_x, cfg0.probes_y, cfg0.probes_z
+    flops = 2.0 * 64 * N * 3 * P
+    peaks = {"fp32": 157.3e12, "bf16": 2.5e15, "fp8": 5.0e15}
+    rad_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}
+    out = {}
+    for name, prec in (("fp32", PRECISION_FP32), ("bf16", PRECISION_BF16), ("fp8", PRECISION_FP8)):
+        cfg = Config.make((nx, ny, nz), step=tuple(cfg0.probe_step), rays=N, hysteresis=0.0,
+                          precision=prec)
+        grid = native.ProbeGrid(cfg)
+        grid.set_radiance(fr, rad)
+        grid.convolve(fr)
+        a = grid.atlas.reshape(ny, nz, 10, nx, 10, 4)[:, :, 1:9, :, 1:9, :3]
+        got = a.transpose(0, 1, 3, 2, 4, 5).reshape(P, 64, 3).transpose(0, 2, 1)
+        rel = np.abs(got - ref) / ref
+        native.synchronize()
+        grid.reset_stats()
+        for _ in range(reps):
+            grid.convolve(fr)
+        native.synchronize()
+        ms = grid.stats()["mean_ms"]["convolve"]
+        hbm = P * 3 * N * rad_bytes[name] + 2 * P * 64 * 16
+        out[name] = {"ms": round(ms, 5), "max_rel_err_vs_fp64": float(f"{rel.max():.3e}"),
+                     "rms_rel_err_vs_fp64": float(f"{np.sqrt((rel ** 2).mean()):.3e}"),
+                     "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                     "compute_frac": round(flops / (ms * 1e-3) / peaks[name], 4),
+                     "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        grid.close()
+        log(f"[bench] conv {name}: {json.dumps(out[name])}")
+    out["bf16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["bf16"]["ms"], 3)
+    out["fp8_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp8"]["ms"], 3)
+    out["shape"] = f"{P} probes x {N} rays x RGB, GEMM 64 x {N} x {3 * P}"
+    return out
+
+
+def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
+    """Error of one convolution of ``grid_conv`` (its precision) against a float64
+    GEMM of the FP32 radiance of the same frame traced by ``grid_ref`` (FP32) from
+    the same atlas.  E is recovered from the blend: E = (new - h old) / (1 - h)."""
+    from anerend_amd import native
+    P = cfg.probes_x * cfg.probes_y * cfg.probes_z
+    N = cfg.rays_per_probe
+    old = grid_conv.atlas
+    grid_ref.atlas = old
+    grid_ref.trace(scene_dev, fr)
+    rad = grid_ref.radiance(fr).astype(np.float64)
+    grid_conv.update(scene_dev, fr)
+    new = grid_conv.atlas
+    h = float(np.float32(cfg.hysteresis))
+    nx, ny, nz = cfg.probes_x, cfg.probes_y, cfg.probes_z
+
+    def interior(a):
+        a = a.astype(np.float64).reshape(ny, nz, 10, nx, 10, 4)[:, :, 1:9, :, 1:9, :3]
+        return a.transpose(0, 1, 3, 2, 4, 5).reshape(P, 64, 3).transpose(0, 2, 1)
+
+    E = (interior(new) - h * interior(old)) / (1.0 - h)
+    d = native.ray_directions(N, list(fr.rotation)).astype(np.float64)
+    W = np.maximum(0.0, texel_dirs64() @ d.T)
+    W[W <= 1e-6] = 0.0
+    Wn = W / W.sum(1, keepdims=True)
+    ref = (rad.reshape(P * 3, N) @ Wn.T).reshape(P, 3, 64)
+    rel = np.abs(E - ref) / np.maximum(np.abs(ref), 1e-3)
+    return {"max_rel": float(f"{rel.max():.3e}"), "rms_rel": float(f"{np.sqrt((rel ** 2).mean()):.3e}"),
+            "floor": 1e-3}
+
+
+def bench_dynamic(args):
+    """Dynamic large scene: 64 x 32 x 64 probes x 256 rays over ~4M triangles whose
+    20,000 instances move every frame (refit), a camera crossing an integer cell
+    every 3 frames (translation), one full probe update per frame with FP8
+    convolution.  Prints one JSON line (ms per frame)."""
+    from anerend_amd import native, scenegen
+    from anerend_amd.types import PRECISION_FP32
+    frames = args.frames
+    cfg = scenegen.dynamic_config()
+    log("[bench] building the dynamic scene")
+    scene, vel = scenegen.dynamic_scene()
+    insts = [scenegen.dynamic_instances(scene, vel, f) for f in range(frames + 1)]
+    frs = [scenegen.dynamic_frame(5, f, cfg) for f in range(frames + 1)]
+    ds = native.DeviceScene(scene)
+    info = ds.info()
+    log(f"[bench] {scene.num_tris} triangles, LBVH {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
+    grid = native.ProbeGrid(cfg)
+    for f in range(args.warmup):  # warm-up frames outside the timed run, then restart the path
+        grid.update(ds, frs[f])
+    native.synchronize()
+    grid.translation_state((0, 0))
+    grid.reset_stats()
+    moves = 0
+    refit_ms = 0.0
+    log(f"[bench] timing {frames} frames")
+    t0 = time.perf_counter()
+    K = args.rebuild_every
+    for f in range(frames):
+        if K > 0 and f % K == 0:
+            ds.rebuild(insts[f])
+        else:
+            ds.refit(insts[f])
+        moves += bool(grid.translate(frs[f].camera_pos[:3]))
+        grid.update(ds, frs[f])
+    native.synchronize()
+    t1 = time.perf_counter()
+    refit_ms = ds.info()["build_ms"]
+    st = grid.stats()
+    ms = (t1 - t0) * 1000.0 / frames
+    log(f"[bench] {ms:.3f} ms/frame, {moves} translations, stages "
+        f"{json.dumps({k: round(v, 4) for k, v in st['mean_ms'].items()})}")
+    # FP8 convolution error against FP64 on the frame after the run
+    ds.refit(insts[frames])
+    ref_grid = native.ProbeGrid(scenegen.dynamic_config(PRECISION_FP32))
+    err = fp64_conv_error(grid, ref_grid, ds, frs[frames], cfg)
+    log(f"[bench] FP8 error vs FP64: {err}")
+    P = cfg.probes_x * cfg.probes_y * cfg.probes_z
+    out = {
+        "metric": "ms per frame (dynamic 4M-triangle scene: refit + translation + full probe update)",
+        "value": round(ms, 4), "unit": "ms/frame", "n_gpus": 1, "steps": frames,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generator, seed 5)",
+        "config": {"workload": "dynamic4m: 64x32x64 probes x 256 rays, 600^2 terrain + 20k moving "
+                               "box/icosphere instances, camera +1/3 m per frame",
+                   "rebuild_every": K,
+                   "probes": P, "rays_per_probe": cfg.rays_per_probe,
+                   "triangles": scene.num_tris, "conv_precision": "fp8",
+                   "probe_rays_per_s": round(P * cfg.rays_per_probe / (ms / 1000.0), 1)},
+        "translations": moves,
+        "bvh_update": (f"rebuild every {K} frames, refit in between" if K > 1 else
+                       "rebuild every frame" if K == 1 else "refit every frame"),
+        "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items()},
+        "last_bvh_update_ms": round(refit_ms, 4),
+        "fp8_error_vs_fp64": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Serving: 64 resident scenes of uneven sizes (10k-2M triangles, grids 4^3 to
+    32x16x32, 128-512 rays), BF16 convolution, requests arriving as a Poisson
+    process at 200/s (open loop), admitted continuously under a 200 GB budget.
+    Reports p50/p99 latency and aggregate rays/s, then the throughput of a burst
+    of requests through the server against the plain per-scene FP32 path."""
+    from anerend_amd import native, scenegen, serving
+    from anerend_amd.types import PRECISION_BF16, PRECISION_FP32
+    specs = scenegen.serving_specs()
+    log(f"[bench] building {len(specs)} scenes")
+    dss, tris = [], 0
+    for k, sp in enumerate(specs):
+        sc = scenegen.serving_scene(4, k, sp)
+        tris += sc.num_tris
+        dss.append(native.DeviceScene(sc))
+    cfgs = [scenegen.serving_config(sp, PRECISION_BF16) for sp in specs]
+    budget = 200 << 30
+    srv = serving.Server(hbm_budget_bytes=budget, max_inflight=args.inflight, num_streams=args.streams)
+    sids = [srv.add_session(d, c) for d, c in zip(dss, cfgs)]
+    log(f"[bench] {tris} triangles resident, {srv.stats()['used_bytes'] / 2**30:.2f} GiB admitted")
+    R, rate = args.requests, 200.0
+    rng = np.random.default_rng(4)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, R))
+    which = rng.integers(0, len(specs), R)
+    frames = [scenegen.serving_frame(4, int(k), i, cfgs[k]) for i, k in enumerate(which)]
+    rays = np.array([np.prod(specs[k][3]) * specs[k][4] for k in which], np.int64)
+    # warm-up: one update per session, outside the measurement
+    for t in [srv.submit(sid, scenegen.serving_frame(4, k, 10 ** 6, cfgs[k]))
+              for k, sid in enumerate(sids)]:
+        srv.poll(t, -1)
+    log(f"[bench] open loop: {R} requests at {rate:.0f}/s")
+    tickets = []
+    t0 = time.perf_counter()
+    for i in range(R):
+        target = t0 + arrive[i]
+        while True:
+            dt = target - time.perf_counter()
+            if dt <= 0:
+                break
+            time.sleep(dt if dt > 0.002 else 0.0)
+        tickets.append(srv.submit(sids[which[i]], frames[i]))
+    res = [srv.poll(t, -1) for t in tickets]
+    t1 = time.perf_counter()
+    assert all(r["status"] == 0 for r in res)
+    lat = np.array([r["latency_ms"] for r in res])
+    que = np.array([r["queue_ms"] for r in res])
+    st = srv.stats()
+    open_loop = {"requests": R, "rate_per_s": rate, "p50_ms": round(float(np.percentile(lat, 50)), 4),
+                 "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                 "mean_ms": round(float(lat.mean()), 4), "max_ms": round(float(lat.max()), 4),
+                 "queue_p99_ms": round(float(np.percentile(que, 99)), 4),
+                 "rays_per_s": round(float(rays.sum()) / (t1 - t0), 1),
+                 "peak_inflight": st["peak_inflight"], "wall_s": round(t1 - t0, 3)}
+    log(f"[bench] open loop {json.dumps(open_loop)}")
+    # saturation: a burst of B requests at once through the server (BF16) ...
+    B = min(args.burst, R)
+    native.synchronize()
+    t0 = time.perf_counter()
+    tk = [srv.submit(sids[which[i]], frames[i]) for i in range(B)]
+    for t in tk:
+        srv.poll(t, -1)
+    t1 = time.perf_counter()
+    serve_rays_s = float(rays[:B].sum()) / (t1 - t0)
+    srv.close()
+    # ... and the same requests through the plain per-scene FP32 path, one stream
+    plain = {}
+    for k in sorted(set(int(x) for x in which[:B])):
+        plain[k] = native.ProbeGrid(scenegen.serving_config(specs[k], PRECISION_FP32))
+        plain[k].update(dss[k], scenegen.serving_frame(4, k, 10 ** 6, cfgs[k]))
+    native.synchronize()
+    t0 = time.perf_counter()
+    for i in range(B):
+        k = int(which[i])
+        plain[k].update(dss[k], frames[i])
+    native.synchronize()
+    t1 = time.perf_counter()
+    plain_rays_s = float(rays[:B].sum()) / (t1 - t0)
+    burst = {"requests": B, "serve_bf16_rays_per_s": round(serve_rays_s, 1),
+             "plain_fp32_rays_per_s": round(plain_rays_s, 1),
+             "speedup": round(serve_rays_s / plain_rays_s, 3)}
+    log(f"[bench] burst {json.dumps(burst)}")
+    out = {
+        "metric": "serving p99 latency (ms), Poisson 200 req/s over 64 resident scenes",
+        "value": open_loop["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": R, "warmup": len(sids),
+        "ms_per_step": open_loop["mean_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator, seed 4)",
+        "config": {"workload": "serve64: 64 scenes, 10k-2M triangles log-uniform, grids "
+                               "{4^3, 8^3, 16x8x16, 32x16x32} x rays {128, 256, 512}, BF16 conv",
+                   "triangles": tris, "hbm_budget_bytes": budget, "max_inflight": args.inflight,
+                   "streams": args.streams},
+        "open_loop": open_loop,
+        "burst": burst,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
     args = parse()
+    if args.workload == "serve64":
+        if args.gpus != 1:
+            print("bench.py runs on exactly one GPU", file=sys.stderr)
+            sys.exit(2)
+        bench_serve(args)
+        return
+    if args.workload == "dynamic4m":
+        if args.gpus != 1:
+            print("bench.py runs on exactly one GPU", file=sys.stderr)
+            sys.exit(2)
+        bench_dynamic(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU "
               "sharding of the probe grid is a design note, not a measured path.", file=sys.stderr)
@@ -133,10 +420,15 @@ def main():
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes": int(algo[dom]), "mean_ms": round(mean[dom], 4)}
 
+    conv = None
+    if not args.no_conv_study:
+        log("[bench] convolution study (FP32 / BF16 / FP8 vs FP64)")
+        conv = conv_study(cfg, frame_at(0))
+
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU oracle baseline on {args.cpu_layers} layers")
-        cpu = cpu_baseline(scene, cfg, frame_at(0), args.cpu_layers)
+        log(f"[bench] CPU oracle baseline for at least {args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(scene, cfg, frame_at, args.cpu_seconds)
         log(f"[bench] CPU {cpu['value']:.4g} rays/s on {cpu['cores']} threads")
 
     out = {
@@ -161,6 +453,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in mean.items() if k != "translate"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "conv_study": conv,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,0 +1,89 @@
+"""Request loop (include/ane_serve.h) on the GPU: every session's atlas after a
+mixed, overlapping burst of requests equals, bit for bit, the atlas of the same
+session updated alone through the plain per-scene path; admission by HBM
+budget; argument errors reported without touching the device."""
+import numpy as np
+import pytest
+
+from anerend_amd import native, scenegen, serving
+from anerend_amd.types import PRECISION_BF16, Config
+from tests.conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _sessions():
+    """Four small scenes of uneven sizes with uneven grids and ray counts."""
+    specs = [
+        (scenegen.room_scene(1), Config.make((8, 8, 8), step=(1, 1, 1), rays=256, hysteresis=0.97),
+         (0.5, 1.5, 0.5)),
+        (scenegen.room_scene(3, num_boxes=20), Config.make((4, 4, 4), step=(1, 1, 1), rays=128),
+         (0.2, 1.0, -0.3)),
+        (scenegen.terrain_scene(7, cells=64, num_instances=400),
+         Config.make((16, 8, 16), step=(1, 2, 1), rays=128, precision=PRECISION_BF16), (0.5, 10.0, 0.5)),
+        (scenegen.terrain_scene(8, cells=40, num_instances=100),
+         Config.make((8, 4, 8), step=(1, 2, 1), rays=512, precision=PRECISION_BF16), (1.5, 9.0, -2.5)),
+    ]
+    return specs
+
+
+def test_mixed_equals_alone():
+    specs = _sessions()
+    scenes = [native.DeviceScene(s) for s, _, _ in specs]
+    srv = serving.Server(hbm_budget_bytes=64 << 30, max_inflight=4, num_streams=3)
+    ids = [srv.add_session(ds, cfg) for ds, (_, cfg, _) in zip(scenes, specs)]
+    # interleaved requests: 3 full updates per session, submitted back to back
+    frames = {}
+    tickets = []
+    for it in range(3):
+        for k, sid in enumerate(ids):
+            cfg, cam = specs[k][1], specs[k][2]
+            fr = scenegen.make_frame(11 + k, camera=cam, config=cfg, iteration=it)
+            frames.setdefault(k, []).append(fr)
+            tickets.append(srv.submit(sid, fr))
+    print(f"serve: {len(tickets)} requests submitted", flush=True)
+    res = [srv.poll(t, timeout_ms=-1) for t in tickets]
+    assert all(r["status"] == 0 for r in res)
+    st = srv.stats()
+    print(f"serve stats {st}", flush=True)
+    assert st["completed"] == len(tickets) and st["failed"] == 0
+    mixed = [srv.atlas(sid) for sid in ids]
+    srv.close()
+    for k, (ds, (_, cfg, _)) in enumerate(zip(scenes, specs)):
+        grid = native.ProbeGrid(cfg)
+        for fr in frames[k]:
+            grid.update(ds, fr)
+        alone = grid.atlas
+        assert np.array_equal(bits(alone), bits(mixed[k])), f"session {k} differs alone vs mixed"
+        grid.close()
+
+
+def test_admission_by_budget_and_errors():
+    scene, cfg, frame = scenegen.workload("room8")
+    ds = native.DeviceScene(scene)
+    scene_bytes = ds.info()["device_bytes"]
+    probe = native.ProbeGrid(cfg)
+    grid_bytes = probe.stats()["device_bytes"]
+    probe.close()
+    # room for the scene and exactly one grid
+    srv = serving.Server(hbm_budget_bytes=scene_bytes + grid_bytes + grid_bytes // 2,
+                         max_inflight=2, num_streams=2)
+    a = srv.add_session(ds, cfg)
+    with pytest.raises(native.AneError) as e:
+        srv.add_session(ds, cfg)
+    assert e.value.status == 3  # ANE_ERR_ADMISSION_REFUSED
+    assert srv.stats()["sessions"] == 1
+    bad = frame.copy()
+    bad.layer_begin, bad.layer_count = 6, 4
+    with pytest.raises(native.AneError):
+        srv.submit(a, bad)
+    with pytest.raises(native.AneError):
+        srv.submit(a + 100, frame)
+    t = srv.submit(a, frame)
+    r = srv.poll(t, timeout_ms=-1)
+    assert r["status"] == 0 and r["rays"] == 512 * 256
+    srv.remove_session(a)
+    assert srv.stats()["used_bytes"] == 0
+    b = srv.add_session(ds, cfg)  # admitted again once the memory is free
+    assert srv.poll(srv.submit(b, frame), timeout_ms=-1)["status"] == 0
+    srv.close()

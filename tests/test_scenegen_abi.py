@@ -16,6 +16,8 @@ from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "ane_ddgi.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                 if f.endswith(".h"))
 
 
 def test_pcg32_known_answers():
@@ -40,9 +42,10 @@ def test_scenes_are_deterministic_and_sized():
 
 def test_library_exports_every_declared_symbol():
     L = native.load()
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     declared = sorted(set(re.findall(r"^\w[\w\s\*]*?\b(ane_\w+)\(", text, re.M)))
-    assert len(declared) >= 25
+    assert len(declared) >= 33
+    assert "ane_serve_submit" in declared
     missing = [d for d in declared if not hasattr(L, d)]
     assert not missing, missing
     assert set(native.EXPORTED) <= set(declared)
@@ -71,6 +74,35 @@ int main(void) {
                          MATERIAL_DTYPE.itemsize, INSTANCE_DTYPE.itemsize, 16]
     assert sizes[6:] == [Frame.rotation.offset, Frame.bake_tile.offset,
                          INSTANCE_DTYPE.fields["custom_index_and_mask"][1], Config.precision.offset]
+
+
+def test_serve_layouts_and_validation(tmp_path):
+    """Serving records match include/ane_serve.h; invalid configs are refused
+    before any device work."""
+    from anerend_amd import serving
+    src = tmp_path / "probe_serve.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "ane_serve.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(ane_serve_config), sizeof(ane_serve_result),
+         sizeof(ane_serve_stats), offsetof(ane_serve_result, latency_ms), offsetof(ane_serve_stats, rays_done));
+  return 0;
+}''')
+    exe = tmp_path / "probe_serve"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert out == [ctypes.sizeof(serving.ServeConfig), ctypes.sizeof(serving.ServeResult),
+                   ctypes.sizeof(serving.ServeStats), serving.ServeResult.latency_ms.offset,
+                   serving.ServeStats.rays_done.offset]
+    L = native.load()
+    h = ctypes.c_void_p()
+    for cfg in (serving.ServeConfig(0, 4, 2), serving.ServeConfig(1 << 30, 0, 2),
+                serving.ServeConfig(1 << 30, 4, 0), serving.ServeConfig(1 << 30, 4, 17)):
+        assert L.ane_serve_open(ctypes.byref(cfg), ctypes.byref(h)) == 1
+        assert not h.value
 
 
 @pytest.mark.parametrize("n,angle,axis", [(256, 0.0, 0), (512, 123.4, 1), (100, 359.0, 2)])
