@@ -558,9 +558,6 @@ struct ane_ddgi_ctx {
   float* d_dirs = nullptr;
   float4* d_dirpre = nullptr;
   float4* d_probe_org = nullptr;
-  float4* d_hitpos = nullptr;
-  uint4* d_hitgeom = nullptr;
-  uint32_t* d_list = nullptr;
   float* d_w32 = nullptr;
   float* d_wsum = nullptr;
   uint16_t* d_wbf16 = nullptr;
@@ -590,7 +587,7 @@ struct ane_ddgi_ctx {
 static void ctx_free(ane_ddgi_ctx* c)
 {
   dfree(c->d_atlas[0]); dfree(c->d_atlas[1]); dfree(c->d_base_dirs); dfree(c->d_dirs);
-  dfree(c->d_dirpre); dfree(c->d_probe_org); dfree(c->d_hitpos); dfree(c->d_hitgeom); dfree(c->d_list);
+  dfree(c->d_dirpre); dfree(c->d_probe_org); 
   dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
@@ -652,14 +649,11 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   ANE_A(d_dirs, c->N * 3);
   ANE_A(d_dirpre, (c->N + 1) * 3);
   ANE_A(d_probe_org, c->P);
-  ANE_A(d_hitpos, PN);
-  ANE_A(d_hitgeom, PN);
-  ANE_A(d_list, PN);
   ANE_A(d_w32, 64 * c->N);
   ANE_A(d_wsum, 64);
   ANE_A(d_wbf16, 64 * c->N);
   ANE_A(d_wfp8, 64 * c->N);
-  ANE_A(d_wfp8_scale, 64);
+  ANE_A(d_wfp8_scale, 128);  // [64] FP8 texel scales, [64] BF16 weight-sum corrections
   ANE_A(d_order, c->N);
   ANE_A(d_hits, PN);
   ANE_A(d_shadow, PN);
@@ -732,6 +726,10 @@ static int frame_to_k(const ane_ddgi_ctx* c, const ane_ddgi_frame* f, FrameK* k)
   k->layer_begin = f->layer_begin;
   k->layer_count = f->layer_count;
   for (int i = 0; i < 16; ++i) k->rot[i] = f->rotation[i];
+  const f3 L = normalize(mk(-f->light_dir[0], -f->light_dir[1], -f->light_dir[2]));
+  k->sun_l[0] = L.x;
+  k->sun_l[1] = L.y;
+  k->sun_l[2] = L.z;
   return ANE_OK;
 }
 
@@ -761,9 +759,6 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
   a.div_magic = rd.magic;
   a.div_shift = rd.shift;
   a.order = c->d_order;
-  a.hitpos = c->d_hitpos;
-  a.list = c->d_list;
-  a.list_count = c->d_work + 2;
   a.hits = c->d_hits;
   a.shadow = c->d_shadow;
   a.stats = c->d_stats;
@@ -802,19 +797,12 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   ANE_TRY(c->timers[kStSetup].end(st));
   TraceArgs a = trace_args(c, s, fk);
   a.work = c->d_work;
+  // primary and shadow rays in one persistent launch (the trace stage)
   ANE_TRY(c->timers[kStTrace].begin(st));
   launch_trace(a, false, false, c->counting, st);
   ANE_TRY(c->timers[kStTrace].end(st));
-  TraceArgs b = a;
-  b.work = c->d_work + 1;
-  b.stats = c->d_stats + 2;
-  ANE_TRY(c->timers[kStShadow].begin(st));
-  launch_hit_compact(s->view, c->d_hits, c->d_order, (int)c->N, a.nrays, c->d_hitpos,
-                     c->d_hitgeom, c->d_shadow, c->d_list, c->d_work + 2, st);
-  launch_trace(b, true, false, c->counting, st);
-  ANE_TRY(c->timers[kStShadow].end(st));
   ANE_TRY(c->timers[kStShade].begin(st));
-  launch_shade(s->view, c->g, fk, c->d_probe_org, c->d_hits, c->d_hitpos, c->d_hitgeom, c->d_shadow,
+  launch_shade(s->view, c->g, fk, c->d_probe_org, c->d_hits, c->d_shadow,
                c->d_atlas[c->cur], c->cfg.precision, c->d_rad, c->d_rad_scale, nprobes,
                a.probe_base, st);
   ANE_CHECK_HIP(hipGetLastError());

@@ -64,6 +64,7 @@ struct FrameK {
   int32_t layer_begin, layer_count;
   int32_t bake_tile[4];
   float rot[16];
+  float sun_l[3];      // normalize(-light_dir): direction to the sun (pbr_light.glsl:103)
 };
 
 struct GridK {

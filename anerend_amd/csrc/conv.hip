@@ -208,8 +208,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv_mfma(GridK g, const void* 
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int t = rt * 16 + 4 * kq + i;
-      float s = 1.0f;
-      if (PREC == ANE_PRECISION_FP8) s = wscale[t] * ps;
+      // per-texel weight scale (FP8 dequantization) x weight-sum correction
+      const float s = (PREC == ANE_PRECISION_FP8) ? wscale[t] * ps : wscale[64 + t];
       const int64_t ti = texel_index(g, q, t);
       const float4 old = atlas[ti];
       atlas[ti] = blend_texel(old, acc[rt][0][i] * s, acc[rt][1][i] * s, acc[rt][2][i] * s, wsum[t],
@@ -234,7 +234,7 @@ void launch_convolve(const GridK& g, const FrameK& f, int precision, const void*
   const unsigned blocks = (unsigned)((num_probes + 16 * kWaves - 1) / (16 * kWaves));
   if (precision == ANE_PRECISION_BF16)
     hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_BF16>, dim3(blocks), dim3(64 * kWaves), 0, s, g,
-                       rad, rad_scale, (const void*)wbf16, (const float*)nullptr, wsum, a4,
+                       rad, rad_scale, (const void*)wbf16, wfp8_scale, wsum, a4,
                        num_probes, probe_base, (int32_t)bake);
   else
     hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_FP8>, dim3(blocks), dim3(64 * kWaves), 0, s, g,

@@ -45,13 +45,10 @@ struct TraceArgs {
   const float4* probe_org; // probe origins of the grid
   uint64_t div_magic;      // ray index -> probe: (gidx * div_magic) >> div_shift == gidx / N
   int32_t div_shift;
-  const float4* hitpos;   // shadow: hit positions (shadow-ray origins) per ray
-  const uint32_t* list;   // shadow: compacted hit rays
-  const uint32_t* list_count;
-  ane_hit* hits;          // closest: out; shadow: in
-  uint8_t* shadow;        // shadow: out (0 lit, 1 shadowed, 255 miss)
+  ane_hit* hits;          // out: closest hit per ray
+  uint8_t* shadow;        // out (probe rays): 0 lit, 1 shadowed, 255 miss
   uint32_t* work;         // dynamic ray counter (zeroed before launch)
-  unsigned long long* stats; // [2] nodes, triangles (counting kernels)
+  unsigned long long* stats; // counting kernels: nodes, triangles [, shadow nodes, shadow triangles]
   uint32_t* overflow;     // set to 1 if a stack would exceed the spill capacity
   int32_t* spill;         // [spill_depth][total_lanes]
   int32_t spill_depth;
@@ -67,7 +64,9 @@ struct TraceArgs {
 
 int trace_lanes_per_launch();
 int trace_stack_lds();
-void launch_trace(const TraceArgs& a, bool shadow, bool any_hit_rays, bool count, hipStream_t s);
+// rays = false: probe rays (closest hit, then the shadow ray of every hit);
+// rays = true: caller rays, closest or any hit
+void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipStream_t s);
 void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
                         const uint32_t* order, float* dirs, float4* dirpre, float4* probe_org, float* w32, float* wsum, uint16_t* wbf16,
                         uint8_t* wfp8, float* wfp8_scale, bool bake, hipStream_t s);
@@ -77,11 +76,8 @@ struct RayDiv {
 };
 // exact x / n for 0 <= x < nrays (nrays < 2^31, 1 <= n <= 4096)
 RayDiv ray_div_magic(int64_t nrays, int n);
-void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
-                        int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
-                        uint32_t* count, hipStream_t s);
 void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
-                  const ane_hit* hits, const float4* hitpos, const uint4* hitgeom,
+                  const ane_hit* hits,
                   const uint8_t* shadow, const float* atlas, int precision, void* rad,
                   float* rad_scale, int num_probes, int probe_base, hipStream_t s);
 

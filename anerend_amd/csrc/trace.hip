@@ -8,10 +8,9 @@
 //   k_trace          persistent wave-wide BVH traversal, one ray per lane: each wave
 //                    claims chunks of consecutive (direction-coherent) rays and refills
 //                    finished lanes by ballot between traversal steps; short stack in
-//                    LDS with a global spill area; closest hit (rgen:90-106) or any hit
-//                    toward the sun (rchit:44-66)
-//   k_hit_compact    hit position (rchit:30-42) of every hit and ballot compaction of
-//                    the hit rays into the shadow-ray list; misses marked
+//                    LDS with a global spill area.  A probe ray's closest hit
+//                    (rgen:90-106) is followed in the same lane by its any-hit shadow
+//                    ray from the hit position (rchit:30-66)
 //   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
 //                    indirect light from the previous atlas, emissive, sky on miss
 //                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
@@ -32,6 +31,9 @@ namespace ane {
 #endif
 #ifndef ANE_TRACE_REFILL
 #define ANE_TRACE_REFILL 32
+#endif
+#ifndef ANE_TRACE_WAVES
+#define ANE_TRACE_WAVES 8  // waves per SIMD the traversal is register-limited to
 #endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring)
@@ -136,12 +138,34 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot
     __syncthreads();
   }
   const float wmax = red[0];
+  __syncthreads();
   const float sc = wmax > 0.0f ? wmax / 448.0f : 1.0f;
-  for (int r = threadIdx.x; r < n; r += blockDim.x)
-    wfp8[(int64_t)t * n + r] = f32_to_e4m3(w32[(int64_t)t * n + r] / sc);
+  // quantized weights and the sums that undo their normalization error: the
+  // epilogue scales texel t by (sum of FP32 weights) / (sum of quantized weights)
+  float s32 = 0.0f, sb = 0.0f, s8 = 0.0f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const float w = w32[(int64_t)t * n + r];
+    const uint8_t q = f32_to_e4m3(w / sc);
+    wfp8[(int64_t)t * n + r] = q;
+    s32 += w;
+    sb += bf16_to_f32(wbf16[(int64_t)t * n + r]);
+    s8 += e4m3_to_f32(q) * sc;
+  }
+  __shared__ float red3[3][256];
+  red3[0][threadIdx.x] = s32;
+  red3[1][threadIdx.x] = sb;
+  red3[2][threadIdx.x] = s8;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k)
+      for (int c = 0; c < 3; ++c) red3[c][threadIdx.x] += red3[c][threadIdx.x + k];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     wsum[t] = sum;
-    wfp8_scale[t] = sc;
+    const float S = red3[0][0], SB = red3[1][0], S8 = red3[2][0];
+    wfp8_scale[t] = S8 > 0.0f ? sc * (S / S8) : sc;  // FP8: dequantization x correction
+    wfp8_scale[64 + t] = SB > 0.0f ? S / SB : 1.0f;  // BF16: correction
   }
 }
 
@@ -156,8 +180,6 @@ void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
                      dirs, dirpre, probe_org, w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake);
 }
 
-// gidx / n for gidx < 2^31 and 1 <= n <= 4096: floor(gidx * ceil(2^44 / n) / 2^44) is exact
-// (the multiplier error times gidx stays below 2^-13 < 1/n)
 // Division of a ray index x < nrays by n as (x * m) >> k, m = ceil(2^k / n).
 // With e = m n - 2^k < n, x m / 2^k = x / n + x e / (n 2^k) and the floor is
 // exact when x e / 2^k < 1, which k = bit_length(nrays * n) guarantees
@@ -202,10 +224,17 @@ __device__ __forceinline__ f3 hit_geom_pos(const SceneK& sc, int32_t id, float u
 
 // ---------------------------------------------------------------------------
 // traversal
+//   RAYS = false: probe rays.  A lane traces its primary ray (closest hit,
+//     rgen:90-106); when it hits, the same lane continues at once with the
+//     shadow ray from the hit position toward the sun (any hit, rchit:44-66).
+//   RAYS = true: caller rays, closest hit or (ANY) any hit.
 // ---------------------------------------------------------------------------
-template <bool SHADOW, bool RAYS, bool COUNT>
-__global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
+template <bool ANY, bool RAYS, bool COUNT>
+__global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_WAVES))) void k_trace(TraceArgs a)
 {
+  constexpr int32_t kDone = 0x7FFFFFFF;  // node sentinel: the lane's traversal has ended
+  constexpr int32_t kPop = 0x7FFFFFFE;   // step result: continue from the stack
+  // (internal nodes are 0 .. ntri - 2 < kPop; leaves are ~position < 0)
   __shared__ int32_t stk[kSS][kTB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -215,20 +244,48 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
   const float tmin = RAYS ? a.t_min : a.g.t_min;
   const float tmax0 = RAYS ? a.t_max : a.g.t_max;
   const int n = a.g.rays;
-  // shadow rays: the compacted hit list, its length lives in device memory
-  const uint32_t nrays = (SHADOW && !RAYS) ? *a.list_count : (uint32_t)a.nrays;
+  const uint32_t nrays = (uint32_t)a.nrays;
+  const int32_t root = (sc.ntri >= 2) ? 0 : (sc.ntri == 1 ? ~0 : kDone);
+  const uint32_t budget = 2u * (uint32_t)sc.ntri + 8u;  // a tree traversal visits < 2 * ntri nodes
 
   int32_t ray = -1;  // active ray (processing index), -1 idle
   bool exhausted = false;
+  bool sph = ANY;    // the lane's ray is an any-hit (shadow) ray
   uint32_t cbeg = 0, cend = 0;  // wave-uniform: remaining part of the claimed chunk
   RayPre r;
   float tbest = 0.0f, bu = 0.0f, bv = 0.0f;
-  int32_t bid = -1, node = 0, sp = 0, out = 0;
-  uint32_t steps_left = 0;  // per-ray step budget: a tree traversal visits < 2 * ntri nodes
-  unsigned long long cn = 0, ct = 0;
+  int32_t bid = -1, node = kDone, sp = 0, out = 0;
+  uint32_t steps_left = 0;
+  unsigned long long cn = 0, ct = 0, sn = 0, st = 0;
 
   for (;;) {
-    // ---- refill finished lanes (between steps) ----
+    // ---- finish ended traversals; a probe ray that hit continues as its shadow ray ----
+    if (ray >= 0 && node == kDone) {
+      if (RAYS || !sph) {
+        ane_hit h;
+        h.tri = bid;
+        h.t = (bid >= 0) ? tbest : 0.0f;
+        h.u = (bid >= 0) ? bu : 0.0f;
+        h.v = (bid >= 0) ? bv : 0.0f;
+        a.hits[out] = h;
+      }
+      if (!RAYS && !sph && bid >= 0) {
+        // shadow ray from the hit position toward the sun (rchit:44-66), same lane
+        HitGeom hg;
+        const f3 p = hit_geom_pos(sc, bid, bu, bv, hg);
+        load_dirpre(r, a.dirpre + 3 * n, p);
+        sph = true;
+        tbest = tmax0;
+        bid = -1;
+        sp = 0;
+        steps_left = budget;
+        node = root;
+      } else {
+        if (!RAYS) a.shadow[out] = sph ? (uint8_t)(bid >= 0 ? 1 : 0) : (uint8_t)255;
+        ray = -1;
+      }
+    }
+    // ---- refill idle lanes (between steps) ----
     const bool idle = (ray < 0) && !exhausted;
     const uint64_t im = __ballot(idle);
     const uint64_t am = __ballot(ray >= 0);
@@ -248,15 +305,9 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
           exhausted = true;
         } else if (rank < avail) {
           const uint32_t gidx = cbeg + rank;
-          f3 o;
           if (RAYS) {
             out = (int32_t)gidx;
-            o = ld3(a.ray_orig + 3 * (int64_t)gidx);
-            ray_setup(r, o, ld3(a.ray_dir + 3 * (int64_t)gidx));
-          } else if (SHADOW) {
-            out = (int32_t)a.list[gidx];
-            const float4 p = a.hitpos[out];
-            load_dirpre(r, a.dirpre + 3 * n, mk(p.x, p.y, p.z));
+            ray_setup(r, ld3(a.ray_orig + 3 * (int64_t)gidx), ld3(a.ray_dir + 3 * (int64_t)gidx));
           } else {
             // independent loads: processing-order ray setup, probe origin, output slot
             const uint32_t ql = (uint32_t)(((uint64_t)gidx * a.div_magic) >> a.div_shift);
@@ -264,14 +315,14 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
             const float4 po = a.probe_org[a.probe_base + (int)ql];
             load_dirpre(r, a.dirpre + 3 * j, mk(po.x, po.y, po.z));
             out = (int32_t)ql * n + (int)a.order[j];
+            sph = false;
           }
           tbest = tmax0;
           bid = -1;
-          bu = bv = 0.0f;
           sp = 0;
-          steps_left = 2u * (uint32_t)sc.ntri + 8u;
+          steps_left = budget;
           ray = (int32_t)gidx;
-          node = (sc.ntri >= 2) ? 0 : (sc.ntri == 1 ? ~0 : 0x7FFFFFFF);
+          node = root;
         }
       }
       cbeg += need < avail ? need : avail;
@@ -280,108 +331,87 @@ __global__ __launch_bounds__(kTB) void k_trace(TraceArgs a)
       if (__ballot(!exhausted) == 0ull) break;
       continue;
     }
-    // ---- one traversal step ----
-    if (ray >= 0) {
-      bool done = (node == 0x7FFFFFFF);
-      if (!done) {
-        bool pop = true;
-        // guards: a malformed hierarchy ends the ray and raises the error flag
-        // instead of reading out of bounds or looping forever
-        if (steps_left == 0u || (node >= 0 && node >= sc.ntri - 1) || (node < 0 && ~node >= sc.ntri)) {
-          *a.overflow = 1u;
-          done = true;
-          pop = false;
-        } else if (node >= 0) {
-          --steps_left;
-          if (COUNT) ++cn;
-          const float4* nd = sc.nodes + 4 * (int64_t)node;
-          const float4 na = nd[0], nb = nd[1], nc = nd[2], nn = nd[3];
-          float tn0, tn1;
-          const bool b0 = box_test(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tbest, tn0);
-          const bool b1 = box_test(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tbest, tn1);
-          const int32_t L = __float_as_int(nn.x), R = __float_as_int(nn.y);
-          if (b0 || b1) {
-            pop = false;
-            if (b0 && b1) {
-              const bool sw = tn1 < tn0;
-              const int32_t nearc = sw ? R : L;
-              const int32_t farc = sw ? L : R;
-              // push the far child (LDS ring; the oldest entry spills when full)
-              const int slot = sp & (kSS - 1);
-              bool ok = true;
-              if (sp >= kSS) {
-                const int si = sp - kSS;
-                if (si < a.spill_depth) {
-                  a.spill[(int64_t)si * a.total_lanes + gl] = stk[slot][tid];
-                } else {
-                  *a.overflow = 1u;
-                  ok = false;
-                }
-              }
-              if (ok) {
-                stk[slot][tid] = farc;
-                ++sp;
-                node = nearc;
-              } else {
-                done = true;
-              }
+    // ---- one traversal step: only node, sp, the best hit and the budget change ----
+    if (ray >= 0 && node != kDone) {
+      int32_t next = kPop;
+      // guards: a malformed hierarchy ends the ray and raises the error flag
+      // instead of reading out of bounds or looping forever
+      const bool bad = steps_left == 0u || (node >= 0 && node >= sc.ntri - 1) ||
+                       (node < 0 && ~node >= sc.ntri);
+      --steps_left;
+      if (bad) {
+        *a.overflow = 1u;
+        next = kDone;
+      } else if (node >= 0) {
+        if (COUNT) {
+          if (sph) ++sn; else ++cn;
+        }
+        const float4* nd = sc.nodes + 4 * (int64_t)node;
+        const float4 na = nd[0], nb = nd[1], nc = nd[2], nn = nd[3];
+        float tn0, tn1;
+        const bool b0 = box_test(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tbest, tn0);
+        const bool b1 = box_test(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tbest, tn1);
+        const int32_t L = __float_as_int(nn.x), R = __float_as_int(nn.y);
+        const bool sw = b0 && b1 && tn1 < tn0;
+        next = (b0 || b1) ? ((b0 && !sw) ? L : R) : kPop;
+        if (b0 && b1) {
+          // push the far child (LDS ring; the oldest entry spills when full)
+          const int32_t farc = sw ? L : R;
+          const int slot = sp & (kSS - 1);
+          if (sp >= kSS) {
+            const int si = sp - kSS;
+            if (si < a.spill_depth) {
+              a.spill[(int64_t)si * a.total_lanes + gl] = stk[slot][tid];
             } else {
-              node = b0 ? L : R;
+              *a.overflow = 1u;
+              next = kDone;
             }
           }
-        } else {
-          --steps_left;
-          if (COUNT) ++ct;
-          const int64_t p = ~node;
-          const float4 v0 = sc.tris[3 * p], v1 = sc.tris[3 * p + 1], v2 = sc.tris[3 * p + 2];
-          float t, u, v;
-          if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin,
-                       tbest, t, u, v)) {
-            const int32_t id = __float_as_int(v0.w);
-            if (SHADOW) {
-              bid = id;
-              tbest = t;
-              bu = u;
-              bv = v;
-              done = true;
-              pop = false;
-            } else if (t < tbest || (t == tbest && id < bid)) {
-              bid = id;
-              tbest = t;
-              bu = u;
-              bv = v;
-            }
-          }
+          stk[slot][tid] = farc;
+          ++sp;
         }
-        if (pop && !done) {
-          if (sp == 0) {
-            done = true;
-          } else {
-            --sp;
-            const int slot = sp & (kSS - 1);
-            node = stk[slot][tid];
-            if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
+      } else {
+        if (COUNT) {
+          if (sph) ++st; else ++ct;
+        }
+        const int64_t p = ~node;
+        const float4 v0 = sc.tris[3 * p], v1 = sc.tris[3 * p + 1], v2 = sc.tris[3 * p + 2];
+        float t, u, v;
+        if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin,
+                     tbest, t, u, v)) {
+          const int32_t id = __float_as_int(v0.w);
+          if (sph || t < tbest || (t == tbest && id < bid)) {
+            bid = id;
+            tbest = t;
+            bu = u;
+            bv = v;
           }
+          if (sph) next = kDone;  // any hit ends a shadow ray
         }
       }
-      if (done) {
-        if (SHADOW && !RAYS) {
-          a.shadow[out] = (bid >= 0) ? 1 : 0;
+      if (next == kPop) {
+        if (sp == 0) {
+          next = kDone;
         } else {
-          ane_hit h;
-          h.tri = bid;
-          h.t = (bid >= 0) ? tbest : 0.0f;
-          h.u = (bid >= 0) ? bu : 0.0f;
-          h.v = (bid >= 0) ? bv : 0.0f;
-          a.hits[out] = h;
+          --sp;
+          const int slot = sp & (kSS - 1);
+          next = stk[slot][tid];
+          if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
         }
-        ray = -1;
       }
+      node = next;
     }
   }
   if (COUNT) {
-    atomicAdd(&a.stats[0], cn);
-    atomicAdd(&a.stats[1], ct);
+    if (RAYS) {  // caller rays: one ray kind, two counters
+      atomicAdd(&a.stats[0], cn + sn);
+      atomicAdd(&a.stats[1], ct + st);
+    } else {
+      atomicAdd(&a.stats[0], cn);
+      atomicAdd(&a.stats[1], ct);
+      atomicAdd(&a.stats[2], sn);
+      atomicAdd(&a.stats[3], st);
+    }
   }
 }
 
@@ -400,74 +430,19 @@ int trace_lanes_per_launch()
   return g_lanes;
 }
 
-void launch_trace(const TraceArgs& a, bool shadow, bool rays, bool count, hipStream_t s)
+void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipStream_t s)
 {
   const int blocks = trace_lanes_per_launch() / kTB;
   TraceArgs b = a;
   b.total_lanes = blocks * kTB;
 #define ANE_TR(S, R, C) hipLaunchKernelGGL((k_trace<S, R, C>), dim3(blocks), dim3(kTB), 0, s, b)
   if (rays) {
-    if (shadow) { if (count) ANE_TR(true, true, true); else ANE_TR(true, true, false); }
+    if (any_hit) { if (count) ANE_TR(true, true, true); else ANE_TR(true, true, false); }
     else { if (count) ANE_TR(false, true, true); else ANE_TR(false, true, false); }
   } else {
-    if (shadow) { if (count) ANE_TR(true, false, true); else ANE_TR(true, false, false); }
-    else { if (count) ANE_TR(false, false, true); else ANE_TR(false, false, false); }
+    if (count) ANE_TR(false, false, true); else ANE_TR(false, false, false);
   }
 #undef ANE_TR
-}
-
-// ---------------------------------------------------------------------------
-// hit positions + compaction of the shadow-ray list: one block compacts 256
-// consecutive rays of the processing order (probe-major, direction-sorted)
-// with a block-local prefix sum and reserves its list range with one atomic,
-// so each 256-ray group stays together and in order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hit_compact(SceneK sc, const ane_hit* hits,
-                                                     const uint32_t* order, int n, int nrays,
-                                                     float4* hitpos, uint4* hitgeom, uint8_t* shadow,
-                                                     uint32_t* list, uint32_t* count)
-{
-  __shared__ uint32_t wcount[4];
-  __shared__ uint32_t base;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  bool hit = false;
-  int out = 0;
-  if (g < nrays) {
-    const int ql = g / n;
-    out = ql * n + (int)order[g - ql * n];
-    const ane_hit h = hits[out];
-    hit = h.tri >= 0;
-    if (hit) {
-      HitGeom hg;
-      const f3 p = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
-      hitpos[out] = make_float4(p.x, p.y, p.z, 1.0f);
-      hitgeom[out] = make_uint4(hg.inst, hg.v[0], hg.v[1], hg.v[2]);
-    } else {
-      shadow[out] = 255;
-    }
-  }
-  const uint64_t hm = __ballot(hit);
-  if (lane == 0) wcount[wave] = (uint32_t)__popcll(hm);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-    base = total ? atomicAdd(count, total) : 0u;
-  }
-  __syncthreads();
-  uint32_t pre = base;
-  for (int w = 0; w < wave; ++w) pre += wcount[w];
-  if (hit) list[pre + (uint32_t)__popcll(hm & lt)] = (uint32_t)out;
-}
-
-void launch_hit_compact(const SceneK& sc, const ane_hit* hits, const uint32_t* order, int n,
-                        int nrays, float4* hitpos, uint4* hitgeom, uint8_t* shadow, uint32_t* list,
-                        uint32_t* count, hipStream_t s)
-{
-  if (nrays <= 0) return;
-  hipLaunchKernelGGL(k_hit_compact, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, sc, hits,
-                     order, n, nrays, hitpos, hitgeom, shadow, list, count);
 }
 
 // ---------------------------------------------------------------------------
@@ -499,14 +474,12 @@ __device__ __forceinline__ float pow5_term(float c)
   return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f);
 }
 
-// calcLight, pbr_light.glsl:98-166 (directional, specular on)
-__device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float rough, f3 pos,
-                         f3 view)
+// calcLight, pbr_light.glsl:98-166 (directional, specular on); V = normalize(view - pos)
+__device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float rough, f3 V)
 {
   f3 light_color = muls(mk(1.0f, 1.0f, 1.0f), f.sun_intensity);
-  f3 V = normalize(sub(view, pos));
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
-  f3 L = normalize(mk(-f.light_dir[0], -f.light_dir[1], -f.light_dir[2]));
+  f3 L = mk(f.sun_l[0], f.sun_l[1], f.sun_l[2]);  // normalize(-lightDir), once per frame
   f3 H = normalize(add(V, L));
   f3 radiance = muls(light_color, 1.0f);
   float ndf = distribution_ggx(N, H, rough);
@@ -552,24 +525,15 @@ __device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float 
   return mk(r0 * (1.0f - ay) + r1 * ay, g0 * (1.0f - ay) + g1 * ay, b0 * (1.0f - ay) + b1 * ay);
 }
 
-// sampleSingleProbe, pbr_light.glsl:168-188
-__device__ f3 sample_single_probe(const GridK& g, const float4* atlas, int ix, int iy, int iz, f3 n)
+// sampleSingleProbe, pbr_light.glsl:168-188: the texture coordinates of the
+// normal's octahedral position inside the tile of probe (ix, iy, iz).  The
+// tile bounds of the 8 probes of sampleProbe take 2 distinct columns and 4
+// distinct rows, so they are computed once each (same operations, same values).
+__device__ __forceinline__ float tile_coord(int s0, int extent, float o)
 {
-  const int W = g.atlas_w, H = g.atlas_h;
-  int sx = 10 * ix + 1;
-  int sy = 10 * g.nz * iy + 10 * iz + 1;
-  int ex = sx + 7, ey = sy + 7;
-  float su = (2.0f * (float)sx + 1.0f) / (2.0f * (float)W);
-  float sv = (2.0f * (float)sy + 1.0f) / (2.0f * (float)H);
-  float eu = (2.0f * (float)ex + 1.0f) / (2.0f * (float)W);
-  float ev = (2.0f * (float)ey + 1.0f) / (2.0f * (float)H);
-  float ox, oy;
-  oct_encode(normalize(n), ox, oy);
-  ox = (ox + 1.0f) * 0.5f;
-  oy = (oy + 1.0f) * 0.5f;
-  float u = su + (eu - su) * ox;
-  float v = sv + (ev - sv) * oy;
-  return bilinear_repeat(atlas, W, H, u, v);
+  const float su = (2.0f * (float)s0 + 1.0f) / (2.0f * (float)extent);
+  const float eu = (2.0f * (float)(s0 + 7) + 1.0f) / (2.0f * (float)extent);
+  return su + (eu - su) * o;
 }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
@@ -599,6 +563,20 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
   f3 alpha = mk(clampf((pos.x - base.x) / step.x, 0.0f, 1.0f),
                 clampf((pos.y - base.y) / step.y, 0.0f, 1.0f),
                 clampf((pos.z - base.z) / step.z, 0.0f, 1.0f));
+  // octahedral position of the normal, shared by the 8 lookups
+  float octx, octy;
+  oct_encode(normalize(n), octx, octy);
+  octx = (octx + 1.0f) * 0.5f;
+  octy = (octy + 1.0f) * 0.5f;
+  const int W = g.atlas_w, H = g.atlas_h;
+  float tu[2], tv[4];
+#pragma unroll
+  for (int dx = 0; dx < 2; ++dx) tu[dx] = tile_coord(10 * (px + dx) + 1, W, octx);
+#pragma unroll
+  for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+      tv[2 * dy + dz] = tile_coord(10 * g.nz * (py + dy) + 10 * (pz + dz) + 1, H, octy);
   // corner order of pbr_light.glsl:318-355, packed as ox | oy << 1 | oz << 2
   const int corners = (0) | (7 << 3) | (3 << 6) | (1 << 9) | (6 << 12) | (2 << 15) | (4 << 18) |
                       (5 << 21);
@@ -617,7 +595,7 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
     w = fmaxf(0.000001f, w);
     if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
     w = w * (tri.x * tri.y * tri.z);
-    f3 s = sample_single_probe(g, atlas, px + ox, py + oy, pz + oz, n);
+    f3 s = bilinear_repeat(atlas, W, H, tu[ox], tv[2 * oy + oz]);
     s = mk(sqrtf(s.x), sqrtf(s.y), sqrtf(s.z));
     sumi = add(sumi, muls(s, w));
     sumw = sumw + w;
@@ -629,9 +607,8 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
 
 // calcIndirectDiffuseLight, pbr_light.glsl:365-388
 __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas, f3 N, f3 albedo,
-                            float metallic, float rough, f3 pos, f3 view)
+                            float metallic, float rough, f3 pos, f3 V)
 {
-  f3 V = normalize(sub(view, pos));
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
   float r = rough * rough;
   float p5 = pow5_term(fmaxf(dot(N, V), 0.0f));
@@ -646,17 +623,15 @@ __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas
 
 // rchit:24-110 (+ rmiss:17-21 for misses)
 __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const float4* atlas,
-                        f3 origin, ane_hit h, float4 hp, uint4 geom, uint8_t shadowed)
+                        f3 origin, ane_hit h, uint8_t shadowed)
 {
   if (h.tri < 0) return mk(f.sky[0], f.sky[1], f.sky[2]);
-  HitGeom hg;  // resolved by k_hit_pos (instance, absolute vertex indices)
-  hg.inst = geom.x;
-  hg.v[0] = geom.y;
-  hg.v[1] = geom.z;
-  hg.v[2] = geom.w;
+  // instance, vertices and position (rchit:27-42): the same arithmetic that gave
+  // the shadow-ray origin in the traversal
+  HitGeom hg;
+  const f3 pos = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
   const ane_instance& in = sc.inst[hg.inst];
   const ane_material& mat = sc.mats[mat_of(in.custom_index_and_mask)];
-  const f3 pos = mk(hp.x, hp.y, hp.z);  // k_hit_compact: rchit:40-42
   const bool lit = shadowed == 0;
   float bw[3] = {1.0f - h.u - h.v, h.u, h.v};
   f3 nrm = mk(0.0f, 0.0f, 0.0f), col = mk(0.0f, 0.0f, 0.0f);
@@ -684,17 +659,16 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   const float rough = mat.met_rough[1];
   f3 emissive = muls(mk(mat.emissive[0], mat.emissive[1], mat.emissive[2]), mat.emissive[3]);
   f3 acc = mk(0.0f, 0.0f, 0.0f);
-  if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, pos, origin);
+  const f3 V = normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
+  if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, V);
   if (f.flags & ANE_FLAG_DDGI_MULTI)
-    acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, origin));
+    acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, V));
   return add(acc, emissive);
 }
 
 template <int PREC, int TB>
 __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
-                                              const ane_hit* hits,
-                                              const float4* hitpos, const uint4* hitgeom,
-                                              const uint8_t* shadow, const float4* atlas, void* rad,
+                                              const ane_hit* hits, const uint8_t* shadow, const float4* atlas, void* rad,
                                               float* rad_scale, int probe_base)
 {
   extern __shared__ float sL[];  // FP8: [3][N] staging for the per-probe scale
@@ -708,15 +682,8 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
   for (int i = threadIdx.x; i < n; i += TB) {
     const int64_t idx = (int64_t)ql * n + i;
     const ane_hit h = hits[idx];
-    float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint4 geom = make_uint4(0u, 0u, 0u, 0u);
-    uint8_t sh = 255;
-    if (h.tri >= 0) {
-      hp = hitpos[idx];
-      geom = hitgeom[idx];
-      sh = shadow[idx];
-    }
-    const f3 L = shade_ray(sc, g, f, atlas, origin, h, hp, geom, sh);
+    const uint8_t sh = (h.tri >= 0) ? shadow[idx] : (uint8_t)255;
+    const f3 L = shade_ray(sc, g, f, atlas, origin, h, sh);
     if (PREC == ANE_PRECISION_FP32) {
       float* o = (float*)rad + (int64_t)ql * 3 * n;
       o[i] = L.x;
@@ -751,34 +718,34 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
 template <int TB>
 static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
                          const ane_hit* hits,
-                         const float4* hitpos, const uint4* hitgeom, const uint8_t* shadow,
+                         const uint8_t* shadow,
                          const float4* a4, int precision, void* rad, float* rad_scale,
                          int num_probes, int probe_base, hipStream_t s)
 {
   if (precision == ANE_PRECISION_FP32)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP32, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
-                       probe_org, hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+                       probe_org, hits, shadow, a4, rad, rad_scale, probe_base);
   else if (precision == ANE_PRECISION_BF16)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_BF16, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
-                       probe_org, hits, hitpos, hitgeom, shadow, a4, rad, rad_scale, probe_base);
+                       probe_org, hits, shadow, a4, rad, rad_scale, probe_base);
   else
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP8, TB>), dim3(num_probes), dim3(TB),
-                       (size_t)12 * g.rays, s, sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, rad,
+                       (size_t)12 * g.rays, s, sc, g, f, probe_org, hits, shadow, a4, rad,
                        rad_scale, probe_base);
 }
 
 void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
-                  const ane_hit* hits, const float4* hitpos, const uint4* hitgeom,
+                  const ane_hit* hits,
                   const uint8_t* shadow, const float* atlas, int precision, void* rad,
                   float* rad_scale, int num_probes, int probe_base, hipStream_t s)
 {
   if (num_probes <= 0) return;
   const float4* a4 = reinterpret_cast<const float4*>(atlas);
   if (g.rays >= 512)
-    shade_launch<512>(sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+    shade_launch<512>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
   else
-    shade_launch<256>(sc, g, f, probe_org, hits, hitpos, hitgeom, shadow, a4, precision, rad, rad_scale,
+    shade_launch<256>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
 }
 

@@ -180,10 +180,9 @@ int main(int argc, char** argv)
     ane_ddgi_stats s{};
     ane_ddgi_stats_get(grid, &s);
     log.push_back(rc.lastTracedFrame);
-    std::printf("[frame %d] layer %d cam (%.3f %.3f %.3f) trace %.3f shadow %.3f shade %.3f conv %.3f trans %.3f ms\n",
+    std::printf("[frame %d] layer %d cam (%.3f %.3f %.3f) trace %.3f shade %.3f conv %.3f trans %.3f ms\n",
                 fi, rc.lastTracedFrame.layer_begin, rc.getCameraPosition()[0], rc.getCameraPosition()[1],
-                rc.getCameraPosition()[2], s.last_ms[1], s.last_ms[2], s.last_ms[3], s.last_ms[4],
-                s.last_ms[5]);
+                rc.getCameraPosition()[2], s.last_ms[1], s.last_ms[3], s.last_ms[4], s.last_ms[5]);
     const float* c = rc.getCameraPosition();
     rc.setCameraPosition(c[0] + vel[0], c[1] + vel[1], c[2] + vel[2]);
   }

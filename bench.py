@@ -234,7 +234,7 @@ def bench_dynamic(args):
         "translations": moves,
         "bvh_update": (f"rebuild every {K} frames, refit in between" if K > 1 else
                        "rebuild every frame" if K == 1 else "refit every frame"),
-        "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items()},
+        "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items() if k != "shadow"},
         "last_bvh_update_ms": round(refit_ms, 4),
         "fp8_error_vs_fp64": err,
     }
@@ -408,11 +408,13 @@ def main():
 
     # roofline of the dominant kernel (algorithmic bytes per launch / mean launch time)
     rays = P * N
+    # the trace stage is one launch: primary rays and, in the same lanes, the
+    # shadow ray of every hit; per ray 16 B hit record + 1 B shadow flag out
     algo = {
-        "trace": work["primary_nodes"] * NODE_BYTES + work["primary_tris"] * TRI_BYTES + rays * HIT_BYTES,
-        "shadow": work["shadow_nodes"] * NODE_BYTES + work["shadow_tris"] * TRI_BYTES + rays * (HIT_BYTES + 1),
+        "trace": (work["primary_nodes"] + work["shadow_nodes"]) * NODE_BYTES
+                 + (work["primary_tris"] + work["shadow_tris"]) * TRI_BYTES + rays * (HIT_BYTES + 1),
     }
-    dom = max(("trace", "shadow", "shade", "convolve"), key=lambda k: mean[k])
+    dom = max(("trace", "shade", "convolve"), key=lambda k: mean[k])
     roof = None
     if dom in algo and mean[dom] > 0:
         ach = algo[dom] / (mean[dom] / 1000.0) / 1e9
@@ -450,7 +452,7 @@ def main():
                    "conv_precision": args.precision, "bvh_depth": info["bvh_depth"],
                    "lbvh_build_ms": round(info["build_ms"], 3),
                    "atlas_bytes": atlas_bytes},
-        "stages_ms": {k: round(v, 4) for k, v in mean.items() if k != "translate"},
+        "stages_ms": {k: round(v, 4) for k, v in mean.items() if k not in ("translate", "shadow")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "conv_study": conv,

@@ -136,7 +136,9 @@ typedef struct ane_scene_info {
 typedef struct ane_ddgi_stats {
   /* device time per stage from hipEvent pairs, ms: the last call and the mean
    * over all calls since the last ane_ddgi_stats_reset.  Stage order:
-   * setup, trace (primary rays), shadow, shade, convolve, translate. */
+   * setup, trace (primary rays and, in the same launch, the shadow ray of
+   * every hit), shadow (kept for the layout: always 0 calls), shade, convolve,
+   * translate. */
   float last_ms[6];
   float mean_ms[6];
   uint64_t calls[6];
