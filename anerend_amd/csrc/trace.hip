@@ -27,7 +27,7 @@ namespace ane {
 #define ANE_TRACE_STACK 16
 #endif
 #ifndef ANE_TRACE_CHUNK
-#define ANE_TRACE_CHUNK 256
+#define ANE_TRACE_CHUNK 128
 #endif
 #ifndef ANE_TRACE_REFILL
 #define ANE_TRACE_REFILL 32
@@ -673,13 +673,38 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
 {
   extern __shared__ float sL[];  // FP8: [3][N] staging for the per-probe scale
   __shared__ float red[TB];
+  // the probe's rays grouped as lit hits, shadowed hits, misses, so that a wave
+  // runs one kind of work (results are stored per ray, the order is immaterial)
+  __shared__ uint16_t lst[4096];
+  __shared__ uint32_t cnt[3], cur[3];
   const int ql = blockIdx.x;
   const int q = probe_base + ql;
   const int n = g.rays;
   const float4 po = probe_org[q];
   const f3 origin = mk(po.x, po.y, po.z);
-  float mx = 0.0f;
+  const int32_t* tri = reinterpret_cast<const int32_t*>(hits + (int64_t)ql * n);
+  const uint8_t* shp = shadow + (int64_t)ql * n;
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0u;
+  __syncthreads();
   for (int i = threadIdx.x; i < n; i += TB) {
+    const int c = tri[4 * i] < 0 ? 2 : (shp[i] == 0 ? 0 : 1);
+    atomicAdd(&cnt[c], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cur[0] = 0u;
+    cur[1] = cnt[0];
+    cur[2] = cnt[0] + cnt[1];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += TB) {
+    const int c = tri[4 * i] < 0 ? 2 : (shp[i] == 0 ? 0 : 1);
+    lst[atomicAdd(&cur[c], 1u)] = (uint16_t)i;
+  }
+  __syncthreads();
+  float mx = 0.0f;
+  for (int j = threadIdx.x; j < n; j += TB) {
+    const int i = lst[j];
     const int64_t idx = (int64_t)ql * n + i;
     const ane_hit h = hits[idx];
     const uint8_t sh = (h.tri >= 0) ? shadow[idx] : (uint8_t)255;
