@@ -236,7 +236,6 @@ struct ane_scene {
   uint32_t* d_arrive = nullptr;
   float* d_node_box = nullptr;
   uint32_t* d_cbounds = nullptr;
-  uint32_t* d_depth = nullptr;
   uint32_t* d_scratch = nullptr;
   int32_t* d_order = nullptr;   // internal nodes bucketed by depth
   uint64_t* d_lvl_key = nullptr;
@@ -263,7 +262,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_tri_rec); dfree(s->d_inst_off);
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
-  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_depth); dfree(s->d_scratch);
+  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_scratch);
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -333,13 +332,11 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
                     s->d_hist, &s->level_begin, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
-  launch_depth(s->d_leaf_parent, s->d_nodes, n, s->d_depth, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
-  uint32_t depth = 0;
-  ANE_CHECK_HIP(hipMemcpyAsync(&depth, s->d_depth, 4, hipMemcpyDeviceToHost, st));
   ANE_CHECK_HIP(hipStreamSynchronize(st));
-  s->depth = (int32_t)depth;
+  // the deepest internal nodes have two leaf children: leaf depth = number of levels
+  s->depth = (n >= 2) ? (int32_t)s->level_begin.size() - 1 : 0;
   ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
   s->build_ms_pending = false;
   return ANE_OK;
@@ -429,7 +426,6 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_arrive, n);
   ANE_A(d_node_box, (n > 1 ? n - 1 : 1) * 6);
   ANE_A(d_cbounds, 6);
-  ANE_A(d_depth, 1);
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
   ANE_A(d_order, n);
   ANE_A(d_lvl_key, n);
@@ -865,6 +861,31 @@ extern "C" int ane_ddgi_translate(ane_ddgi_ctx* c, const float* cam, int32_t* sh
   ANE_TRY(c->timers[kStTrans].end(st));
   c->cur = 1 - c->cur;
   if (shifted) *shifted = 1;
+  return ANE_OK;
+}
+
+// The spare atlas buffer (the translation target) stages the binary16 texels.
+extern "C" int ane_ddgi_atlas_export_rgba16f(ane_ddgi_ctx* c, uint16_t* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  hipStream_t st = S(stream);
+  uint2* stage = reinterpret_cast<uint2*>(c->d_atlas[c->cur ^ 1]);
+  launch_atlas_to_half(reinterpret_cast<const float4*>(c->d_atlas[c->cur]), stage, c->texels, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_CHECK_HIP(hipMemcpyAsync(host, stage, sizeof(uint2) * c->texels, hipMemcpyDeviceToHost, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_atlas_import_rgba16f(ane_ddgi_ctx* c, const uint16_t* host, void* stream)
+{
+  if (!c || !host) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  hipStream_t st = S(stream);
+  uint2* stage = reinterpret_cast<uint2*>(c->d_atlas[c->cur ^ 1]);
+  ANE_CHECK_HIP(hipMemcpyAsync(stage, host, sizeof(uint2) * c->texels, hipMemcpyHostToDevice, st));
+  launch_half_to_atlas(stage, reinterpret_cast<float4*>(c->d_atlas[c->cur]), c->texels, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
   return ANE_OK;
 }
 

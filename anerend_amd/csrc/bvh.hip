@@ -497,19 +497,6 @@ __global__ __launch_bounds__(1024) void k_fit_top(const float4* tris, NodeBuild*
   }
 }
 
-__global__ __launch_bounds__(256) void k_depth(const int32_t* leaf_parent, const NodeBuild* nodes,
-                                               int n, uint32_t* depth)
-{
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  uint32_t d = 0;
-  int node = leaf_parent[p];
-  while (node >= 0 && node < n - 1 && d < 1024u) {
-    ++d;
-    node = nodes[node].d.z;
-  }
-  atomicMax(depth, d);
-}
 
 // ---------------------------------------------------------------------------
 // host launchers
@@ -599,13 +586,5 @@ void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, co
   hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, tris, nodes, node_box, order, t);
 }
 
-void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,
-                  hipStream_t s)
-{
-  fill_u32(depth, 0u, 1, s);
-  if (n < 2) return;
-  hipLaunchKernelGGL(k_depth, dim3(blocks_for(n, 256)), dim3(256), 0, s, leaf_parent, nodes, n,
-                     depth);
-}
 
 }  // namespace ane

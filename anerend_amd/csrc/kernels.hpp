@@ -31,9 +31,6 @@ int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, 
 // Level-synchronous bottom-up boxes (no atomics, no inter-workgroup fences).
 void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
                 const std::vector<int32_t>& level_begin, hipStream_t s);
-void launch_depth(const int32_t* leaf_parent, const NodeBuild* nodes, int n, uint32_t* depth,
-                  hipStream_t s);
-
 // trace.hip
 struct TraceArgs {
   SceneK sc;
@@ -94,6 +91,9 @@ void launch_dequantize(int precision, const void* src, const float* scale, float
 // translate.hip
 void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, int dz,
                       hipStream_t s);
+// atlas <-> RGBA16F (binary16 per channel, round to nearest even), texels float4 <-> uint2
+void launch_atlas_to_half(const float4* src, uint2* dst, int64_t texels, hipStream_t s);
+void launch_half_to_atlas(const uint2* src, float4* dst, int64_t texels, hipStream_t s);
 
 // numeric formats (software RNE conversions shared by shade and quantize)
 __host__ __device__ __forceinline__ uint16_t f32_to_bf16(float x)

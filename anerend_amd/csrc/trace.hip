@@ -32,8 +32,11 @@ namespace ane {
 #ifndef ANE_TRACE_REFILL
 #define ANE_TRACE_REFILL 32
 #endif
+#ifndef ANE_TRACE_UNROLL
+#define ANE_TRACE_UNROLL 5  // traversal steps per refill check
+#endif
 #ifndef ANE_TRACE_WAVES
-#define ANE_TRACE_WAVES 8  // waves per SIMD the traversal is register-limited to
+#define ANE_TRACE_WAVES 7  // waves per SIMD the traversal is register-limited to
 #endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring)
@@ -331,18 +334,23 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
       if (__ballot(!exhausted) == 0ull) break;
       continue;
     }
-    // ---- one traversal step: only node, sp, the best hit and the budget change ----
+    // ---- traversal steps: only node, sp, the best hit and the budget change ----
+#pragma unroll
+    for (int u = 0; u < ANE_TRACE_UNROLL; ++u)
     if (ray >= 0 && node != kDone) {
       int32_t next = kPop;
       // guards: a malformed hierarchy ends the ray and raises the error flag
       // instead of reading out of bounds or looping forever
-      const bool bad = steps_left == 0u || (node >= 0 && node >= sc.ntri - 1) ||
-                       (node < 0 && ~node >= sc.ntri);
+      // internal node index < ntri - 1, leaf position ~node < ntri
+      const bool isnode = node >= 0;
+      const uint32_t idx = isnode ? (uint32_t)node : (uint32_t)~node;
+      const uint32_t lim = isnode ? (uint32_t)sc.ntri - 1u : (uint32_t)sc.ntri;
+      const bool bad = (steps_left == 0u) | (idx >= lim);
       --steps_left;
       if (bad) {
         *a.overflow = 1u;
         next = kDone;
-      } else if (node >= 0) {
+      } else if (isnode) {
         if (COUNT) {
           if (sph) ++sn; else ++cn;
         }

@@ -37,4 +37,46 @@ void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, in
                      g.atlas_h, 10 * g.nz, sx, sz, src, dst);
 }
 
+// ---------------------------------------------------------------------------
+// RGBA16F form of the atlas, the format the reference stores a baked tile in
+// (VulkanRenderer.cpp:2705-2770: R16G16B16A16_SFLOAT image copied to a buffer):
+// IEEE binary16 per channel, round to nearest even, texel order unchanged.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_atlas_to_half(const float4* src, uint2* dst, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = src[i];
+  const _Float16 h[4] = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+  uint16_t b[4];
+  __builtin_memcpy(b, h, 8);
+  dst[i] = make_uint2((uint32_t)b[0] | ((uint32_t)b[1] << 16), (uint32_t)b[2] | ((uint32_t)b[3] << 16));
+}
+
+__global__ __launch_bounds__(256) void k_half_to_atlas(const uint2* src, float4* dst, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint2 p = src[i];
+  const uint16_t b[4] = {(uint16_t)(p.x & 0xFFFFu), (uint16_t)(p.x >> 16), (uint16_t)(p.y & 0xFFFFu),
+                         (uint16_t)(p.y >> 16)};
+  _Float16 h[4];
+  __builtin_memcpy(h, b, 8);
+  dst[i] = make_float4((float)h[0], (float)h[1], (float)h[2], (float)h[3]);
+}
+
+void launch_atlas_to_half(const float4* src, uint2* dst, int64_t texels, hipStream_t s)
+{
+  if (texels <= 0) return;
+  hipLaunchKernelGGL(k_atlas_to_half, dim3((unsigned)((texels + 255) / 256)), dim3(256), 0, s, src, dst,
+                     texels);
+}
+
+void launch_half_to_atlas(const uint2* src, float4* dst, int64_t texels, hipStream_t s)
+{
+  if (texels <= 0) return;
+  hipLaunchKernelGGL(k_half_to_atlas, dim3((unsigned)((texels + 255) / 256)), dim3(256), 0, s, src, dst,
+                     texels);
+}
+
 }  // namespace ane

@@ -71,6 +71,8 @@ _SIGS = [
     ("ane_ddgi_atlas_set", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_atlas_device", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("ane_ddgi_translation_state", ctypes.c_int, [_vp, _vp, _i32]),
+    ("ane_ddgi_atlas_export_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_atlas_import_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_radiance_set", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("ane_ddgi_radiance_get", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_debug_rays", ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -234,6 +236,17 @@ class ProbeGrid:
         h = ctypes.c_int64()
         check(load().ane_ddgi_atlas_device(self._h, ctypes.byref(p), ctypes.byref(w), ctypes.byref(h)))
         return p.value, w.value, h.value
+
+    def atlas_rgba16f(self):
+        """Atlas as the reference's baked-tile texture: [H][W][4] IEEE binary16."""
+        a = np.zeros(self.config.atlas_shape, np.float16)
+        check(load().ane_ddgi_atlas_export_rgba16f(self._h, ptr(a), None))
+        return a
+
+    def set_atlas_rgba16f(self, value):
+        a = np.ascontiguousarray(value, np.float16)
+        assert a.shape == tuple(self.config.atlas_shape)
+        check(load().ane_ddgi_atlas_import_rgba16f(self._h, ptr(a), None))
 
     def translation_state(self, cell=None):
         c = np.zeros(2, np.int32) if cell is None else np.ascontiguousarray(cell, np.int32)

@@ -207,6 +207,13 @@ int ane_ddgi_atlas_set(ane_ddgi_ctx* ctx, const float* host_rgba, void* stream);
 int ane_ddgi_atlas_device(ane_ddgi_ctx* ctx, float** device_rgba, int64_t* width,
                           int64_t* height);
 int ane_ddgi_translation_state(ane_ddgi_ctx* ctx, int32_t* last_cell2, int32_t set);
+/* The atlas in the format the reference stores a baked tile
+ * (VulkanRenderer.cpp:2705-2770 downloadDDGIAtlas, asset::Texture RGBA16F_SFLOAT):
+ * width 10*probes_x, height 10*probes_z*probes_y, 4 x IEEE binary16 per texel,
+ * row-major; export rounds to nearest even (values beyond 65504 become inf),
+ * import widens exactly. */
+int ane_ddgi_atlas_export_rgba16f(ane_ddgi_ctx* ctx, uint16_t* host_rgba16f, void* stream);
+int ane_ddgi_atlas_import_rgba16f(ane_ddgi_ctx* ctx, const uint16_t* host_rgba16f, void* stream);
 
 /* ---- radiance (ProbeRaysIrrTex) in the context's precision ----
  * set: host FP32 [probe][3][ray] for the layers of `frame`, quantized on upload. */

@@ -208,3 +208,23 @@ def test_room8_matches_golden_fixture():
         assert sha(sh) == want["shadow_sha256"]
     ref = np.load(os.path.join(here, "room8_atlas.npz"))["atlas"]
     assert rel_err(grid.atlas[::3, ::3], ref) <= RTOL
+
+
+def test_atlas_rgba16f_export_import():
+    """The baked-tile format of the reference (RGBA16F): export equals an IEEE
+    binary16 round-to-nearest-even of the FP32 atlas bit for bit (numpy's
+    float16 cast), including values that round to inf; import widens exactly."""
+    scene, cfg, frame = scenegen.workload("room8")
+    ds = native.DeviceScene(scene)
+    grid = native.ProbeGrid(cfg)
+    grid.update(ds, frame)
+    a = grid.atlas
+    a[0, 0] = (65504.0, 65520.0, 1e-7, -3.0e-5)  # max finite, first value rounding to inf, subnormals
+    a[0, 1] = (1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, -0.0, 70000.0)  # ties to even, signed zero
+    grid.atlas = a
+    h = grid.atlas_rgba16f()
+    want = a.astype(np.float16)
+    assert np.array_equal(h.view(np.uint16), want.view(np.uint16))
+    fresh = native.ProbeGrid(cfg)
+    fresh.set_atlas_rgba16f(h)
+    assert np.array_equal(bits(fresh.atlas), bits(h.astype(np.float32)))
