@@ -726,6 +726,7 @@ static int frame_to_k(const ane_ddgi_ctx* c, const ane_ddgi_frame* f, FrameK* k)
   k->sun_l[0] = L.x;
   k->sun_l[1] = L.y;
   k->sun_l[2] = L.z;
+  k->sky_intensity = f->sky_intensity;
   return ANE_OK;
 }
 
@@ -886,6 +887,23 @@ extern "C" int ane_ddgi_atlas_import_rgba16f(ane_ddgi_ctx* c, const uint16_t* ho
   launch_half_to_atlas(stage, reinterpret_cast<float4*>(c->d_atlas[c->cur]), c->texels, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_gi_lookup(ane_ddgi_ctx* c, const ane_ddgi_frame* f, const ane_gbuffer_texel* gbuf,
+                                  int64_t n, float* out_rgba, void* stream)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  if (n < 0) return fail(ANE_ERR_INVALID_ARG, "negative texel count");
+  if (n > 0 && (!gbuf || !out_rgba)) return fail(ANE_ERR_INVALID_ARG, "NULL G-buffer or output");
+  if (!f) return fail(ANE_ERR_INVALID_ARG, "frame is NULL");
+  FrameK fk;
+  ane_ddgi_frame full = *f;
+  full.layer_begin = 0;  // the lookup reads the whole atlas; the layer range is not used
+  full.layer_count = c->g.ny;
+  ANE_TRY(frame_to_k(c, &full, &fk));
+  launch_gi_lookup(c->g, fk, c->d_atlas[c->cur], gbuf, n, out_rgba, S(stream));
+  ANE_CHECK_HIP(hipGetLastError());
   return ANE_OK;
 }
 

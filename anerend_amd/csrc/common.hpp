@@ -65,6 +65,7 @@ struct FrameK {
   int32_t bake_tile[4];
   float rot[16];
   float sun_l[3];      // normalize(-light_dir): direction to the sun (pbr_light.glsl:103)
+  float sky_intensity; // ubo.skyIntensity as given (deferred_tiled.comp:272)
 };
 
 struct GridK {

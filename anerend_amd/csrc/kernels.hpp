@@ -88,6 +88,10 @@ void launch_quantize(int precision, const float* src, void* dst, float* scale, i
 void launch_dequantize(int precision, const void* src, const float* scale, float* dst,
                        int num_probes, int n, hipStream_t s);
 
+// deferred GI lookup: gbuf = n x 3 float4 (ane_gbuffer_texel), out = n x float4
+void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const void* gbuf, int64_t n,
+                      float* out, hipStream_t s);
+
 // translate.hip
 void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, int dz,
                       hipStream_t s);

@@ -39,7 +39,7 @@ namespace ane {
 #define ANE_TRACE_WAVES 7  // waves per SIMD the traversal is register-limited to
 #endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
-static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring)
+static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring, any size)
 static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         if (b0 && b1) {
           // push the far child (LDS ring; the oldest entry spills when full)
           const int32_t farc = sw ? L : R;
-          const int slot = sp & (kSS - 1);
+          const int slot = (int)((uint32_t)sp % (uint32_t)kSS);
           if (sp >= kSS) {
             const int si = sp - kSS;
             if (si < a.spill_depth) {
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
           next = kDone;
         } else {
           --sp;
-          const int slot = sp & (kSS - 1);
+          const int slot = (int)((uint32_t)sp % (uint32_t)kSS);
           next = stk[slot][tid];
           if (sp >= kSS) stk[slot][tid] = a.spill[(int64_t)(sp - kSS) * a.total_lanes + gl];
         }
@@ -780,6 +780,44 @@ void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float
   else
     shade_launch<256>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
+}
+
+// ---------------------------------------------------------------------------
+// deferred GI lookup (deferred_tiled.comp:264-298): ambient light of G-buffer
+// texels from the atlas, through the shading's calcIndirectDiffuseLight
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gi_lookup(GridK g, FrameK f, const float4* atlas,
+                                                   const float4* gbuf, int64_t n, float4* out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = gbuf[3 * i], b = gbuf[3 * i + 1], c = gbuf[3 * i + 2];
+  const f3 pos = mk(a.x, a.y, a.z), N = mk(b.x, b.y, b.z), albedo = mk(c.x, c.y, c.z);
+  const float sh = clampf(a.w, 0.3f, 1.0f);
+  f3 amb;
+  if (f.flags & ANE_FLAG_DDGI) {
+    const f3 cam = mk(f.cam[0], f.cam[1], f.cam[2]);
+    const f3 d = sub(cam, pos);
+    const float dist = sqrtf(dot(d, d));
+    if (dist > (float)g.nx * g.step[0] / 2.0f) {
+      amb = muls(mul(muls(mk(0.3f, 0.3f, 0.3f), f.sky_intensity), albedo), sh);
+    } else {
+      const f3 V = normalize(sub(cam, pos));
+      amb = calc_indirect(g, f, atlas, N, albedo, b.w, c.w, pos, V);
+    }
+  } else {
+    amb = muls(mul(mk(0.3f, 0.3f, 0.3f), albedo), sh);
+  }
+  out[i] = make_float4(amb.x, amb.y, amb.z, 0.0f);
+}
+
+void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const void* gbuf, int64_t n,
+                      float* out, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gi_lookup, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, f,
+                     reinterpret_cast<const float4*>(atlas), reinterpret_cast<const float4*>(gbuf), n,
+                     reinterpret_cast<float4*>(out));
 }
 
 }  // namespace ane

@@ -72,6 +72,7 @@ _SIGS = [
     ("ane_ddgi_atlas_device", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("ane_ddgi_translation_state", ctypes.c_int, [_vp, _vp, _i32]),
     ("ane_ddgi_atlas_export_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
+    ("ane_ddgi_gi_lookup", ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
     ("ane_ddgi_atlas_import_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_radiance_set", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("ane_ddgi_radiance_get", ctypes.c_int, [_vp, _vp, _vp]),
@@ -109,6 +110,39 @@ def check(status):
     if status != ANE_OK:
         raise AneError(status, load().ane_last_error().decode())
     return status
+
+
+class DeviceBuffer:
+    """Plain device allocation for entry points that take device pointers."""
+
+    _hip = None
+
+    def __init__(self, nbytes):
+        if DeviceBuffer._hip is None:
+            DeviceBuffer._hip = ctypes.CDLL("libamdhip64.so")
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        if DeviceBuffer._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.nbytes, 1))) != 0:
+            raise AneError(2, f"hipMalloc of {self.nbytes} bytes failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if DeviceBuffer._hip.hipMemcpy(self.ptr, ptr(a), ctypes.c_size_t(a.nbytes), 1) != 0:
+            raise AneError(4, "hipMemcpy to device failed")
+
+    def download(self, a):
+        assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
+        if DeviceBuffer._hip.hipMemcpy(ptr(a), self.ptr, ctypes.c_size_t(a.nbytes), 2) != 0:
+            raise AneError(4, "hipMemcpy to host failed")
+        return a
+
+    def close(self):
+        if self.ptr.value:
+            DeviceBuffer._hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    __del__ = close
 
 
 def ray_directions(n, rotation16):
@@ -247,6 +281,20 @@ class ProbeGrid:
         a = np.ascontiguousarray(value, np.float16)
         assert a.shape == tuple(self.config.atlas_shape)
         check(load().ane_ddgi_atlas_import_rgba16f(self._h, ptr(a), None))
+
+    def gi_lookup(self, frame, gbuf):
+        """Deferred GI lookup of G-buffer texels (GBUFFER_DTYPE or n x 12 float32) -> n x 4."""
+        g = np.ascontiguousarray(gbuf).view(np.float32).reshape(-1, 12)
+        n = len(g)
+        src, dst = DeviceBuffer(g.nbytes), DeviceBuffer(16 * n)
+        try:
+            src.upload(g)
+            check(load().ane_ddgi_gi_lookup(self._h, ctypes.byref(frame), src.ptr, n, dst.ptr, None))
+            synchronize()
+            return dst.download(np.zeros((n, 4), np.float32))
+        finally:
+            src.close()
+            dst.close()
 
     def translation_state(self, cell=None):
         c = np.zeros(2, np.int32) if cell is None else np.ascontiguousarray(cell, np.int32)

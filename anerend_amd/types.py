@@ -42,6 +42,12 @@ INSTANCE_DTYPE = np.dtype([
 ])
 assert INSTANCE_DTYPE.itemsize == 64
 
+GBUFFER_DTYPE = np.dtype([  # ane_gbuffer_texel, 48 B (deferred_tiled.comp:264-298 inputs)
+    ("pos", "<f4", 3), ("shadow", "<f4"), ("normal", "<f4", 3), ("metallic", "<f4"),
+    ("albedo", "<f4", 3), ("roughness", "<f4"),
+])
+assert GBUFFER_DTYPE.itemsize == 48
+
 HIT_DTYPE = np.dtype([("tri", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 
 # scene_ubo.glsl:1-12

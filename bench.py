@@ -176,6 +176,51 @@ def conv_study(cfg0, frame, reps=20):
     return out
 
 
+def gi_lookup_study(grid, frame, width=1920, height=1080, reps=20):
+    """Deferred GI lookup of a 1920x1080 G-buffer (seeded texels within 24 m of the
+    camera, so that both the probe and the far-field branch run) on the current
+    atlas; kernel time from HIP events over ``reps`` launches."""
+    import ctypes
+    from anerend_amd import native
+    from anerend_amd.types import GBUFFER_DTYPE
+    n = width * height
+    rng = np.random.default_rng(6)
+    g = np.zeros(n, GBUFFER_DTYPE)
+    cam = np.array(frame.camera_pos[:3], np.float32)
+    g["pos"] = cam + rng.uniform(-24, 24, (n, 3)).astype(np.float32) * np.float32([1, 0.3, 1])
+    nrm = rng.normal(size=(n, 3))
+    g["normal"] = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    g["albedo"] = rng.uniform(0, 1, (n, 3))
+    g["roughness"] = rng.uniform(0.05, 1, n)
+    g["shadow"] = rng.uniform(0, 1, n)
+    src, dst = native.DeviceBuffer(g.nbytes), native.DeviceBuffer(16 * n)
+    hip = native.DeviceBuffer._hip
+    L = native.load()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    try:
+        src.upload(g)
+        native.check(L.ane_ddgi_gi_lookup(grid.handle, ctypes.byref(frame), src.ptr, n, dst.ptr, None))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            native.check(L.ane_ddgi_gi_lookup(grid.handle, ctypes.byref(frame), src.ptr, n, dst.ptr, None))
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        t = ms.value / reps
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        src.close()
+        dst.close()
+    algo = n * (48 + 16)
+    return {"texels": n, "ms": round(t, 5), "texels_per_s": round(n / (t * 1e-3), 1),
+            "gbs": round(algo / (t * 1e-3) / 1e9, 1), "hbm_frac": round(algo / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_texel": "48 B G-buffer in + 16 B out (atlas taps served from cache)"}
+
+
 def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
     """Error of one convolution of ``grid_conv`` (its precision) against a float64
     GEMM of the FP32 radiance of the same frame traced by ``grid_ref`` (FP32) from
@@ -461,6 +506,10 @@ def main():
                 "traffic_source": "profiles/trace_terrain1m_traffic.md" if traffic else None,
                 "algorithmic_bytes": int(algo[dom]), "mean_ms": round(mean[dom], 4)}
 
+    log("[bench] deferred GI lookup, 1920x1080")
+    gi = gi_lookup_study(grid, frame_at(0))
+    log(f"[bench] gi lookup {json.dumps(gi)}")
+
     conv = None
     if not args.no_conv_study:
         log("[bench] convolution study (FP32 / BF16 / FP8 vs FP64)")
@@ -495,6 +544,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "conv_study": conv,
+        "gi_lookup": gi,
     }
     print(json.dumps(out), flush=True)
 

@@ -148,6 +148,18 @@ typedef struct ane_ddgi_stats {
   int64_t device_bytes;   /* HBM held by the context */
 } ane_ddgi_stats;
 
+/* One G-buffer texel for the deferred GI lookup (deferred_tiled.comp:264-298 inputs):
+ * world position, the sun shadow factor of the pixel, world normal, metallic,
+ * albedo (linear), roughness. */
+typedef struct ane_gbuffer_texel {
+  float pos[3];
+  float shadow;
+  float normal[3];
+  float metallic;
+  float albedo[3];
+  float roughness;
+} ane_gbuffer_texel;
+
 typedef struct ane_ddgi_ctx ane_ddgi_ctx;
 typedef struct ane_scene ane_scene;
 
@@ -200,6 +212,16 @@ int ane_ddgi_convolve(ane_ddgi_ctx* ctx, const ane_ddgi_frame* frame, void* stre
 /* trace then convolve. */
 int ane_ddgi_update(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_frame* frame,
                     void* stream);
+
+/* Deferred GI lookup (deferred_tiled.comp:264-298 with calcIndirectDiffuseLight,
+ * pbr_light.glsl:365-388): ambient light of n G-buffer texels from the current
+ * atlas.  With ANE_FLAG_DDGI set in frame->flags: texels farther from the
+ * camera than probes_x * probe_step.x / 2 get skyIntensity * 0.3 * albedo *
+ * clamp(shadow, 0.3, 1), the others kD * sampleProbe(pos, normal) * albedo;
+ * without it 0.3 * albedo * clamp(shadow, 0.3, 1).  gbuf and out_rgba
+ * (n x float4, w = 0) are DEVICE pointers, ordered on `stream`. */
+int ane_ddgi_gi_lookup(ane_ddgi_ctx* ctx, const ane_ddgi_frame* frame, const ane_gbuffer_texel* gbuf,
+                       int64_t n, float* out_rgba, void* stream);
 
 /* ---- probe state (ProbeRaysConvTex) ---- */
 int ane_ddgi_atlas_get(ane_ddgi_ctx* ctx, float* host_rgba, void* stream);

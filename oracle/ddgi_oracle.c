@@ -879,6 +879,36 @@ static f3 calc_indirect(const ora_config* c, const ora_frame* f, const float* at
   return mul(kD, diffuse);
 }
 
+/* deferred_tiled.comp:264-298 (live atlas; baked tiles are out of scope):
+ * gbuf = n x 12 floats (pos, shadow, normal, metallic, albedo, roughness),
+ * out = n x 4 floats (ambient rgb, 0). */
+void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, const float* gbuf,
+                   int64_t n, float* out)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float* t = gbuf + 12 * i;
+    f3 pos = mk(t[0], t[1], t[2]), N = mk(t[4], t[5], t[6]), albedo = mk(t[8], t[9], t[10]);
+    float sh = clampf(t[3], 0.3f, 1.0f);
+    f3 amb;
+    if (f->flags & ORA_FLAG_DDGI) {
+      f3 cam = mk(f->camera_pos[0], f->camera_pos[1], f->camera_pos[2]);
+      f3 d = sub(cam, pos);
+      float dist = sqrtf(dot(d, d)); /* distance(ubo.cameraPos.xyz, fragWorldPos) */
+      if (dist > (float)c->probes_x * c->probe_step[0] / 2.0f)
+        amb = muls(mul(muls(mk(0.3f, 0.3f, 0.3f), f->sky_intensity), albedo), sh);
+      else
+        amb = calc_indirect(c, f, atlas, N, albedo, t[7], t[11], pos, cam);
+    } else {
+      amb = muls(mul(mk(0.3f, 0.3f, 0.3f), albedo), sh);
+    }
+    out[4 * i + 0] = amb.x;
+    out[4 * i + 1] = amb.y;
+    out[4 * i + 2] = amb.z;
+    out[4 * i + 3] = 0.0f;
+  }
+}
+
 void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, const float* atlas,
                    const float* origin, const ora_hit* hit, float* out, int32_t* shadowed,
                    uint64_t* counters)

@@ -48,7 +48,8 @@
 extern "C" {
 #endif
 
-/* scene_ubo.glsl:6,12 */
+/* scene_ubo.glsl:5,6,12 */
+#define ORA_FLAG_DDGI (1u << 4)
 #define ORA_FLAG_DDGI_MULTI (1u << 5)
 #define ORA_FLAG_BAKE (1u << 11)
 
@@ -147,6 +148,10 @@ void ora_trace_bruteforce(const ora_scene* s, const float* orig, const float* di
 
 /* ---- shading (rchit/rmiss) ---- */
 void ora_sky_radiance(const ora_frame* fr, float* out3);
+/* deferred GI lookup (deferred_tiled.comp:264-298): gbuf n x 12 floats
+ * (pos, shadow, normal, metallic, albedo, roughness) -> out n x 4 (rgb, 0) */
+void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, const float* gbuf,
+                   int64_t n, float* out);
 void ora_shade_hit(const ora_scene* s, const ora_config* cfg, const ora_frame* fr,
                    const float* atlas, const float* origin, const ora_hit* hit, float* out3,
                    int32_t* shadowed, uint64_t* counters);

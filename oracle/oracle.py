@@ -40,6 +40,7 @@ def lib():
         L.ora_scene_max_depth.argtypes = [vp]
         L.ora_scene_world_tris.argtypes = [vp, vp]
         L.ora_scene_bvh.argtypes = [vp] * 7
+        L.ora_gi_lookup.argtypes = [vp, vp, vp, vp, i64, vp]
         L.ora_trace.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp, vp]
         L.ora_trace_bruteforce.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp]
         L.ora_fibonacci_dirs.argtypes = [i32, vp]
@@ -207,6 +208,14 @@ def sample_probe(cfg, frame, atlas, pos, normal):
     p = np.ascontiguousarray(pos, np.float32)
     n = np.ascontiguousarray(normal, np.float32)
     lib().ora_sample_probe(_ref(cfg), _ref(frame), ptr(atlas), ptr(p), ptr(n), ptr(out))
+    return out
+
+
+def gi_lookup(cfg, frame, atlas, gbuf):
+    """Deferred GI lookup of G-buffer texels (n x 12 float32) -> n x 4."""
+    g = np.ascontiguousarray(gbuf, np.float32).reshape(-1, 12)
+    out = np.zeros((len(g), 4), np.float32)
+    lib().ora_gi_lookup(_ref(cfg), _ref(frame), ptr(atlas), ptr(g), len(g), ptr(out))
     return out
 
 

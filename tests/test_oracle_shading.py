@@ -143,3 +143,36 @@ def test_emissive_and_miss():
     miss["tri"] = -1
     out, sh = osc.shade_hit(CFG, frame((0, -1, 0)), atlas, origin, miss[0])
     assert sh == -1 and np.allclose(out, 0.3, rtol=1e-5)
+
+
+def _gbuf(pos, normal, albedo, metallic=0.0, rough=0.5, shadow=1.0):
+    g = np.zeros(12, np.float32)
+    g[0:3], g[3], g[4:7], g[7], g[8:11], g[11] = pos, shadow, normal, metallic, albedo, rough
+    return g
+
+
+def test_gi_lookup_closed_forms():
+    """Deferred GI lookup (deferred_tiled.comp:264-298) at cases that fix its value:
+    constant atlas c, view along the normal -> kD * albedo * c * pi/2 with
+    kD = (1 - F0)(1 - metallic); beyond half the grid width -> skyIntensity * 0.3 *
+    albedo * clamp(shadow, 0.3, 1); DDGI off -> 0.3 * albedo * clamp(shadow, 0.3, 1)."""
+    cfg = Config.make((8, 4, 8), step=(1, 2, 1), rays=64)
+    fr = scenegen.make_frame(3, camera=(0.5, 3.0, 0.5), config=cfg, flags=FLAG_DDGI)
+    fr.sky_intensity = 0.8
+    atlas = O.new_atlas(cfg)
+    c = np.array([0.6, 0.3, 1.1], np.float32)
+    atlas[..., :3] = c
+    albedo = np.array([0.5, 0.7, 0.2], np.float32)
+    cam = np.array(fr.camera_pos[:3], np.float32)
+    near = _gbuf(cam - np.float32([0, 2.0, 0]), (0, 1, 0), albedo)           # V = N
+    far = _gbuf(cam + np.float32([30.0, 0, 0]), (0, 1, 0), albedo, shadow=0.1)
+    out = O.gi_lookup(cfg, fr, atlas, np.stack([near, far]))
+    kd = (1 - 0.04) * (1 - 0.0)
+    assert np.allclose(out[0, :3], kd * albedo * c * 0.5 * 3.1415926, rtol=2e-6)
+    assert np.allclose(out[1, :3], 0.8 * 0.3 * albedo * 0.3, rtol=1e-6)
+    fr.flags = 0
+    off = O.gi_lookup(cfg, fr, atlas, np.stack([_gbuf(cam, (0, 1, 0), albedo, shadow=0.5),
+                                                _gbuf(cam, (0, 1, 0), albedo, shadow=2.0)]))
+    assert np.allclose(off[0, :3], 0.3 * albedo * 0.5, rtol=1e-6)
+    assert np.allclose(off[1, :3], 0.3 * albedo * 1.0, rtol=1e-6)
+    assert np.all(out[:, 3] == 0) and np.all(off[:, 3] == 0)
