@@ -254,6 +254,16 @@ struct ane_scene {
   int up_slot = 0;
   int64_t bytes = 0;
   SceneK view{};
+  // shadow-mask workspace (grows to the largest image seen)
+  int64_t shadow_px = 0;
+  float* d_sh_orig = nullptr;
+  float* d_sh_dir = nullptr;
+  ane_hit* d_sh_hits = nullptr;
+  uint32_t* d_sh_work = nullptr;
+  unsigned long long* d_sh_stats = nullptr;
+  uint32_t* d_sh_ov = nullptr;
+  int32_t* d_sh_spill = nullptr;
+  int64_t sh_spill_words = 0;
 };
 
 static void scene_free(ane_scene* s)
@@ -264,6 +274,8 @@ static void scene_free(ane_scene* s)
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_scratch);
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
+  dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
+  dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (int k = 0; k < 2; ++k) {
@@ -1040,6 +1052,78 @@ extern "C" int ane_ddgi_stats_reset(ane_ddgi_ctx* c)
   for (auto& t : c->timers) {
     ANE_TRY(t.harvest_all());
     t.reset();
+  }
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t width, int32_t height,
+                                     const float* ivp16, const float* light3, float* out_vis,
+                                     uint64_t* counters2, void* stream)
+{
+  if (!s || !ivp16 || !light3) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (width < 0 || height < 0) return fail(ANE_ERR_INVALID_ARG, "negative image size");
+  const int64_t n = (int64_t)width * height;
+  if (n == 0) return ANE_OK;
+  if (!depth || !out_vis) return fail(ANE_ERR_INVALID_ARG, "NULL depth or output");
+  if (n >= ((int64_t)1 << 31)) return fail(ANE_ERR_INVALID_ARG, "image too large");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(ivp16[k])) return fail(ANE_ERR_INVALID_ARG, "non-finite inverse view-projection");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(light3[k])) return fail(ANE_ERR_INVALID_ARG, "non-finite light direction");
+  hipStream_t st = S(stream);
+  if (n > s->shadow_px) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits);
+    s->shadow_px = 0;
+    ANE_TRY(dalloc(&s->d_sh_orig, 3 * n, nullptr));
+    ANE_TRY(dalloc(&s->d_sh_dir, 3 * n, nullptr));
+    ANE_TRY(dalloc(&s->d_sh_hits, n, nullptr));
+    s->shadow_px = n;
+  }
+  if (!s->d_sh_work) {
+    ANE_TRY(dalloc(&s->d_sh_work, 1, nullptr));
+    ANE_TRY(dalloc(&s->d_sh_stats, 2, nullptr));
+    ANE_TRY(dalloc(&s->d_sh_ov, 1, nullptr));
+  }
+  const int64_t lanes = trace_lanes_per_launch();
+  const int64_t sd = s->depth - trace_stack_lds() + 1 > 0 ? s->depth - trace_stack_lds() + 1 : 0;
+  if (sd * lanes > s->sh_spill_words) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_sh_spill);
+    s->sh_spill_words = 0;
+    ANE_TRY(dalloc(&s->d_sh_spill, sd * lanes, nullptr));
+    s->sh_spill_words = sd * lanes;
+  }
+  launch_shadow_rays(depth, width, height, ivp16, light3, s->d_sh_orig, s->d_sh_dir, st);
+  fill_u32(s->d_sh_work, 0u, 1, st);
+  fill_u32(reinterpret_cast<uint32_t*>(s->d_sh_stats), 0u, 4, st);
+  fill_u32(s->d_sh_ov, 0u, 1, st);
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g.rays = 1;
+  a.hits = s->d_sh_hits;
+  a.work = s->d_sh_work;
+  a.stats = s->d_sh_stats;
+  a.overflow = s->d_sh_ov;
+  a.spill = s->d_sh_spill;
+  a.spill_depth = (int32_t)sd;
+  a.nrays = (int32_t)n;
+  a.ray_orig = s->d_sh_orig;
+  a.ray_dir = s->d_sh_dir;
+  a.t_min = 0.1f;  // shadow.rgen:66-67
+  a.t_max = 50.0f;
+  launch_trace(a, true, true, counters2 != nullptr, st);
+  launch_shadow_resolve(depth, s->d_sh_hits, n, out_vis, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  if (counters2) {
+    unsigned long long c2[2] = {0, 0};
+    uint32_t ov = 0;
+    ANE_CHECK_HIP(hipMemcpyAsync(c2, s->d_sh_stats, 16, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipMemcpyAsync(&ov, s->d_sh_ov, 4, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    counters2[0] = c2[0];
+    counters2[1] = c2[1];
+    if (ov) return fail(ANE_ERR_DEVICE, "traversal stack overflow");
   }
   return ANE_OK;
 }

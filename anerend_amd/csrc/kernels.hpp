@@ -92,6 +92,11 @@ void launch_dequantize(int precision, const void* src, const float* scale, float
 void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const void* gbuf, int64_t n,
                       float* out, hipStream_t s);
 
+// RT hard shadows: per-pixel rays (orig, dirs: W*H*3 floats) and visibility
+void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
+                        float* orig, float* dirs, hipStream_t s);
+void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s);
+
 // translate.hip
 void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, int dz,
                       hipStream_t s);

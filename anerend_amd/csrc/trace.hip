@@ -308,9 +308,12 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
           exhausted = true;
         } else if (rank < avail) {
           const uint32_t gidx = cbeg + rank;
+          bool none = false;
           if (RAYS) {
             out = (int32_t)gidx;
-            ray_setup(r, ld3(a.ray_orig + 3 * (int64_t)gidx), ld3(a.ray_dir + 3 * (int64_t)gidx));
+            const f3 d = ld3(a.ray_dir + 3 * (int64_t)gidx);
+            none = d.x == 0.0f && d.y == 0.0f && d.z == 0.0f;  // a zero direction is no ray
+            ray_setup(r, ld3(a.ray_orig + 3 * (int64_t)gidx), d);
           } else {
             // independent loads: processing-order ray setup, probe origin, output slot
             const uint32_t ql = (uint32_t)(((uint64_t)gidx * a.div_magic) >> a.div_shift);
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
           sp = 0;
           steps_left = budget;
           ray = (int32_t)gidx;
-          node = root;
+          node = none ? kDone : root;
         }
       }
       cbeg += need < avail ? need : avail;
@@ -818,6 +821,66 @@ void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const
   hipLaunchKernelGGL(k_gi_lookup, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, f,
                      reinterpret_cast<const float4*>(atlas), reinterpret_cast<const float4*>(gbuf), n,
                      reinterpret_cast<float4*>(out));
+}
+
+// ---------------------------------------------------------------------------
+// RT hard shadows (shadow.rgen:29-82): rays from the depth buffer's world
+// positions toward the sun, then visibility from the any-hit results
+// ---------------------------------------------------------------------------
+struct Mat4 {
+  float m[16];  // column-major
+};
+
+__global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, int H, Mat4 ivp, f3 dir,
+                                                     float* orig, float* dirs)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)W * H) return;
+  const int y = (int)(i / W), x = (int)(i - (int64_t)y * W);
+  const float d = depth[i];
+  // pixelToUv (helpers.glsl:1-4), then calcWorldPos (shadow.rgen:20-27)
+  const float u = (2.0f * (float)x + 1.0f) / (2.0f * (float)W);
+  const float v = (2.0f * (float)y + 1.0f) / (2.0f * (float)H);
+  const float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
+  const float* m = ivp.m;
+  const float px = m[0] * cx + m[4] * cy + m[8] * d + m[12];
+  const float py = m[1] * cx + m[5] * cy + m[9] * d + m[13];
+  const float pz = m[2] * cx + m[6] * cy + m[10] * d + m[14];
+  const float pw = m[3] * cx + m[7] * cy + m[11] * d + m[15];
+  orig[3 * i + 0] = px / pw;
+  orig[3 * i + 1] = py / pw;
+  orig[3 * i + 2] = pz / pw;
+  // sky pixels get a zero direction, which the traversal takes as no ray
+  const bool sky = d == 1.0f;
+  dirs[3 * i + 0] = sky ? 0.0f : dir.x;
+  dirs[3 * i + 1] = sky ? 0.0f : dir.y;
+  dirs[3 * i + 2] = sky ? 0.0f : dir.z;
+}
+
+__global__ __launch_bounds__(256) void k_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n,
+                                                        float* vis)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vis[i] = (depth[i] == 1.0f || hits[i].tri < 0) ? 1.0f : 0.0f;
+}
+
+void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
+                        float* orig, float* dirs, hipStream_t s)
+{
+  Mat4 m;
+  for (int k = 0; k < 16; ++k) m.m[k] = ivp16[k];
+  const f3 dir = mk(-light3[0], -light3[1], -light3[2]);
+  const int64_t n = (int64_t)W * H;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_shadow_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, W, H, m, dir,
+                     orig, dirs);
+}
+
+void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_shadow_resolve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, hits, n, vis);
 }
 
 }  // namespace ane

@@ -82,6 +82,7 @@ _SIGS = [
     ("ane_ddgi_stats_reset", ctypes.c_int, [_vp]),
     ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
+    ("ane_scene_shadow_mask", ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("ane_scene_trace_rays", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
     ("ane_ddgi_ray_directions", ctypes.c_int, [_i32, _vp, _vp]),
 ]
@@ -205,6 +206,25 @@ class DeviceScene:
         out = np.zeros((n, 9), np.float32)
         check(load().ane_scene_debug_world_tris(self._h, ptr(out)))
         return out
+
+    def shadow_mask(self, depth, inv_view_proj16, light_dir3, count=False):
+        """RT hard shadows of a host depth buffer [h][w] -> visibility [h][w] (and counters)."""
+        d = np.ascontiguousarray(depth, np.float32)
+        h, w = d.shape
+        ivp = np.ascontiguousarray(inv_view_proj16, np.float32)
+        light = np.ascontiguousarray(light_dir3, np.float32)
+        src, dst = DeviceBuffer(d.nbytes), DeviceBuffer(d.nbytes)
+        ctr = np.zeros(2, np.uint64)
+        try:
+            src.upload(d)
+            check(load().ane_scene_shadow_mask(self._h, src.ptr, w, h, ptr(ivp), ptr(light), dst.ptr,
+                                               ptr(ctr) if count else None, None))
+            synchronize()
+            vis = dst.download(np.zeros((h, w), np.float32))
+        finally:
+            src.close()
+            dst.close()
+        return (vis, ctr) if count else vis
 
     def trace_rays(self, orig, dirs, tmin=0.1, tmax=50.0, any_hit=False, count=False, stream=None):
         orig = np.ascontiguousarray(orig, np.float32)

@@ -221,6 +221,82 @@ def gi_lookup_study(grid, frame, width=1920, height=1080, reps=20):
             "bytes_per_texel": "48 B G-buffer in + 16 B out (atlas taps served from cache)"}
 
 
+def view_proj(eye, target, fovy_deg=60.0, aspect=16 / 9, near=0.1, far=500.0):
+    """Column-major Vulkan-style (depth 0..1) view-projection and its inverse, float64."""
+    eye, target = np.asarray(eye, float), np.asarray(target, float)
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, [0.0, 1.0, 0.0])
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = s, u, -f
+    V[:3, 3] = -V[:3, :3] @ eye
+    t = 1.0 / np.tan(np.radians(fovy_deg) / 2)
+    P = np.zeros((4, 4))
+    P[0, 0], P[1, 1] = t / aspect, t
+    P[2, 2], P[2, 3], P[3, 2] = far / (near - far), -far * near / (far - near), -1.0
+    VP = P @ V
+    return VP, np.linalg.inv(VP)
+
+
+def shadow_mask_study(ds, frame, width=1920, height=1080, reps=10):
+    """RT hard shadows (shadow.rgen) of a 1920x1080 view of the scene: the depth
+    buffer comes from camera rays cast through the library, then the shadow mask
+    is timed with HIP events over ``reps`` launches."""
+    import ctypes
+    from anerend_amd import native
+    eye = np.array([0.5, 20.0, 0.5])
+    VP, IVP = view_proj(eye, (60.0, 0.0, 60.0))
+    y, x = np.mgrid[0:height, 0:width]
+    uv = np.stack([(2 * x + 1) / (2 * width), (2 * y + 1) / (2 * height)], -1).reshape(-1, 2)
+    clip = np.concatenate([uv * 2 - 1, np.ones((len(uv), 1)), np.ones((len(uv), 1))], 1)
+    far = clip @ IVP.T
+    far = far[:, :3] / far[:, 3:]
+    d = far - eye
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    org = np.tile(eye.astype(np.float32), (len(d), 1))
+    hits = ds.trace_rays(org, d, tmin=0.0, tmax=490.0)
+    hit = hits["tri"] >= 0
+    pos = eye + d.astype(float) * hits["t"][:, None].astype(float)
+    ndc = np.concatenate([pos, np.ones((len(pos), 1))], 1) @ VP.T
+    depth = np.where(hit, ndc[:, 2] / ndc[:, 3], 1.0).astype(np.float32).reshape(height, width)
+    ivp16 = IVP.T.ravel().astype(np.float32)
+    light = np.array(frame.light_dir[:3], np.float32)
+    _, ctr = ds.shadow_mask(depth, ivp16, light, count=True)
+    n = width * height
+    src, dst = native.DeviceBuffer(4 * n), native.DeviceBuffer(4 * n)
+    hip = native.DeviceBuffer._hip
+    L = native.load()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    try:
+        src.upload(depth)
+        lp = light.ctypes.data_as(ctypes.c_void_p)
+        ip = ivp16.ctypes.data_as(ctypes.c_void_p)
+        native.check(L.ane_scene_shadow_mask(ds.handle, src.ptr, width, height, ip, lp, dst.ptr, None, None))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            native.check(L.ane_scene_shadow_mask(ds.handle, src.ptr, width, height, ip, lp, dst.ptr, None, None))
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        t = ms.value / reps
+        vis = dst.download(np.zeros((height, width), np.float32))
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        src.close()
+        dst.close()
+    rays = int((depth != 1.0).sum())
+    algo = int(ctr[0]) * NODE_BYTES + int(ctr[1]) * TRI_BYTES + n * (4 + 24 + 16 + 4)
+    return {"pixels": n, "shadow_rays": rays, "ms": round(t, 5), "rays_per_s": round(rays / (t * 1e-3), 1),
+            "shadowed_fraction": round(float((vis == 0).mean()), 4), "node_visits": int(ctr[0]),
+            "triangle_tests": int(ctr[1]), "algorithmic_gbs": round(algo / (t * 1e-3) / 1e9, 1)}
+
+
 def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
     """Error of one convolution of ``grid_conv`` (its precision) against a float64
     GEMM of the FP32 radiance of the same frame traced by ``grid_ref`` (FP32) from
@@ -510,6 +586,10 @@ def main():
     gi = gi_lookup_study(grid, frame_at(0))
     log(f"[bench] gi lookup {json.dumps(gi)}")
 
+    log("[bench] RT hard shadows, 1920x1080")
+    shadows = shadow_mask_study(ds, frame_at(0))
+    log(f"[bench] shadow mask {json.dumps(shadows)}")
+
     conv = None
     if not args.no_conv_study:
         log("[bench] convolution study (FP32 / BF16 / FP8 vs FP64)")
@@ -545,6 +625,7 @@ def main():
         "cpu_baseline": cpu,
         "conv_study": conv,
         "gi_lookup": gi,
+        "shadow_mask": shadows,
     }
     print(json.dumps(out), flush=True)
 

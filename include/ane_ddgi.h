@@ -256,10 +256,23 @@ int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
 int ane_scene_debug_bvh(const ane_scene* scene, uint64_t* keys, uint32_t* ids, int32_t* left,
                         int32_t* right, int32_t* parent, float* boxes);
 int ane_scene_debug_world_tris(const ane_scene* scene, float* tris9);
-/* Trace arbitrary rays (host arrays) against a scene: closest or any hit. */
+/* Trace arbitrary rays (host arrays) against a scene: closest or any hit.  A
+ * zero direction is no ray (a miss, no traversal). */
 int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const float* dirs,
                          int64_t num_rays, float t_min, float t_max, int32_t any_hit,
                          ane_hit* hits, uint64_t* counters2, void* stream);
+
+/* RT hard shadows (ShadowRayTracingPass, shadow.rgen:29-82): for every pixel of a
+ * width x height depth buffer, world position = (invViewProj * (uv * 2 - 1, depth, 1)).xyz / w
+ * with uv = pixelToUv(pixel) (helpers.glsl:1-4), then an any-hit ray from it along
+ * -light_dir (not normalized, as the reference) over [0.1, 50]: visibility 0 if
+ * something is hit, else 1; depth == 1 (sky) gives 1 without a ray.
+ * inv_view_proj16 is column-major; depth and out_visibility are DEVICE pointers
+ * (width * height floats), ordered on `stream`.  counters2 (host, nullable)
+ * receives node visits and triangle tests (the stream is then synchronized). */
+int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, int32_t height,
+                          const float* inv_view_proj16, const float* light_dir3, float* out_visibility,
+                          uint64_t* counters2, void* stream);
 
 /* ---- host-only helpers (no device work) ---- */
 /* Fibonacci-lattice ray directions rotated by mat3(rot16) (rgen:35-46,86-88). */

@@ -605,6 +605,7 @@ static void trace_one(const ora_scene* s, f3 o, f3 d, float tmin, float tmax, in
   uint64_t cn = 0, ct = 0;
   int64_t n = s->ntri;
   if (n == 0) goto done;
+  if (d.x == 0.0f && d.y == 0.0f && d.z == 0.0f) goto done; /* a zero direction is no ray */
   int32_t stack[ORA_STACK];
   int sp = 0;
   int32_t node = (n == 1) ? ~0 : 0;
@@ -654,6 +655,31 @@ void ora_trace(const ora_scene* s, const float* orig, const float* dir, int64_t 
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : cn, ct)
   for (int64_t i = 0; i < nrays; ++i)
     trace_one(s, ld3(orig + 3 * i), ld3(dir + 3 * i), tmin, tmax, any_hit, out + i, &cn, &ct);
+  if (counters) { counters[0] += cn; counters[1] += ct; }
+}
+
+/* shadow.rgen:29-82: visibility of each depth-buffer pixel toward the sun */
+void ora_shadow_mask(const ora_scene* s, const float* depth, int32_t w, int32_t h, const float* ivp,
+                     const float* light, float* out, uint64_t* counters)
+{
+  uint64_t cn = 0, ct = 0;
+  f3 dir = mk(-light[0], -light[1], -light[2]);
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : cn, ct)
+  for (int64_t i = 0; i < (int64_t)w * h; ++i) {
+    int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    float d = depth[i];
+    if (d == 1.0f) { out[i] = 1.0f; continue; }
+    float u = (2.0f * (float)x + 1.0f) / (2.0f * (float)w);   /* pixelToUv, helpers.glsl:1-4 */
+    float v = (2.0f * (float)y + 1.0f) / (2.0f * (float)h);
+    float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;         /* calcWorldPos, shadow.rgen:20-27 */
+    float px = ivp[0] * cx + ivp[4] * cy + ivp[8] * d + ivp[12];
+    float py = ivp[1] * cx + ivp[5] * cy + ivp[9] * d + ivp[13];
+    float pz = ivp[2] * cx + ivp[6] * cy + ivp[10] * d + ivp[14];
+    float pw = ivp[3] * cx + ivp[7] * cy + ivp[11] * d + ivp[15];
+    ora_hit hit;
+    trace_one(s, mk(px / pw, py / pw, pz / pw), dir, 0.1f, 50.0f, 1, &hit, &cn, &ct);
+    out[i] = hit.tri >= 0 ? 0.0f : 1.0f;
+  }
   if (counters) { counters[0] += cn; counters[1] += ct; }
 }
 

@@ -136,6 +136,10 @@ int32_t ora_scene_max_depth(const ora_scene* s);
 void ora_scene_world_tris(const ora_scene* s, float* out);
 /* LBVH export: keys/ids in sorted order, internal children (>= 0 internal,
  * < 0 leaf = ~sorted_position), parents, internal node boxes [n-1][6] (lo, hi) */
+/* RT hard shadows (shadow.rgen:29-82): visibility (1 lit, 0 shadowed) of each
+ * pixel of a w x h depth buffer; ivp = column-major inverse view-projection */
+void ora_shadow_mask(const ora_scene* s, const float* depth, int32_t w, int32_t h, const float* ivp,
+                     const float* light, float* out, uint64_t* counters);
 void ora_scene_bvh(const ora_scene* s, uint64_t* keys, uint32_t* sorted_ids, int32_t* left,
                    int32_t* right, int32_t* parent, float* node_boxes);
 

@@ -41,6 +41,7 @@ def lib():
         L.ora_scene_world_tris.argtypes = [vp, vp]
         L.ora_scene_bvh.argtypes = [vp] * 7
         L.ora_gi_lookup.argtypes = [vp, vp, vp, vp, i64, vp]
+        L.ora_shadow_mask.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp]
         L.ora_trace.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp, vp]
         L.ora_trace_bruteforce.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp]
         L.ora_fibonacci_dirs.argtypes = [i32, vp]
@@ -112,6 +113,17 @@ class OracleScene:
         boxes = np.zeros((m, 6), np.float32)
         lib().ora_scene_bvh(self.h, ptr(keys), ptr(ids), ptr(left), ptr(right), ptr(parent), ptr(boxes))
         return dict(keys=keys, ids=ids, left=left, right=right, parent=parent, boxes=boxes)
+
+    def shadow_mask(self, depth, inv_view_proj16, light_dir3):
+        """RT hard shadows of a depth buffer [h][w] -> (visibility [h][w], counters[2])."""
+        d = np.ascontiguousarray(depth, np.float32)
+        h, w = d.shape
+        ivp = np.ascontiguousarray(inv_view_proj16, np.float32)
+        light = np.ascontiguousarray(light_dir3, np.float32)
+        out = np.zeros((h, w), np.float32)
+        ctr = np.zeros(2, np.uint64)
+        lib().ora_shadow_mask(self.h, ptr(d), w, h, ptr(ivp), ptr(light), ptr(out), ptr(ctr))
+        return out, ctr
 
     def trace(self, orig, dirs, tmin=0.1, tmax=50.0, any_hit=False, brute=False):
         orig = np.ascontiguousarray(orig, np.float32)

@@ -176,3 +176,58 @@ def test_gi_lookup_closed_forms():
     assert np.allclose(off[0, :3], 0.3 * albedo * 0.5, rtol=1e-6)
     assert np.allclose(off[1, :3], 0.3 * albedo * 1.0, rtol=1e-6)
     assert np.all(out[:, 3] == 0) and np.all(off[:, 3] == 0)
+
+
+def _view_proj(eye, target, fovy_deg=60.0, aspect=16 / 9, near=0.1, far=500.0):
+    """Column-major Vulkan-style (depth 0..1) view-projection and its inverse, float64."""
+    eye, target = np.asarray(eye, float), np.asarray(target, float)
+    f = target - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, [0.0, 1.0, 0.0])
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = s, u, -f
+    V[:3, 3] = -V[:3, :3] @ eye
+    t = 1.0 / np.tan(np.radians(fovy_deg) / 2)
+    P = np.zeros((4, 4))
+    P[0, 0], P[1, 1] = t / aspect, t
+    P[2, 2], P[2, 3], P[3, 2] = far / (near - far), -far * near / (far - near), -1.0
+    VP = P @ V
+    return VP, np.linalg.inv(VP)
+
+
+def _shadow_origins(depth, ivp16):
+    """World positions of the depth pixels with the operations of shadow.rgen:20-27 in float32."""
+    h, w = depth.shape
+    f = np.float32
+    y, x = np.mgrid[0:h, 0:w].astype(f)
+    u = (f(2) * x + f(1)) / (f(2) * f(w))
+    v = (f(2) * y + f(1)) / (f(2) * f(h))
+    cx, cy, d = u * f(2) - f(1), v * f(2) - f(1), depth.astype(f)
+    m = np.asarray(ivp16, f)
+    p = [m[r] * cx + m[4 + r] * cy + m[8 + r] * d + m[12 + r] for r in range(4)]
+    return np.stack([p[0] / p[3], p[1] / p[3], p[2] / p[3]], -1)
+
+
+def test_shadow_mask_equals_brute_force():
+    """shadow.rgen on the oracle: any-hit visibility from the depth buffer's world
+    positions toward -light_dir equals brute force over all triangles; sky pixels
+    (depth 1) are lit without a ray."""
+    scene = scenegen.room_scene(1)
+    osc = O.OracleScene(scene)
+    _, ivp = _view_proj((0.3, 1.8, 4.2), (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)       # column-major
+    rng = np.random.default_rng(4)
+    h, w = 24, 40
+    depth = rng.uniform(0.95, 0.9995, (h, w)).astype(np.float32)
+    depth[rng.uniform(size=(h, w)) < 0.1] = 1.0
+    light = np.array([-0.3, -0.9, 0.2], np.float32)  # not normalized on purpose
+    vis, ctr = osc.shadow_mask(depth, ivp16, light)
+    org = _shadow_origins(depth, ivp16).reshape(-1, 3)
+    dirs = np.tile(-light, (len(org), 1))
+    hb = osc.trace(org, dirs, tmin=0.1, tmax=50.0, any_hit=True, brute=True)
+    want = np.where(hb["tri"] >= 0, 0.0, 1.0).astype(np.float32).reshape(h, w)
+    want[depth == 1.0] = 1.0
+    assert np.array_equal(vis, want)
+    assert 0 < (vis == 0).sum() < vis.size and ctr[0] > 0
