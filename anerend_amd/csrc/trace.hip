@@ -461,6 +461,15 @@ void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipSt
 // ---------------------------------------------------------------------------
 static constexpr float kPi = 3.14159265359f;  // pbr_light.glsl:8
 
+// Hardware reciprocal, reciprocal square root and square root (1 ulp) for the
+// shading's continuous terms.  Radiance carries a relative tolerance; no branch
+// of the shading depends on these values, and the hit positions (shadow-ray
+// origins) keep correctly rounded arithmetic.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ f3 hw_normalize(f3 v) { return muls(v, hw_rsqrt(dot(v, v))); }
+
 __device__ __forceinline__ float distribution_ggx(f3 N, f3 H, float rough)
 {
   float a = rough * rough;
@@ -470,19 +479,21 @@ __device__ __forceinline__ float distribution_ggx(f3 N, f3 H, float rough)
   float num = a2 + 0.001f;
   float den = ndoth2 * (a2 - 1.0f) + 1.0f;
   den = kPi * den * den;
-  return num / den;
+  return num * hw_rcp(den);
 }
 
 __device__ __forceinline__ float schlick_ggx(float ndotv, float rough)
 {
   float r = rough + 1.0f;
-  float k = (r * r) / 8.0f;
-  return ndotv / (ndotv * (1.0f - k) + k);
+  float k = (r * r) * 0.125f;  // / 8 exactly
+  return ndotv * hw_rcp(ndotv * (1.0f - k) + k);
 }
 
 __device__ __forceinline__ float pow5_term(float c)
 {
-  return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f);
+  const float x = clampf(1.0f - c, 0.0f, 1.0f);
+  const float x2 = x * x;
+  return x2 * x2 * x;
 }
 
 // calcLight, pbr_light.glsl:98-166 (directional, specular on); V = normalize(view - pos)
@@ -491,7 +502,7 @@ __device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float
   f3 light_color = muls(mk(1.0f, 1.0f, 1.0f), f.sun_intensity);
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
   f3 L = mk(f.sun_l[0], f.sun_l[1], f.sun_l[2]);  // normalize(-lightDir), once per frame
-  f3 H = normalize(add(V, L));
+  f3 H = hw_normalize(add(V, L));
   f3 radiance = muls(light_color, 1.0f);
   float ndf = distribution_ggx(N, H, rough);
   float ndotv_ = fmaxf(dot(N, V), 0.0f);
@@ -504,9 +515,9 @@ __device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float
   float ndotv = fmaxf(dot(N, V), 0.0f);
   f3 numer = muls(F, ndf * G);
   float denom = 4.0f * ndotv * ndotl + 0.0001f;
-  f3 spec = divs(numer, denom);
+  f3 spec = muls(numer, hw_rcp(denom));
   spec = mk(clampf(spec.x, 0.0f, 1.0f), clampf(spec.y, 0.0f, 1.0f), clampf(spec.z, 0.0f, 1.0f));
-  f3 lo = add(divs(mul(kD, albedo), kPi), spec);
+  f3 lo = add(muls(mul(kD, albedo), 1.0f / kPi), spec);
   return muls(mul(lo, radiance), ndotl);
 }
 
@@ -600,18 +611,18 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
     f3 ppos = add(base, mul(mk((float)ox, (float)oy, (float)oz), step));
     f3 tri = mk(ox ? alpha.x : 1.0f - alpha.x, oy ? alpha.y : 1.0f - alpha.y,
                 oz ? alpha.z : 1.0f - alpha.z);
-    f3 dir = normalize(sub(ppos, pos));
-    float x = fmaxf(0.0001f, (dot(dir, n) + 1.0f) * 0.5f);
+    const f3 dv = sub(ppos, pos);
+    float x = fmaxf(0.0001f, (dot(dv, n) * hw_rsqrt(dot(dv, dv)) + 1.0f) * 0.5f);
     float w = x * x + 0.2f;
     w = fmaxf(0.000001f, w);
     if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
     w = w * (tri.x * tri.y * tri.z);
     f3 s = bilinear_repeat(atlas, W, H, tu[ox], tv[2 * oy + oz]);
-    s = mk(sqrtf(s.x), sqrtf(s.y), sqrtf(s.z));
+    s = mk(hw_sqrt(s.x), hw_sqrt(s.y), hw_sqrt(s.z));
     sumi = add(sumi, muls(s, w));
     sumw = sumw + w;
   }
-  f3 net = divs(sumi, sumw);
+  f3 net = muls(sumi, hw_rcp(sumw));
   net = mul(net, net);
   return muls(muls(net, 0.5f), 3.1415926f);
 }
@@ -670,7 +681,7 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   const float rough = mat.met_rough[1];
   f3 emissive = muls(mk(mat.emissive[0], mat.emissive[1], mat.emissive[2]), mat.emissive[3]);
   f3 acc = mk(0.0f, 0.0f, 0.0f);
-  const f3 V = normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
+  const f3 V = hw_normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
   if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, V);
   if (f.flags & ANE_FLAG_DDGI_MULTI)
     acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, V));
