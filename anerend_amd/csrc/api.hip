@@ -590,6 +590,16 @@ struct ane_ddgi_ctx {
   bool counting = false;
   ane_ddgi_stats stats{};
   int64_t bytes = 0;
+  // mirror-reflection workspace, sized on first use: [0] reflection rays,
+  // [1] their shadow rays
+  float* d_mr_orig[2] = {nullptr, nullptr};
+  float* d_mr_dir[2] = {nullptr, nullptr};
+  ane_hit* d_mr_hits[2] = {nullptr, nullptr};
+  uint32_t* d_mr_work = nullptr;             // [2] ray counters
+  unsigned long long* d_mr_stats = nullptr;  // [4]
+  uint32_t* d_mr_ov = nullptr;
+  int32_t* d_mr_spill = nullptr;
+  int64_t mr_px = 0, mr_spill_words = 0;
 };
 
 static void ctx_free(ane_ddgi_ctx* c)
@@ -599,6 +609,8 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
+  for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
+  dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
   for (auto& t : c->timers) t.destroy();
 }
 
@@ -916,6 +928,96 @@ extern "C" int ane_ddgi_gi_lookup(ane_ddgi_ctx* c, const ane_ddgi_frame* f, cons
   ANE_TRY(frame_to_k(c, &full, &fk));
   launch_gi_lookup(c->g, fk, c->d_atlas[c->cur], gbuf, n, out_rgba, S(stream));
   ANE_CHECK_HIP(hipGetLastError());
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_mirror_reflections(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f,
+                                           const float* depth, const float* normal4, int32_t width,
+                                           int32_t height, const float* ivp16, float* out_rgba,
+                                           uint64_t* counters4, void* stream)
+{
+  if (!c || !s || !f || !ivp16) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (width < 0 || height < 0) return fail(ANE_ERR_INVALID_ARG, "negative image size");
+  const int64_t n = (int64_t)width * height;
+  if (n == 0) return ANE_OK;
+  if (!depth || !normal4 || !out_rgba) return fail(ANE_ERR_INVALID_ARG, "NULL depth, normal or output");
+  if (n >= ((int64_t)1 << 31)) return fail(ANE_ERR_INVALID_ARG, "image too large");
+  for (int k = 0; k < 16; ++k)
+    if (!std::isfinite(ivp16[k])) return fail(ANE_ERR_INVALID_ARG, "non-finite inverse view-projection");
+  if (c->device != s->device) return fail(ANE_ERR_INVALID_ARG, "context and scene on different devices");
+  FrameK fk;
+  ane_ddgi_frame full = *f;
+  full.layer_begin = 0;  // the hit shading reads the whole atlas; the layer range is not used
+  full.layer_count = c->g.ny;
+  ANE_TRY(frame_to_k(c, &full, &fk));
+  hipStream_t st = S(stream);
+  if (n > c->mr_px) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
+    c->mr_px = 0;
+    for (int k = 0; k < 2; ++k) {
+      ANE_TRY(dalloc(&c->d_mr_orig[k], 3 * n, nullptr));
+      ANE_TRY(dalloc(&c->d_mr_dir[k], 3 * n, nullptr));
+      ANE_TRY(dalloc(&c->d_mr_hits[k], n, nullptr));
+    }
+    c->mr_px = n;
+  }
+  if (!c->d_mr_work) {
+    ANE_TRY(dalloc(&c->d_mr_work, 2, nullptr));
+    ANE_TRY(dalloc(&c->d_mr_stats, 4, nullptr));
+    ANE_TRY(dalloc(&c->d_mr_ov, 1, nullptr));
+  }
+  const int64_t lanes = trace_lanes_per_launch();
+  const int64_t sd = s->depth - trace_stack_lds() + 1 > 0 ? s->depth - trace_stack_lds() + 1 : 0;
+  if (sd * lanes > c->mr_spill_words) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    dfree(c->d_mr_spill);
+    c->mr_spill_words = 0;
+    ANE_TRY(dalloc(&c->d_mr_spill, sd * lanes, nullptr));
+    c->mr_spill_words = sd * lanes;
+  }
+  fill_u32(c->d_mr_work, 0u, 2, st);
+  fill_u32(reinterpret_cast<uint32_t*>(c->d_mr_stats), 0u, 8, st);
+  fill_u32(c->d_mr_ov, 0u, 1, st);
+  // mirror_reflections.rgen:57-79: reflection rays over [0.1, 50], closest hit
+  launch_mirror_rays(depth, normal4, width, height, ivp16, f->camera_pos, c->d_mr_orig[0], c->d_mr_dir[0], st);
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g.rays = 1;
+  a.overflow = c->d_mr_ov;
+  a.spill = c->d_mr_spill;
+  a.spill_depth = (int32_t)sd;
+  a.nrays = (int32_t)n;
+  a.t_min = 0.1f;
+  a.t_max = 50.0f;
+  a.hits = c->d_mr_hits[0];
+  a.work = c->d_mr_work;
+  a.stats = c->d_mr_stats;
+  a.ray_orig = c->d_mr_orig[0];
+  a.ray_dir = c->d_mr_dir[0];
+  launch_trace(a, false, true, counters4 != nullptr, st);
+  // mirror_reflections.rchit:44-66: shadow rays from the hits, any hit over [0.1, 50]
+  const f3 sun = normalize(mk(-fk.light_dir[0], -fk.light_dir[1], -fk.light_dir[2]));
+  const float sun3[3] = {sun.x, sun.y, sun.z};
+  launch_mirror_shadow_rays(s->view, c->d_mr_hits[0], n, sun3, c->d_mr_orig[1], c->d_mr_dir[1], st);
+  a.hits = c->d_mr_hits[1];
+  a.work = c->d_mr_work + 1;
+  a.stats = c->d_mr_stats + 2;
+  a.ray_orig = c->d_mr_orig[1];
+  a.ray_dir = c->d_mr_dir[1];
+  launch_trace(a, true, true, counters4 != nullptr, st);
+  launch_mirror_shade(s->view, c->g, fk, c->d_atlas[c->cur], depth, c->d_mr_orig[0], c->d_mr_hits[0],
+                      c->d_mr_hits[1], n, out_rgba, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  if (counters4) {
+    unsigned long long c4[4] = {0, 0, 0, 0};
+    uint32_t ov = 0;
+    ANE_CHECK_HIP(hipMemcpyAsync(c4, c->d_mr_stats, 32, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipMemcpyAsync(&ov, c->d_mr_ov, 4, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 4; ++k) counters4[k] = c4[k];
+    if (ov) return fail(ANE_ERR_DEVICE, "traversal stack overflow");
+  }
   return ANE_OK;
 }
 

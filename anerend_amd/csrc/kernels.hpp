@@ -96,6 +96,14 @@ void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const
 void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
                         float* orig, float* dirs, hipStream_t s);
 void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s);
+// mirror reflections: reflection rays, the shadow rays of their hits, shading
+void launch_mirror_rays(const float* depth, const float* normal4, int W, int H, const float* ivp16,
+                        const float* cam3, float* orig, float* dirs, hipStream_t s);
+void launch_mirror_shadow_rays(const SceneK& sc, const ane_hit* hits, int64_t n, const float* sun3, float* orig,
+                               float* dirs, hipStream_t s);
+void launch_mirror_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float* atlas,
+                         const float* depth, const float* orig, const ane_hit* hits, const ane_hit* shadow_hits,
+                         int64_t n, float* out, hipStream_t s);
 
 // translate.hip
 void launch_translate(const GridK& g, const float4* src, float4* dst, int dx, int dz,

@@ -312,7 +312,10 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
           if (RAYS) {
             out = (int32_t)gidx;
             const f3 d = ld3(a.ray_dir + 3 * (int64_t)gidx);
-            none = d.x == 0.0f && d.y == 0.0f && d.z == 0.0f;  // a zero direction is no ray
+            // a zero or non-finite direction is no ray (a NaN direction would pass
+            // every slab test and visit the whole hierarchy)
+            none = (d.x == 0.0f && d.y == 0.0f && d.z == 0.0f) || !__builtin_isfinite(d.x) ||
+                   !__builtin_isfinite(d.y) || !__builtin_isfinite(d.z);
             ray_setup(r, ld3(a.ray_orig + 3 * (int64_t)gidx), d);
           } else {
             // independent loads: processing-order ray setup, probe origin, output slot
@@ -643,15 +646,19 @@ __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas
   return mul(kD, diffuse);
 }
 
-// rchit:24-110 (+ rmiss:17-21 for misses)
+// rchit:24-110 (+ rmiss:17-21 for misses).  MIRROR: mirror_reflections.rchit,
+// which transforms the normal by mat3(ObjectToWorld) (:74) and gates the
+// indirect term on ANE_FLAG_DDGI (:107).  The world hit position goes to *hit_pos.
+template <bool MIRROR>
 __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const float4* atlas,
-                        f3 origin, ane_hit h, uint8_t shadowed)
+                        f3 origin, ane_hit h, uint8_t shadowed, f3* hit_pos = nullptr)
 {
   if (h.tri < 0) return mk(f.sky[0], f.sky[1], f.sky[2]);
   // instance, vertices and position (rchit:27-42): the same arithmetic that gave
   // the shadow-ray origin in the traversal
   HitGeom hg;
   const f3 pos = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
+  if (hit_pos) *hit_pos = pos;
   const ane_instance& in = sc.inst[hg.inst];
   const ane_material& mat = sc.mats[mat_of(in.custom_index_and_mask)];
   const bool lit = shadowed == 0;
@@ -668,10 +675,19 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
     col = add(col, muls(cj, bw[j]));
   }
   nrm = normalize(nrm);
-  const float* wi = sc.inv_xform + 12 * (int64_t)hg.inst;
-  f3 nw = mk(nrm.x * wi[0] + nrm.y * wi[4] + nrm.z * wi[8],
-             nrm.x * wi[1] + nrm.y * wi[5] + nrm.z * wi[9],
-             nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
+  f3 nw;
+  if (MIRROR) {
+    // mat3(ObjectToWorld) * n over the row-major 3x4 instance transform
+    const float* t = in.transform;
+    nw = mk(t[0] * nrm.x + t[1] * nrm.y + t[2] * nrm.z, t[4] * nrm.x + t[5] * nrm.y + t[6] * nrm.z,
+            t[8] * nrm.x + t[9] * nrm.y + t[10] * nrm.z);
+  } else {
+    // n * WorldToObject: row vector times the inverse transform
+    const float* wi = sc.inv_xform + 12 * (int64_t)hg.inst;
+    nw = mk(nrm.x * wi[0] + nrm.y * wi[4] + nrm.z * wi[8],
+            nrm.x * wi[1] + nrm.y * wi[5] + nrm.z * wi[9],
+            nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
+  }
   nrm = normalize(nw);
   col = mk(to_linear1(col.x), to_linear1(col.y), to_linear1(col.z));
   f3 albedo = col;
@@ -683,7 +699,7 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   f3 acc = mk(0.0f, 0.0f, 0.0f);
   const f3 V = hw_normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
   if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, V);
-  if (f.flags & ANE_FLAG_DDGI_MULTI)
+  if (f.flags & (MIRROR ? ANE_FLAG_DDGI : ANE_FLAG_DDGI_MULTI))
     acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, V));
   return add(acc, emissive);
 }
@@ -730,7 +746,7 @@ __global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, cons
     const int64_t idx = (int64_t)ql * n + i;
     const ane_hit h = hits[idx];
     const uint8_t sh = (h.tri >= 0) ? shadow[idx] : (uint8_t)255;
-    const f3 L = shade_ray(sc, g, f, atlas, origin, h, sh);
+    const f3 L = shade_ray<false>(sc, g, f, atlas, origin, h, sh);
     if (PREC == ANE_PRECISION_FP32) {
       float* o = (float*)rad + (int64_t)ql * 3 * n;
       o[i] = L.x;
@@ -842,14 +858,12 @@ struct Mat4 {
   float m[16];  // column-major
 };
 
-__global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, int H, Mat4 ivp, f3 dir,
-                                                     float* orig, float* dirs)
+// world position of pixel i of a W-wide depth buffer: pixelToUv
+// (helpers.glsl:1-4), then calcWorldPos (shadow.rgen:20-27,
+// mirror_reflections.rgen:25-33)
+__device__ __forceinline__ f3 pixel_world_pos(int64_t i, int W, int H, float d, const Mat4& ivp)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)W * H) return;
   const int y = (int)(i / W), x = (int)(i - (int64_t)y * W);
-  const float d = depth[i];
-  // pixelToUv (helpers.glsl:1-4), then calcWorldPos (shadow.rgen:20-27)
   const float u = (2.0f * (float)x + 1.0f) / (2.0f * (float)W);
   const float v = (2.0f * (float)y + 1.0f) / (2.0f * (float)H);
   const float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
@@ -858,9 +872,19 @@ __global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, 
   const float py = m[1] * cx + m[5] * cy + m[9] * d + m[13];
   const float pz = m[2] * cx + m[6] * cy + m[10] * d + m[14];
   const float pw = m[3] * cx + m[7] * cy + m[11] * d + m[15];
-  orig[3 * i + 0] = px / pw;
-  orig[3 * i + 1] = py / pw;
-  orig[3 * i + 2] = pz / pw;
+  return mk(px / pw, py / pw, pz / pw);
+}
+
+__global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, int H, Mat4 ivp, f3 dir,
+                                                     float* orig, float* dirs)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)W * H) return;
+  const float d = depth[i];
+  const f3 p = pixel_world_pos(i, W, H, d, ivp);
+  orig[3 * i + 0] = p.x;
+  orig[3 * i + 1] = p.y;
+  orig[3 * i + 2] = p.z;
   // sky pixels get a zero direction, which the traversal takes as no ray
   const bool sky = d == 1.0f;
   dirs[3 * i + 0] = sky ? 0.0f : dir.x;
@@ -892,6 +916,110 @@ void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, f
 {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_shadow_resolve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, hits, n, vis);
+}
+
+// ---------------------------------------------------------------------------
+// mirror reflections (mirror_reflections.rgen/.rchit/.rmiss): a closest-hit
+// ray per depth-buffer pixel along the view vector reflected about the
+// G-buffer normal, a shadow ray from each hit toward the sun, then the hit
+// shading of the probe rays with the mirror variant's normal and DDGI gate
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mirror_rays(const float* depth, const float4* normal4, int W, int H,
+                                                     Mat4 ivp, f3 cam, float* orig, float* dirs)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)W * H) return;
+  const float d = depth[i];
+  const f3 p = pixel_world_pos(i, W, H, d, ivp);
+  orig[3 * i + 0] = p.x;
+  orig[3 * i + 1] = p.y;
+  orig[3 * i + 2] = p.z;
+  f3 r = mk(0.0f, 0.0f, 0.0f);  // sky pixels: zero direction, no ray
+  if (d != 1.0f) {
+    const float4 nq = normal4[i];
+    const f3 n = normalize(mk(nq.x, nq.y, nq.z));
+    const f3 v = normalize(sub(p, cam));
+    r = sub(v, muls(n, 2.0f * dot(n, v)));  // reflect(v, n)
+  }
+  dirs[3 * i + 0] = r.x;
+  dirs[3 * i + 1] = r.y;
+  dirs[3 * i + 2] = r.z;
+}
+
+// the shadow ray of every reflection hit (mirror_reflections.rchit:44-66)
+__global__ __launch_bounds__(256) void k_mirror_shadow_rays(SceneK sc, const ane_hit* hits, int64_t n, f3 sun,
+                                                            float* orig, float* dirs)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ane_hit h = hits[i];
+  f3 p = mk(0.0f, 0.0f, 0.0f), d = mk(0.0f, 0.0f, 0.0f);
+  if (h.tri >= 0) {
+    HitGeom hg;
+    p = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
+    d = sun;
+  }
+  orig[3 * i + 0] = p.x;
+  orig[3 * i + 1] = p.y;
+  orig[3 * i + 2] = p.z;
+  dirs[3 * i + 0] = d.x;
+  dirs[3 * i + 1] = d.y;
+  dirs[3 * i + 2] = d.z;
+}
+
+// out = (irradiance, distance to the hit); a miss: (sky, 1e6); a sky pixel: 0
+__global__ __launch_bounds__(256) void k_mirror_shade(SceneK sc, GridK g, FrameK f, const float4* atlas,
+                                                      const float* depth, const float* orig,
+                                                      const ane_hit* hits, const ane_hit* shadow_hits,
+                                                      int64_t n, float4* out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (depth[i] == 1.0f) {
+    out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const ane_hit h = hits[i];
+  if (h.tri < 0) {
+    out[i] = make_float4(f.sky[0], f.sky[1], f.sky[2], 1e6f);
+    return;
+  }
+  const f3 o = mk(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]);
+  const uint8_t shadowed = shadow_hits[i].tri >= 0 ? 1 : 0;
+  f3 pos;
+  const f3 L = shade_ray<true>(sc, g, f, atlas, o, h, shadowed, &pos);
+  const f3 dd = sub(pos, o);
+  out[i] = make_float4(L.x, L.y, L.z, sqrtf(dot(dd, dd)));
+}
+
+void launch_mirror_rays(const float* depth, const float* normal4, int W, int H, const float* ivp16,
+                        const float* cam3, float* orig, float* dirs, hipStream_t s)
+{
+  Mat4 m;
+  for (int k = 0; k < 16; ++k) m.m[k] = ivp16[k];
+  const int64_t n = (int64_t)W * H;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth,
+                     reinterpret_cast<const float4*>(normal4), W, H, m, mk(cam3[0], cam3[1], cam3[2]), orig,
+                     dirs);
+}
+
+void launch_mirror_shadow_rays(const SceneK& sc, const ane_hit* hits, int64_t n, const float* sun3, float* orig,
+                               float* dirs, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror_shadow_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sc, hits, n,
+                     mk(sun3[0], sun3[1], sun3[2]), orig, dirs);
+}
+
+void launch_mirror_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float* atlas,
+                         const float* depth, const float* orig, const ane_hit* hits, const ane_hit* shadow_hits,
+                         int64_t n, float* out, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror_shade, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sc, g, f,
+                     reinterpret_cast<const float4*>(atlas), depth, orig, hits, shadow_hits, n,
+                     reinterpret_cast<float4*>(out));
 }
 
 }  // namespace ane

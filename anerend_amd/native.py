@@ -73,6 +73,7 @@ _SIGS = [
     ("ane_ddgi_translation_state", ctypes.c_int, [_vp, _vp, _i32]),
     ("ane_ddgi_atlas_export_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_gi_lookup", ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    ("ane_ddgi_mirror_reflections", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp]),
     ("ane_ddgi_atlas_import_rgba16f", ctypes.c_int, [_vp, _vp, _vp]),
     ("ane_ddgi_radiance_set", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("ane_ddgi_radiance_get", ctypes.c_int, [_vp, _vp, _vp]),
@@ -315,6 +316,28 @@ class ProbeGrid:
         finally:
             src.close()
             dst.close()
+
+    def mirror_reflections(self, scene, frame, depth, normal4, inv_view_proj16, count=False):
+        """Mirror reflections of a host depth [h][w] + normal [h][w][4] buffer ->
+        [h][w][4] (irradiance, hit distance) (and counters[4])."""
+        d = np.ascontiguousarray(depth, np.float32)
+        h, w = d.shape
+        nrm = np.ascontiguousarray(normal4, np.float32).reshape(h, w, 4)
+        ivp = np.ascontiguousarray(inv_view_proj16, np.float32)
+        bufs = [DeviceBuffer(d.nbytes), DeviceBuffer(nrm.nbytes), DeviceBuffer(nrm.nbytes)]
+        ctr = np.zeros(4, np.uint64)
+        try:
+            bufs[0].upload(d)
+            bufs[1].upload(nrm)
+            check(load().ane_ddgi_mirror_reflections(self._h, scene.handle, ctypes.byref(frame), bufs[0].ptr,
+                                                     bufs[1].ptr, w, h, ptr(ivp), bufs[2].ptr,
+                                                     ptr(ctr) if count else None, None))
+            synchronize()
+            out = bufs[2].download(np.zeros((h, w, 4), np.float32))
+        finally:
+            for b in bufs:
+                b.close()
+        return (out, ctr) if count else out
 
     def translation_state(self, cell=None):
         c = np.zeros(2, np.int32) if cell is None else np.ascontiguousarray(cell, np.int32)

@@ -240,12 +240,10 @@ def view_proj(eye, target, fovy_deg=60.0, aspect=16 / 9, near=0.1, far=500.0):
     return VP, np.linalg.inv(VP)
 
 
-def shadow_mask_study(ds, frame, width=1920, height=1080, reps=10):
-    """RT hard shadows (shadow.rgen) of a 1920x1080 view of the scene: the depth
-    buffer comes from camera rays cast through the library, then the shadow mask
-    is timed with HIP events over ``reps`` launches."""
-    import ctypes
-    from anerend_amd import native
+def camera_gbuffer(ds, width, height):
+    """Depth buffer and normals of a width x height view of the scene from
+    (0.5, 20, 0.5) toward (60, 0, 60), from camera rays cast through the library.
+    Normals are the depth buffer's world-position gradients, facing the camera."""
     eye = np.array([0.5, 20.0, 0.5])
     VP, IVP = view_proj(eye, (60.0, 0.0, 60.0))
     y, x = np.mgrid[0:height, 0:width]
@@ -261,7 +259,25 @@ def shadow_mask_study(ds, frame, width=1920, height=1080, reps=10):
     pos = eye + d.astype(float) * hits["t"][:, None].astype(float)
     ndc = np.concatenate([pos, np.ones((len(pos), 1))], 1) @ VP.T
     depth = np.where(hit, ndc[:, 2] / ndc[:, 3], 1.0).astype(np.float32).reshape(height, width)
-    ivp16 = IVP.T.ravel().astype(np.float32)
+    p = pos.reshape(height, width, 3)
+    gx = np.gradient(p, axis=1)
+    gy = np.gradient(p, axis=0)
+    nrm = np.cross(gy, gx)
+    nrm *= np.where(np.sum(nrm * (eye - p), -1) < 0, -1.0, 1.0)[..., None]
+    ln = np.linalg.norm(nrm, axis=-1, keepdims=True)
+    nrm = np.where(ln > 1e-12, nrm / np.maximum(ln, 1e-12), np.array([0.0, 1.0, 0.0]))
+    normal4 = np.zeros((height, width, 4), np.float32)
+    normal4[..., :3] = nrm
+    return depth, normal4, IVP.T.ravel().astype(np.float32)
+
+
+def shadow_mask_study(ds, frame, width=1920, height=1080, reps=10):
+    """RT hard shadows (shadow.rgen) of a 1920x1080 view of the scene: the depth
+    buffer comes from camera rays cast through the library, then the shadow mask
+    is timed with HIP events over ``reps`` launches."""
+    import ctypes
+    from anerend_amd import native
+    depth, _, ivp16 = camera_gbuffer(ds, width, height)
     light = np.array(frame.light_dir[:3], np.float32)
     _, ctr = ds.shadow_mask(depth, ivp16, light, count=True)
     n = width * height
@@ -295,6 +311,53 @@ def shadow_mask_study(ds, frame, width=1920, height=1080, reps=10):
     return {"pixels": n, "shadow_rays": rays, "ms": round(t, 5), "rays_per_s": round(rays / (t * 1e-3), 1),
             "shadowed_fraction": round(float((vis == 0).mean()), 4), "node_visits": int(ctr[0]),
             "triangle_tests": int(ctr[1]), "algorithmic_gbs": round(algo / (t * 1e-3) / 1e9, 1)}
+
+
+def mirror_study(grid, ds, frame, width=1920, height=1080, reps=10):
+    """Mirror reflections (mirror_reflections.rgen/.rchit) of a 1920x1080 view of
+    the scene with the probe atlas as indirect light, timed with HIP events over
+    ``reps`` calls (reflection rays, their shadow rays, the hit shading)."""
+    import ctypes
+    from anerend_amd import native
+    depth, normal4, ivp16 = camera_gbuffer(ds, width, height)
+    out, ctr = grid.mirror_reflections(ds, frame, depth, normal4, ivp16, count=True)
+    n = width * height
+    bufs = [native.DeviceBuffer(4 * n), native.DeviceBuffer(16 * n), native.DeviceBuffer(16 * n)]
+    hip = native.DeviceBuffer._hip
+    L = native.load()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    try:
+        bufs[0].upload(depth)
+        bufs[1].upload(normal4)
+        ip = ivp16.ctypes.data_as(ctypes.c_void_p)
+        args = (grid.handle, ds.handle, ctypes.byref(frame), bufs[0].ptr, bufs[1].ptr, width, height, ip,
+                bufs[2].ptr, None, None)
+        native.check(L.ane_ddgi_mirror_reflections(*args))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            native.check(L.ane_ddgi_mirror_reflections(*args))
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        t = ms.value / reps
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        for b in bufs:
+            b.close()
+    rays = int((depth != 1.0).sum())
+    hit = (depth != 1.0) & (out[..., 3] < 1e6)
+    # traversal (nodes, triangles) + per pixel: depth 4, normal 16, two ray sets
+    # (24 B setup + 16 B hit each), output 16
+    algo = (int(ctr[0]) + int(ctr[2])) * NODE_BYTES + (int(ctr[1]) + int(ctr[3])) * TRI_BYTES + n * (
+        4 + 16 + 2 * (24 + 16) + 16)
+    return {"pixels": n, "reflection_rays": rays, "reflection_hits": int(hit.sum()), "ms": round(t, 5),
+            "rays_per_s": round(rays / (t * 1e-3), 1), "node_visits": int(ctr[0]), "triangle_tests": int(ctr[1]),
+            "shadow_node_visits": int(ctr[2]), "shadow_triangle_tests": int(ctr[3]),
+            "algorithmic_gbs": round(algo / (t * 1e-3) / 1e9, 1)}
 
 
 def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
@@ -590,6 +653,10 @@ def main():
     shadows = shadow_mask_study(ds, frame_at(0))
     log(f"[bench] shadow mask {json.dumps(shadows)}")
 
+    log("[bench] mirror reflections, 1920x1080")
+    mirror = mirror_study(grid, ds, frame_at(0))
+    log(f"[bench] mirror reflections {json.dumps(mirror)}")
+
     conv = None
     if not args.no_conv_study:
         log("[bench] convolution study (FP32 / BF16 / FP8 vs FP64)")
@@ -626,6 +693,7 @@ def main():
         "conv_study": conv,
         "gi_lookup": gi,
         "shadow_mask": shadows,
+        "mirror_reflections": mirror,
     }
     print(json.dumps(out), flush=True)
 

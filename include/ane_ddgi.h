@@ -223,6 +223,24 @@ int ane_ddgi_update(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_fr
 int ane_ddgi_gi_lookup(ane_ddgi_ctx* ctx, const ane_ddgi_frame* frame, const ane_gbuffer_texel* gbuf,
                        int64_t n, float* out_rgba, void* stream);
 
+/* Mirror reflections (replaces the MirrorReflections pass's vkCmdTraceRaysKHR,
+ * mirror_reflections.rgen:35-82 + .rchit:24-119 + .rmiss:17-22).  Per pixel of a
+ * width x height depth buffer: the world position (inv_view_proj16, column-major,
+ * as ane_scene_shadow_mask), the normal normalize(normal4[pixel].xyz), a
+ * closest-hit ray over [0.1, 50] along reflect(normalize(pos - camera_pos),
+ * normal); at a hit, a shadow ray toward -light_dir and the probe hit shading with
+ * the normal taken through mat3(ObjectToWorld) and the indirect term from the
+ * context's atlas when frame->flags has ANE_FLAG_DDGI.  out_rgba = (irradiance,
+ * distance to the hit); a miss gives (sky radiance, 1e6), a sky pixel (depth 1)
+ * gives 0.  A zero normal gives a non-finite direction, which is no ray (a miss).  depth, normal4 (4 floats per pixel) and out_rgba are DEVICE pointers,
+ * ordered on `stream`.  counters4 (host, nullable): node visits and triangle
+ * tests of the reflection rays, then of the shadow rays (the stream is then
+ * synchronized). */
+int ane_ddgi_mirror_reflections(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_frame* frame,
+                                const float* depth, const float* normal4, int32_t width, int32_t height,
+                                const float* inv_view_proj16, float* out_rgba, uint64_t* counters4,
+                                void* stream);
+
 /* ---- probe state (ProbeRaysConvTex) ---- */
 int ane_ddgi_atlas_get(ane_ddgi_ctx* ctx, float* host_rgba, void* stream);
 int ane_ddgi_atlas_set(ane_ddgi_ctx* ctx, const float* host_rgba, void* stream);
@@ -257,7 +275,7 @@ int ane_scene_debug_bvh(const ane_scene* scene, uint64_t* keys, uint32_t* ids, i
                         int32_t* right, int32_t* parent, float* boxes);
 int ane_scene_debug_world_tris(const ane_scene* scene, float* tris9);
 /* Trace arbitrary rays (host arrays) against a scene: closest or any hit.  A
- * zero direction is no ray (a miss, no traversal). */
+ * zero or non-finite direction is no ray (a miss, no traversal). */
 int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const float* dirs,
                          int64_t num_rays, float t_min, float t_max, int32_t any_hit,
                          ane_hit* hits, uint64_t* counters2, void* stream);

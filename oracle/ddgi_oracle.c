@@ -605,7 +605,9 @@ static void trace_one(const ora_scene* s, f3 o, f3 d, float tmin, float tmax, in
   uint64_t cn = 0, ct = 0;
   int64_t n = s->ntri;
   if (n == 0) goto done;
-  if (d.x == 0.0f && d.y == 0.0f && d.z == 0.0f) goto done; /* a zero direction is no ray */
+  /* a zero or non-finite direction is no ray */
+  if ((d.x == 0.0f && d.y == 0.0f && d.z == 0.0f) || !isfinite(d.x) || !isfinite(d.y) || !isfinite(d.z))
+    goto done;
   int32_t stack[ORA_STACK];
   int sp = 0;
   int32_t node = (n == 1) ? ~0 : 0;
@@ -935,9 +937,15 @@ void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, 
   }
 }
 
-void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, const float* atlas,
-                   const float* origin, const ora_hit* hit, float* out, int32_t* shadowed,
-                   uint64_t* counters)
+/* Radiance of a closest hit seen from `origin`: the probe closest-hit shader
+ * (rchit:24-110) or, with mirror set, mirror_reflections.rchit, which differs in
+ * the normal transform (mat3(ObjectToWorld) * n, :72) and the indirect-light
+ * gate (UBO_DDGI_FLAG, :96).  The shadow ray covers [tmin, tmax]; the world hit
+ * position is returned in pos_out (nullable). */
+static void shade_surface(const ora_scene* s, const ora_config* c, const ora_frame* f,
+                          const float* atlas, const float* origin, const ora_hit* hit, int mirror,
+                          float tmin, float tmax, float* out, int32_t* shadowed, uint64_t* counters,
+                          f3* pos_out)
 {
   if (hit->tri < 0) {
     ora_sky_radiance(f, out);
@@ -962,8 +970,9 @@ void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, 
   f3 sdir = normalize(mk(-f->light_dir[0], -f->light_dir[1], -f->light_dir[2]));
   ora_hit sh;
   uint64_t cn = 0, ct = 0;
-  trace_one(s, pos, sdir, c->t_min, c->t_max, 1, &sh, &cn, &ct);
+  trace_one(s, pos, sdir, tmin, tmax, 1, &sh, &cn, &ct);
   if (counters) { counters[0] += cn; counters[1] += ct; }
+  if (pos_out) *pos_out = pos;
   int lit = sh.tri < 0; /* rmiss:19 sets shadow = 0 */
   /* rchit:70-74; unpackVertex bindless.glsl:98-109: color = (d0.w, d1.xy), normal = (d1.zw, d2.x) */
   f3 nrm = mk(0.0f, 0.0f, 0.0f), col = mk(0.0f, 0.0f, 0.0f);
@@ -975,11 +984,19 @@ void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, 
     col = add(col, muls(cj, bw[j]));
   }
   nrm = normalize(nrm);
-  /* normal * gl_WorldToObjectEXT: r[j] = sum_i n[i] * W2O[i][j] (row i, column j) */
-  const float* wi = s->inv_xform + 12 * (int64_t)k;
-  f3 nw = mk(nrm.x * wi[0] + nrm.y * wi[4] + nrm.z * wi[8],
-             nrm.x * wi[1] + nrm.y * wi[5] + nrm.z * wi[9],
-             nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
+  f3 nw;
+  if (mirror) {
+    /* mat3(gl_ObjectToWorldEXT) * normal: r[j] = sum_i O2W[j][i] * n[i] */
+    const float* t = in->transform;
+    nw = mk(t[0] * nrm.x + t[1] * nrm.y + t[2] * nrm.z, t[4] * nrm.x + t[5] * nrm.y + t[6] * nrm.z,
+            t[8] * nrm.x + t[9] * nrm.y + t[10] * nrm.z);
+  } else {
+    /* normal * gl_WorldToObjectEXT: r[j] = sum_i n[i] * W2O[i][j] (row i, column j) */
+    const float* wi = s->inv_xform + 12 * (int64_t)k;
+    nw = mk(nrm.x * wi[0] + nrm.y * wi[4] + nrm.z * wi[8],
+            nrm.x * wi[1] + nrm.y * wi[5] + nrm.z * wi[9],
+            nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
+  }
   nrm = normalize(nw);
   col = mk(to_linear1(col.x), to_linear1(col.y), to_linear1(col.z));
   /* getSurfaceDataFromMat, bindless.glsl:181-237, untextured branch */
@@ -992,11 +1009,67 @@ void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, 
   f3 view = ld3(origin);
   f3 acc = mk(0.0f, 0.0f, 0.0f);
   if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, pos, view);
-  if (f->flags & ORA_FLAG_DDGI_MULTI)
+  if (f->flags & (mirror ? ORA_FLAG_DDGI : ORA_FLAG_DDGI_MULTI))
     acc = add(acc, calc_indirect(c, f, atlas, nrm, albedo, metallic, rough, pos, view));
   acc = add(acc, emissive);
   st3(out, acc);
   if (shadowed) *shadowed = lit ? 0 : 1;
+}
+
+void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, const float* atlas,
+                   const float* origin, const ora_hit* hit, float* out, int32_t* shadowed,
+                   uint64_t* counters)
+{
+  shade_surface(s, c, f, atlas, origin, hit, 0, c->t_min, c->t_max, out, shadowed, counters, NULL);
+}
+
+/* mirror_reflections.rgen:35-86 + .rchit + .rmiss: per pixel of a depth buffer,
+ * the world position (as shadow.rgen), the normal from normal4 (xyz of 4 floats,
+ * normalized), a closest-hit ray along reflect(normalize(pos - cameraPos), normal)
+ * over [0.1, 50], shaded as above; out4 = (irradiance, distance to the hit), a
+ * miss gives (sky, 1e6), a sky pixel (depth 1) gives 0.  counters4 (nullable):
+ * reflection node visits and triangle tests, shadow node visits and tests. */
+void ora_mirror_reflections(const ora_scene* s, const ora_config* c, const ora_frame* f,
+                            const float* atlas, const float* depth, const float* normal4, int32_t w,
+                            int32_t h, const float* ivp, float* out4, uint64_t* counters4)
+{
+  uint64_t rn = 0, rt = 0, sn = 0, stt = 0;
+  f3 cam = mk(f->camera_pos[0], f->camera_pos[1], f->camera_pos[2]);
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : rn, rt, sn, stt)
+  for (int64_t i = 0; i < (int64_t)w * h; ++i) {
+    float* o = out4 + 4 * i;
+    float d = depth[i];
+    if (d == 1.0f) { o[0] = o[1] = o[2] = o[3] = 0.0f; continue; }
+    int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    float u = (2.0f * (float)x + 1.0f) / (2.0f * (float)w);
+    float v = (2.0f * (float)y + 1.0f) / (2.0f * (float)h);
+    float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
+    float px = ivp[0] * cx + ivp[4] * cy + ivp[8] * d + ivp[12];
+    float py = ivp[1] * cx + ivp[5] * cy + ivp[9] * d + ivp[13];
+    float pz = ivp[2] * cx + ivp[6] * cy + ivp[10] * d + ivp[14];
+    float pw = ivp[3] * cx + ivp[7] * cy + ivp[11] * d + ivp[15];
+    f3 pos = mk(px / pw, py / pw, pz / pw);
+    f3 nrm = normalize(ld3(normal4 + 4 * i));
+    f3 vv = normalize(sub(pos, cam));
+    f3 dir = sub(vv, muls(nrm, 2.0f * dot(nrm, vv))); /* reflect(v, normal) */
+    ora_hit hit;
+    uint64_t a = 0, b = 0;
+    trace_one(s, pos, dir, 0.1f, 50.0f, 0, &hit, &a, &b);
+    rn += a; rt += b;
+    if (hit.tri < 0) {
+      ora_sky_radiance(f, o);
+      o[3] = 1e6f;
+      continue;
+    }
+    float origin[3] = {pos.x, pos.y, pos.z};
+    uint64_t sc[2] = {0, 0};
+    f3 hp;
+    shade_surface(s, c, f, atlas, origin, &hit, 1, 0.1f, 50.0f, o, NULL, sc, &hp);
+    sn += sc[0]; stt += sc[1];
+    f3 dd = sub(hp, pos);
+    o[3] = sqrtf(dot(dd, dd)); /* distance(pos, gl_WorldRayOriginEXT) */
+  }
+  if (counters4) { counters4[0] += rn; counters4[1] += rt; counters4[2] += sn; counters4[3] += stt; }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -156,6 +156,11 @@ void ora_sky_radiance(const ora_frame* fr, float* out3);
  * (pos, shadow, normal, metallic, albedo, roughness) -> out n x 4 (rgb, 0) */
 void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, const float* gbuf,
                    int64_t n, float* out);
+/* Mirror reflections (mirror_reflections.rgen/.rchit/.rmiss): out4 per pixel =
+ * (irradiance, hit distance); normal4 = 4 floats per pixel (xyz used). */
+void ora_mirror_reflections(const ora_scene* s, const ora_config* c, const ora_frame* f,
+                            const float* atlas, const float* depth, const float* normal4, int32_t w,
+                            int32_t h, const float* ivp, float* out4, uint64_t* counters4);
 void ora_shade_hit(const ora_scene* s, const ora_config* cfg, const ora_frame* fr,
                    const float* atlas, const float* origin, const ora_hit* hit, float* out3,
                    int32_t* shadowed, uint64_t* counters);

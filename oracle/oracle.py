@@ -42,6 +42,7 @@ def lib():
         L.ora_scene_bvh.argtypes = [vp] * 7
         L.ora_gi_lookup.argtypes = [vp, vp, vp, vp, i64, vp]
         L.ora_shadow_mask.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp]
+        L.ora_mirror_reflections.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp]
         L.ora_trace.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp, vp]
         L.ora_trace_bruteforce.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp]
         L.ora_fibonacci_dirs.argtypes = [i32, vp]
@@ -123,6 +124,18 @@ class OracleScene:
         out = np.zeros((h, w), np.float32)
         ctr = np.zeros(2, np.uint64)
         lib().ora_shadow_mask(self.h, ptr(d), w, h, ptr(ivp), ptr(light), ptr(out), ptr(ctr))
+        return out, ctr
+
+    def mirror_reflections(self, cfg, frame, atlas, depth, normal4, inv_view_proj16):
+        """Mirror reflections of a depth + normal buffer -> ([h][w][4] (rgb, dist), counters[4])."""
+        d = np.ascontiguousarray(depth, np.float32)
+        h, w = d.shape
+        nrm = np.ascontiguousarray(normal4, np.float32).reshape(h, w, 4)
+        ivp = np.ascontiguousarray(inv_view_proj16, np.float32)
+        out = np.zeros((h, w, 4), np.float32)
+        ctr = np.zeros(4, np.uint64)
+        lib().ora_mirror_reflections(self.h, _ref(cfg), _ref(frame), ptr(atlas), ptr(d), ptr(nrm), w, h,
+                                     ptr(ivp), ptr(out), ptr(ctr))
         return out, ctr
 
     def trace(self, orig, dirs, tmin=0.1, tmax=50.0, any_hit=False, brute=False):

@@ -36,8 +36,11 @@ def test_trace_rays_bit_exact(room):
     org = rng.uniform([-4.5, -0.4, -4.5], [4.5, 3.4, 4.5], (n, 3)).astype(np.float32)
     d = rng.normal(size=(n, 3))
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    # zero and non-finite directions are no rays (a miss, no traversal)
+    d[:4] = [[0, 0, 0], [np.nan, 0, 1], [np.inf, 0, 0], [0, -np.inf, np.nan]]
     for any_hit in (False, True):
         hg, cg = ds.trace_rays(org, d, any_hit=any_hit, count=True)
+        assert np.all(hg["tri"][:4] == -1)
         ho, co = osc.trace(org, d, any_hit=any_hit)
         assert np.array_equal(hg["tri"], ho["tri"])
         for f in ("t", "u", "v"):

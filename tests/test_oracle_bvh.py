@@ -180,3 +180,16 @@ def test_refit_moves_geometry_and_keeps_topology():
     inst["custom_index_and_mask"][3] ^= (1 << 10)
     with pytest.raises(ValueError):
         osc.refit(inst)
+
+
+def test_degenerate_directions_are_no_rays(osc):
+    """A zero or non-finite direction is no ray: a miss with no node visits.  A NaN
+    direction would otherwise pass every slab test (fmaxf/fminf drop NaN) and
+    visit the whole hierarchy."""
+    d = np.array([[0, 0, 0], [np.nan, 0, 1], [np.inf, 0, 0], [0, -np.inf, np.nan]], np.float32)
+    o = np.zeros((4, 3), np.float32)
+    for any_hit in (False, True):
+        h, ctr = osc.trace(o, d, any_hit=any_hit)
+        assert np.all(h["tri"] == -1) and np.all(ctr == 0)
+    h, ctr = osc.trace(o[:1], np.array([[0, -1, 0]], np.float32))
+    assert ctr[0] > 0

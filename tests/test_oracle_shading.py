@@ -231,3 +231,91 @@ def test_shadow_mask_equals_brute_force():
     want[depth == 1.0] = 1.0
     assert np.array_equal(vis, want)
     assert 0 < (vis == 0).sum() < vis.size and ctr[0] > 0
+
+
+def _rotated_room(seed=5, boxes=24):
+    """room_scene with every box rotated and scaled uniformly (similarity transforms)."""
+    sc = scenegen.room_scene(seed, boxes)
+    rng = np.random.default_rng(seed)
+    for k in range(1, len(sc.instances)):
+        t = sc.instances["transform"][k].reshape(3, 4).astype(np.float64)
+        x, y, z, w = rng.normal(size=4) / 1.0
+        q = np.array([x, y, z, w]) / math.sqrt(x * x + y * y + z * z + w * w)
+        x, y, z, w = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        M = R * rng.uniform(0.3, 1.0)
+        sc.instances["transform"][k] = np.concatenate([M, t[:, 3:]], 1).astype(np.float32).ravel()
+    return sc
+
+
+def _unit(v):
+    """GLSL normalize in float32 with the oracle's order: v / sqrt((x*x + y*y) + z*z)."""
+    l = np.sqrt((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2])
+    with np.errstate(invalid="ignore"):   # a zero vector gives NaN, as in GLSL
+        return v / l[..., None]
+
+
+def _mirror_rays(depth, normal4, ivp16, cam):
+    """mirror_reflections.rgen:51-64 in float32: origin, reflect(normalize(pos - cam), n)."""
+    f = np.float32
+    pos = _shadow_origins(depth, ivp16)
+    n = _unit(np.asarray(normal4, f)[..., :3])
+    v = _unit(pos - np.asarray(cam, f))
+    d = (n[..., 0] * v[..., 0] + n[..., 1] * v[..., 1]) + n[..., 2] * v[..., 2]
+    return pos, v - n * (f(2) * d)[..., None]
+
+
+def _mirror_case(seed=6, h=18, w=32):
+    scene = _rotated_room()
+    cfg = Config.make((8, 4, 8), step=(1.25, 1, 1.25), rays=64)
+    cam = (0.3, 1.8, 4.2)
+    fr = scenegen.make_frame(seed, camera=cam, config=cfg, flags=FLAG_DDGI)
+    rng = np.random.default_rng(seed)
+    atlas = O.new_atlas(cfg)
+    atlas[..., :3] = rng.uniform(0.05, 1.5, atlas[..., :3].shape).astype(np.float32)
+    _, ivp = _view_proj(cam, (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    depth = rng.uniform(0.95, 0.9995, (h, w)).astype(np.float32)
+    depth[rng.uniform(size=(h, w)) < 0.1] = 1.0
+    normal4 = rng.normal(size=(h, w, 4)).astype(np.float32)
+    normal4[h // 2, :4] = 0.0   # zero normals: non-finite directions, no rays
+    return scene, cfg, fr, atlas, depth, normal4, ivp16
+
+
+def test_mirror_reflections_brute_force_and_probe_shading():
+    """mirror_reflections.rgen/.rchit/.rmiss on the oracle.  The reflection ray's
+    closest hit equals brute force over all triangles; a sky pixel gives 0, a miss
+    (sky radiance, 1e6), a hit the distance to it (= t along the unit direction).
+    For similarity transforms mat3(ObjectToWorld) * n and n * WorldToObject have
+    the same direction, so with ANE_FLAG_DDGI the radiance equals the probe hit
+    shading (rchit) with DDGI_MULTI seen from the reflection origin; without
+    ANE_FLAG_DDGI (DDGI_MULTI alone does not count) it equals the probe shading
+    without the indirect term."""
+    scene, cfg, fr, atlas, depth, normal4, ivp16 = _mirror_case()
+    osc = O.OracleScene(scene)
+    out, ctr = osc.mirror_reflections(cfg, fr, atlas, depth, normal4, ivp16)
+    pos, dirs = _mirror_rays(depth, normal4, ivp16, fr.camera_pos[:3])
+    pos, dirs = pos.reshape(-1, 3), dirs.reshape(-1, 3)
+    hb = osc.trace(pos, dirs, tmin=0.1, tmax=50.0, brute=True)
+    o = out.reshape(-1, 4)
+    sky = depth.ravel() == 1.0
+    hit = ~sky & (hb["tri"] >= 0)
+    miss = ~sky & (hb["tri"] < 0)
+    assert np.all(o[sky] == 0.0)
+    assert sky.sum() > 0 and miss.sum() > 0 and hit.sum() > 0.5 * (~sky).sum()
+    none = np.zeros(1, HIT_DTYPE)[0]
+    none["tri"] = -1
+    sky_rad, _ = osc.shade_hit(cfg, fr, atlas, pos[0], none)
+    assert np.all(o[miss, 3] == np.float32(1e6)) and np.all(o[miss, :3] == sky_rad)
+    assert np.allclose(o[hit, 3], hb["t"][hit], rtol=1e-5, atol=1e-6)
+    assert ctr[0] > 0 and ctr[1] > 0 and ctr[2] > 0
+    idx = np.flatnonzero(hit)[::3]
+    for flags, probe_flags in ((FLAG_DDGI, FLAG_DDGI | FLAG_DDGI_MULTI), (FLAG_DDGI_MULTI, 0)):
+        fr.flags = flags
+        got = osc.mirror_reflections(cfg, fr, atlas, depth, normal4, ivp16)[0].reshape(-1, 4)
+        fr.flags = probe_flags
+        for i in idx:
+            want, _ = osc.shade_hit(cfg, fr, atlas, pos[i], hb[i])
+            assert np.allclose(got[i, :3], want, rtol=1e-5, atol=1e-6), (flags, i, got[i], want)
