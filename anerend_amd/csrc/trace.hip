@@ -38,10 +38,17 @@ namespace ane {
 #ifndef ANE_TRACE_WAVES
 #define ANE_TRACE_WAVES 7  // waves per SIMD the traversal is register-limited to
 #endif
+#ifndef ANE_TRACE_LEAFBATCH
+#define ANE_TRACE_LEAFBATCH 8  // waiting leaves that trigger a wave-wide leaf batch (0: no batching)
+#endif
+#ifndef ANE_TRACE_SPEC
+#define ANE_TRACE_SPEC 0  // 1: probe rays keep visiting internal nodes while a leaf waits
+#endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring, any size)
 static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
+static constexpr int kLeafBatch = ANE_TRACE_LEAFBATCH;
 
 int trace_stack_lds() { return kSS; }
 
@@ -237,7 +244,17 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
 {
   constexpr int32_t kDone = 0x7FFFFFFF;  // node sentinel: the lane's traversal has ended
   constexpr int32_t kPop = 0x7FFFFFFE;   // step result: continue from the stack
-  // (internal nodes are 0 .. ntri - 2 < kPop; leaves are ~position < 0)
+  constexpr int32_t kWait = 0x7FFFFFFD;  // stack empty, the last leaf still waits for its test
+  // (internal nodes are 0 .. ntri - 2 < kWait; leaves are ~position < 0)
+  // Leaf batching: a lane that reaches a leaf parks it in `pend`, pops its next
+  // node and waits; the triangle tests of all parked leaves run together once
+  // kLeafBatch lanes wait or no lane can go on, so that a wave runs the leaf
+  // code once per batch instead of in nearly every step.  The parked leaf is
+  // tested before the popped node is visited, so visits, hits and counts are
+  // those of the one-step-at-a-time traversal.  SPEC lanes instead keep
+  // visiting internal nodes with the older best distance (more visits, the same
+  // closest hit).
+  constexpr bool SPEC = ANE_TRACE_SPEC && !RAYS && !COUNT;
   __shared__ int32_t stk[kSS][kTB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -258,6 +275,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   RayPre r;
   float tbest = 0.0f, bu = 0.0f, bv = 0.0f;
   int32_t bid = -1, node = kDone, sp = 0, out = 0;
+  int32_t pend = 0;  // parked leaf (~position < 0), 0: none
   uint32_t steps_left = 0;
   unsigned long long cn = 0, ct = 0, sn = 0, st = 0;
 
@@ -342,8 +360,10 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
     }
     // ---- traversal steps: only node, sp, the best hit and the budget change ----
 #pragma unroll
-    for (int u = 0; u < ANE_TRACE_UNROLL; ++u)
-    if (ray >= 0 && node != kDone) {
+    for (int u = 0; u < ANE_TRACE_UNROLL; ++u) {
+    const bool go = ray >= 0 && node != kDone &&
+                    (kLeafBatch == 0 || (node != kWait && (SPEC ? !(node < 0 && pend != 0) : pend == 0)));
+    if (go) {
       int32_t next = kPop;
       // guards: a malformed hierarchy ends the ray and raises the error flag
       // instead of reading out of bounds or looping forever
@@ -356,6 +376,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
       if (bad) {
         *a.overflow = 1u;
         next = kDone;
+        pend = 0;
       } else if (isnode) {
         if (COUNT) {
           if (sph) ++sn; else ++cn;
@@ -384,6 +405,8 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
           stk[slot][tid] = farc;
           ++sp;
         }
+      } else if (kLeafBatch > 0) {
+        pend = node;  // the leaf waits for the next batch
       } else {
         if (COUNT) {
           if (sph) ++st; else ++ct;
@@ -405,7 +428,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
       }
       if (next == kPop) {
         if (sp == 0) {
-          next = kDone;
+          next = (kLeafBatch > 0 && pend != 0) ? kWait : kDone;
         } else {
           --sp;
           const int slot = (int)((uint32_t)sp % (uint32_t)kSS);
@@ -414,6 +437,38 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         }
       }
       node = next;
+    }
+    if (kLeafBatch > 0) {
+      const uint64_t pm = __ballot(pend != 0);
+      if (pm != 0ull) {
+        const bool can = ray >= 0 && node != kDone && node != kWait &&
+                         (SPEC ? !(node < 0 && pend != 0) : pend == 0);
+        if (__popcll(pm) >= kLeafBatch || __ballot(can) == 0ull) {
+          if (pend != 0) {
+            if (COUNT) {
+              if (sph) ++st; else ++ct;
+            }
+            const int64_t p = ~pend;
+            const float4 v0 = sc.tris[3 * p], v1 = sc.tris[3 * p + 1], v2 = sc.tris[3 * p + 2];
+            float t, u, v;
+            bool ended = node == kWait;  // nothing left to visit
+            if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin,
+                         tbest, t, u, v)) {
+              const int32_t id = __float_as_int(v0.w);
+              if (sph || t < tbest || (t == tbest && id < bid)) {
+                bid = id;
+                tbest = t;
+                bu = u;
+                bv = v;
+              }
+              if (sph) ended = true;  // any hit ends a shadow ray
+            }
+            if (ended) node = kDone;
+            pend = 0;
+          }
+        }
+      }
+    }
     }
   }
   if (COUNT) {
