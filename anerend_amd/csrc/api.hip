@@ -577,6 +577,7 @@ struct ane_ddgi_ctx {
   void* d_rad = nullptr;
   float* d_rad_scale = nullptr;
   float* d_stage = nullptr;  // FP32 staging for radiance get/set
+  float2* d_tiles = nullptr;  // tile texture bounds: nx + 1 columns, then (ny + 1) * (nz + 1) rows
   uint32_t* d_work = nullptr;
   unsigned long long* d_stats = nullptr;
   uint32_t* d_overflow = nullptr;
@@ -609,6 +610,7 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_w32); dfree(c->d_wsum); dfree(c->d_wbf16); dfree(c->d_wfp8); dfree(c->d_wfp8_scale);
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
+  dfree(c->d_tiles);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
   dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
   for (auto& t : c->timers) t.destroy();
@@ -687,12 +689,34 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   ANE_A(d_work, 4);
   ANE_A(d_stats, 4);
   ANE_A(d_overflow, 1);
+  ANE_A(d_tiles, (int64_t)(g.nx + 1) + (int64_t)(g.ny + 1) * (g.nz + 1));
 #undef ANE_A
   hipError_t e;
   for (auto& t : c->timers)
     if ((st = t.init()) != ANE_OK) {
       ctx_free(c); delete c; return st;
     }
+  {
+    // tile texture bounds, the operations of sampleSingleProbe (pbr_light.glsl:177-183)
+    // in binary32: start = (2 * s0 + 1) / (2 * extent), end for s0 + 7
+    std::vector<float2> tiles((size_t)(g.nx + 1) + (size_t)(g.ny + 1) * (g.nz + 1));
+    auto bounds = [](int s0, int extent) {
+      const float su = (2.0f * (float)s0 + 1.0f) / (2.0f * (float)extent);
+      const float eu = (2.0f * (float)(s0 + 7) + 1.0f) / (2.0f * (float)extent);
+      return make_float2(su, eu - su);
+    };
+    for (int x = 0; x <= g.nx; ++x) tiles[(size_t)x] = bounds(10 * x + 1, g.atlas_w);
+    for (int y = 0; y <= g.ny; ++y)
+      for (int z = 0; z <= g.nz; ++z)
+        tiles[(size_t)(g.nx + 1) + (size_t)y * (g.nz + 1) + z] = bounds(10 * g.nz * y + 10 * z + 1, g.atlas_h);
+    e = hipMemcpy(c->d_tiles, tiles.data(), tiles.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      ctx_free(c); delete c;
+      return fail(ANE_ERR_DEVICE, "tile table upload: %s", hipGetErrorString(e));
+    }
+    g.tile_u = c->d_tiles;
+    g.tile_v = c->d_tiles + (g.nx + 1);
+  }
   // atlas cleared to (0, 0, 0, 1): IrradianceProbeConvolvePass.cpp:20-24
   std::vector<float> clear((size_t)c->texels * 4);
   for (int64_t i = 0; i < c->texels; ++i) {

@@ -75,6 +75,11 @@ struct GridK {
   float hysteresis;
   float t_min, t_max;
   int32_t atlas_w, atlas_h;
+  // texture bounds of the probe tiles (sampleSingleProbe, pbr_light.glsl:177-183):
+  // (start, end - start) of tile column 0..nx and of tile row (y, z) at
+  // y * (nz + 1) + z for y 0..ny, z 0..nz; device memory
+  const float2* tile_u;
+  const float2* tile_v;
 };
 
 // Device view of a scene (all pointers device memory).

@@ -41,6 +41,12 @@ namespace ane {
 #ifndef ANE_TRACE_LEAFBATCH
 #define ANE_TRACE_LEAFBATCH 8  // waiting leaves that trigger a wave-wide leaf batch (0: no batching)
 #endif
+#ifndef ANE_SHADE_WAVES
+#define ANE_SHADE_WAVES 1  // waves per SIMD the shading kernels are register-limited to (1: compiler's choice)
+#endif
+#ifndef ANE_SHADE_UNROLL
+#define ANE_SHADE_UNROLL 1  // probe corners of sampleProbe unrolled together (register use vs loads in flight)
+#endif
 #ifndef ANE_TRACE_SPEC
 #define ANE_TRACE_SPEC 0  // 1: probe rays keep visiting internal nodes while a leaf waits
 #endif
@@ -49,6 +55,7 @@ static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (
 static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 static constexpr int kLeafBatch = ANE_TRACE_LEAFBATCH;
+static constexpr int kShadeUnroll = ANE_SHADE_UNROLL;
 
 int trace_stack_lds() { return kSS; }
 
@@ -528,6 +535,15 @@ __device__ __forceinline__ float hw_rsqrt(float x) { return __builtin_amdgcn_rsq
 __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ f3 hw_normalize(f3 v) { return muls(v, hw_rsqrt(dot(v, v))); }
 
+// toLinear (bindless.glsl:240-247) with the power as hardware exp2(2.4 * log2 x)
+// (about 1e-6 relative on the branch's range x >= 0.09): a continuous term of
+// the hit color
+__device__ __forceinline__ float to_linear_hw(float c)
+{
+  return (c < 0.04045f) ? c / 12.92f
+                        : __builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf((c + 0.055f) / 1.055f));
+}
+
 __device__ __forceinline__ float distribution_ggx(f3 N, f3 H, float rough)
 {
   float a = rough * rough;
@@ -585,6 +601,19 @@ __device__ __forceinline__ int imod(int a, int m)
   return r < 0 ? r + m : r;
 }
 
+// a mod m for REPEAT addressing.  Tile texel coordinates lie in [-1, 3m) (the
+// tiles one past the grid edge wrap once), which a conditional add and two
+// conditional subtractions reduce; anything else (a non-finite coordinate)
+// takes the general modulo.
+__device__ __forceinline__ int wrap_repeat(int a, int m)
+{
+  int r = a < 0 ? a + m : a;
+  r = r >= m ? r - m : r;
+  r = r >= m ? r - m : r;
+  if (r < 0 || r >= m) r = imod(a, m);
+  return r;
+}
+
 // texture() with LINEAR filtering and REPEAT addressing (PipelineUtil.h:98-112),
 // binary32 weights.
 __device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float v)
@@ -593,8 +622,8 @@ __device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float 
   float y = v * (float)H - 0.5f;
   float fx = floorf(x), fy = floorf(y);
   float ax = x - fx, ay = y - fy;
-  int x0 = imod((int)fx, W), x1 = imod((int)fx + 1, W);
-  int y0 = imod((int)fy, H), y1 = imod((int)fy + 1, H);
+  int x0 = wrap_repeat((int)fx, W), x1 = wrap_repeat((int)fx + 1, W);
+  int y0 = wrap_repeat((int)fy, H), y1 = wrap_repeat((int)fy + 1, H);
   float4 t00 = atlas[(int64_t)y0 * W + x0];
   float4 t10 = atlas[(int64_t)y0 * W + x1];
   float4 t01 = atlas[(int64_t)y1 * W + x0];
@@ -605,16 +634,12 @@ __device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float 
   return mk(r0 * (1.0f - ay) + r1 * ay, g0 * (1.0f - ay) + g1 * ay, b0 * (1.0f - ay) + b1 * ay);
 }
 
-// sampleSingleProbe, pbr_light.glsl:168-188: the texture coordinates of the
-// normal's octahedral position inside the tile of probe (ix, iy, iz).  The
-// tile bounds of the 8 probes of sampleProbe take 2 distinct columns and 4
-// distinct rows, so they are computed once each (same operations, same values).
-__device__ __forceinline__ float tile_coord(int s0, int extent, float o)
-{
-  const float su = (2.0f * (float)s0 + 1.0f) / (2.0f * (float)extent);
-  const float eu = (2.0f * (float)(s0 + 7) + 1.0f) / (2.0f * (float)extent);
-  return su + (eu - su) * o;
-}
+// sampleSingleProbe, pbr_light.glsl:168-188: the texture coordinate of the
+// normal's octahedral position inside the tile of probe (ix, iy, iz) is
+// start + (end - start) * o, with the tile's (start, end - start) from the
+// grid's table (the same binary32 operations, done once per grid).  The 8
+// probes of sampleProbe take 2 distinct columns and 4 distinct rows.
+__device__ __forceinline__ float tile_coord(float2 t, float o) { return t.x + t.y * o; }
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
@@ -651,18 +676,18 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
   const int W = g.atlas_w, H = g.atlas_h;
   float tu[2], tv[4];
 #pragma unroll
-  for (int dx = 0; dx < 2; ++dx) tu[dx] = tile_coord(10 * (px + dx) + 1, W, octx);
+  for (int dx = 0; dx < 2; ++dx) tu[dx] = tile_coord(g.tile_u[px + dx], octx);
 #pragma unroll
   for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
     for (int dz = 0; dz < 2; ++dz)
-      tv[2 * dy + dz] = tile_coord(10 * g.nz * (py + dy) + 10 * (pz + dz) + 1, H, octy);
+      tv[2 * dy + dz] = tile_coord(g.tile_v[(py + dy) * (g.nz + 1) + pz + dz], octy);
   // corner order of pbr_light.glsl:318-355, packed as ox | oy << 1 | oz << 2
   const int corners = (0) | (7 << 3) | (3 << 6) | (1 << 9) | (6 << 12) | (2 << 15) | (4 << 18) |
                       (5 << 21);
   float sumw = 0.0f;
   f3 sumi = mk(0.0f, 0.0f, 0.0f);
-#pragma unroll
+#pragma unroll kShadeUnroll
   for (int k = 0; k < 8; ++k) {
     const int c = (corners >> (3 * k)) & 7;
     const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
@@ -675,7 +700,8 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
     w = fmaxf(0.000001f, w);
     if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
     w = w * (tri.x * tri.y * tri.z);
-    f3 s = bilinear_repeat(atlas, W, H, tu[ox], tv[2 * oy + oz]);
+    const float tvs = oy ? (oz ? tv[3] : tv[2]) : (oz ? tv[1] : tv[0]);
+    f3 s = bilinear_repeat(atlas, W, H, ox ? tu[1] : tu[0], tvs);
     s = mk(hw_sqrt(s.x), hw_sqrt(s.y), hw_sqrt(s.z));
     sumi = add(sumi, muls(s, w));
     sumw = sumw + w;
@@ -744,7 +770,7 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
             nrm.x * wi[2] + nrm.y * wi[6] + nrm.z * wi[10]);
   }
   nrm = normalize(nw);
-  col = mk(to_linear1(col.x), to_linear1(col.y), to_linear1(col.z));
+  col = mk(to_linear_hw(col.x), to_linear_hw(col.y), to_linear_hw(col.z));
   f3 albedo = col;
   if (mat.base_col_fac[0] + mat.base_col_fac[1] + mat.base_col_fac[2] > 0.1f)
     albedo = mk(mat.base_col_fac[0], mat.base_col_fac[1], mat.base_col_fac[2]);
@@ -760,7 +786,7 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
 }
 
 template <int PREC, int TB>
-__global__ __launch_bounds__(TB) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
                                               const ane_hit* hits, const uint8_t* shadow, const float4* atlas, void* rad,
                                               float* rad_scale, int probe_base)
 {
@@ -871,7 +897,7 @@ void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float
 // deferred GI lookup (deferred_tiled.comp:264-298): ambient light of G-buffer
 // texels from the atlas, through the shading's calcIndirectDiffuseLight
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gi_lookup(GridK g, FrameK f, const float4* atlas,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_gi_lookup(GridK g, FrameK f, const float4* atlas,
                                                    const float4* gbuf, int64_t n, float4* out)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1023,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_mirror_shadow_rays(SceneK sc, const ane
 }
 
 // out = (irradiance, distance to the hit); a miss: (sky, 1e6); a sky pixel: 0
-__global__ __launch_bounds__(256) void k_mirror_shade(SceneK sc, GridK g, FrameK f, const float4* atlas,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_mirror_shade(SceneK sc, GridK g, FrameK f, const float4* atlas,
                                                       const float* depth, const float* orig,
                                                       const ane_hit* hits, const ane_hit* shadow_hits,
                                                       int64_t n, float4* out)
