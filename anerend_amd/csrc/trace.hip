@@ -47,6 +47,9 @@ namespace ane {
 #ifndef ANE_SHADE_UNROLL
 #define ANE_SHADE_UNROLL 1  // probe corners of sampleProbe unrolled together (register use vs loads in flight)
 #endif
+#ifndef ANE_TRACE_LEAFBATCH_RAYS
+#define ANE_TRACE_LEAFBATCH_RAYS 0  // the same for caller rays
+#endif
 #ifndef ANE_TRACE_SPEC
 #define ANE_TRACE_SPEC 0  // 1: probe rays keep visiting internal nodes while a leaf waits
 #endif
@@ -54,7 +57,8 @@ static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring, any size)
 static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
-static constexpr int kLeafBatch = ANE_TRACE_LEAFBATCH;
+static constexpr int kLeafBatchProbe = ANE_TRACE_LEAFBATCH;
+static constexpr int kLeafBatchRays = ANE_TRACE_LEAFBATCH_RAYS;
 static constexpr int kShadeUnroll = ANE_SHADE_UNROLL;
 
 int trace_stack_lds() { return kSS; }
@@ -262,6 +266,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   // visiting internal nodes with the older best distance (more visits, the same
   // closest hit).
   constexpr bool SPEC = ANE_TRACE_SPEC && !RAYS && !COUNT;
+  constexpr int kLeafBatch = RAYS ? kLeafBatchRays : kLeafBatchProbe;
   __shared__ int32_t stk[kSS][kTB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
