@@ -138,6 +138,7 @@ struct StageTimer {
   hipEvent_t e[kSlots] = {};
   bool pending[kSlots] = {};
   int next = 0, cur = -1;
+  bool off = false;  // no events (while an update is captured into a graph)
   double acc_ms = 0.0;
   uint64_t count = 0;
   float last_ms = 0.0f;
@@ -181,6 +182,7 @@ struct StageTimer {
   }
   int begin(hipStream_t st)
   {
+    if (off) return 0;
     cur = next;
     int r = harvest(cur);
     if (r) return r;
@@ -189,6 +191,7 @@ struct StageTimer {
   }
   int end(hipStream_t st)
   {
+    if (off) return 0;
     ANE_CHECK_HIP(hipEventRecord(e[cur], st));
     pending[cur] = true;
     next = (cur + 1) % kSlots;
@@ -591,6 +594,7 @@ struct ane_ddgi_ctx {
   bool counting = false;
   ane_ddgi_stats stats{};
   int64_t bytes = 0;
+  hipGraphExec_t gexec = nullptr;  // the last captured update (request loop)
   // mirror-reflection workspace, sized on first use: [0] reflection rays,
   // [1] their shadow rays
   float* d_mr_orig[2] = {nullptr, nullptr};
@@ -611,6 +615,7 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
   dfree(c->d_tiles);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
   dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
   for (auto& t : c->timers) t.destroy();
@@ -889,6 +894,53 @@ extern "C" int ane_ddgi_update(ane_ddgi_ctx* c, const ane_scene* s, const ane_dd
   ANE_TRY(ane_ddgi_trace(c, s, f, stream));
   return ane_ddgi_convolve(c, f, stream);
 }
+
+namespace ane {
+// The update of ane_ddgi_update captured from `st` into a graph and launched on
+// it (used by the request loop).  The context keeps the executable graph and
+// updates it in place from each new capture (kernel arguments change with the
+// frame; the node structure changes only with the layer range or the bake flag,
+// which re-instantiates it).  Validation and allocation happen before the
+// capture; stage timers record nothing meanwhile, so the graph holds kernel
+// nodes only (counters are cleared by a fill kernel).  Counting contexts update
+// directly.
+int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st)
+{
+  if (!c || !s) return fail(ANE_ERR_INVALID_ARG, "NULL context or scene");
+  FrameK fk;
+  ANE_TRY(frame_to_k(c, f, &fk));
+  ANE_TRY(ensure_spill(c, s->depth));
+  if (c->counting) return ane_ddgi_update(c, s, f, st);
+  for (auto& t : c->timers) t.off = true;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    for (auto& t : c->timers) t.off = false;
+    return fail(ANE_ERR_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+  }
+  const int r = ane_ddgi_update(c, s, f, st);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(st, &g);
+  for (auto& t : c->timers) t.off = false;
+  if (r != ANE_OK || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return r != ANE_OK ? r : fail(ANE_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  if (c->gexec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult how = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate(c->gexec, g, &bad, &how) != hipSuccess || how != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();  // a structural change: instantiate anew
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+  }
+  if (!c->gexec) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(c->gexec, st);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(ANE_ERR_DEVICE, "graph launch: %s", hipGetErrorString(e));
+  return ANE_OK;
+}
+}  // namespace ane
 
 extern "C" int ane_ddgi_translate(ane_ddgi_ctx* c, const float* cam, int32_t* shifted, void* stream)
 {

@@ -29,6 +29,7 @@
 
 namespace ane {
 int fail(int status, const char* fmt, ...);
+int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st);
 }
 using ane::fail;
 
@@ -177,7 +178,8 @@ void ane_serve::run()
       std::vector<std::pair<uint64_t, int>> launched;
       launched.reserve(batch.size());
       for (Launch& b : batch) {
-        int st = ane_ddgi_update(b.ctx, b.scene, &b.frame, b.stream);
+        int st = cfg.capture_graphs ? ane::ddgi_update_graph(b.ctx, b.scene, &b.frame, b.stream)
+                                    : ane_ddgi_update(b.ctx, b.scene, &b.frame, b.stream);
         if (st == ANE_OK && hipEventRecord(b.done, b.stream) != hipSuccess) st = ANE_ERR_DEVICE;
         launched.push_back({b.ticket, st});
       }
@@ -225,6 +227,9 @@ extern "C" int ane_serve_open(const ane_serve_config* cfg, ane_serve** out)
   if (cfg->hbm_budget_bytes <= 0) return fail(ANE_ERR_INVALID_ARG, "hbm_budget_bytes must be positive");
   if (cfg->max_inflight < 1 || cfg->max_inflight > 256)
     return fail(ANE_ERR_INVALID_ARG, "max_inflight must be in 1..256");
+  if (cfg->capture_graphs != 0 && cfg->capture_graphs != 1)
+    return fail(ANE_ERR_INVALID_ARG, "capture_graphs must be 0 or 1");
+  if (cfg->reserved != 0) return fail(ANE_ERR_INVALID_ARG, "reserved must be 0");
   if (cfg->num_streams < 1 || cfg->num_streams > 16)
     return fail(ANE_ERR_INVALID_ARG, "num_streams must be in 1..16");
   ane_serve* s = new ane_serve();

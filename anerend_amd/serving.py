@@ -13,7 +13,9 @@ ANE_ERR_NOT_READY = 5
 class ServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64),
                 ("max_inflight", ctypes.c_int32),
-                ("num_streams", ctypes.c_int32)]
+                ("num_streams", ctypes.c_int32),
+                ("capture_graphs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -32,16 +34,16 @@ class ServeStats(ctypes.Structure):
                 ("used_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64)]
 
 
-assert ctypes.sizeof(ServeConfig) == 16
+assert ctypes.sizeof(ServeConfig) == 24
 assert ctypes.sizeof(ServeResult) == 32
 
 
 class Server:
     """Owns the request loop; sessions reference DeviceScenes the caller keeps alive."""
 
-    def __init__(self, hbm_budget_bytes, max_inflight=8, num_streams=4):
+    def __init__(self, hbm_budget_bytes, max_inflight=8, num_streams=4, capture_graphs=True):
         L = native.load()
-        c = ServeConfig(int(hbm_budget_bytes), int(max_inflight), int(num_streams))
+        c = ServeConfig(int(hbm_budget_bytes), int(max_inflight), int(num_streams), int(bool(capture_graphs)), 0)
         self._h = ctypes.c_void_p()
         native.check(L.ane_serve_open(ctypes.byref(c), ctypes.byref(self._h)))
         self.configs = {}

@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--requests", type=int, default=2000, help="serve64: open-loop requests")
     p.add_argument("--burst", type=int, default=256, help="serve64: requests of the saturation burst")
     p.add_argument("--inflight", type=int, default=8, help="serve64: requests on the device at once")
+    p.add_argument("--no-graphs", action="store_true", help="serve64: direct launches instead of captured graphs")
     p.add_argument("--streams", type=int, default=4, help="serve64: HIP streams of the server")
     p.add_argument("--frames", type=int, default=300, help="frames of the dynamic4m workload")
     p.add_argument("--rebuild-every", type=int, default=1,
@@ -477,7 +478,8 @@ def bench_serve(args):
         dss.append(native.DeviceScene(sc))
     cfgs = [scenegen.serving_config(sp, PRECISION_BF16) for sp in specs]
     budget = 200 << 30
-    srv = serving.Server(hbm_budget_bytes=budget, max_inflight=args.inflight, num_streams=args.streams)
+    srv = serving.Server(hbm_budget_bytes=budget, max_inflight=args.inflight, num_streams=args.streams,
+                         capture_graphs=not args.no_graphs)
     sids = [srv.add_session(d, c) for d, c in zip(dss, cfgs)]
     log(f"[bench] {tris} triangles resident, {srv.stats()['used_bytes'] / 2**30:.2f} GiB admitted")
     R, rate = args.requests, 200.0
@@ -549,7 +551,7 @@ def bench_serve(args):
         "config": {"workload": "serve64: 64 scenes, 10k-2M triangles log-uniform, grids "
                                "{4^3, 8^3, 16x8x16, 32x16x32} x rays {128, 256, 512}, BF16 conv",
                    "triangles": tris, "hbm_budget_bytes": budget, "max_inflight": args.inflight,
-                   "streams": args.streams},
+                   "streams": args.streams, "capture_graphs": not args.no_graphs},
         "open_loop": open_loop,
         "burst": burst,
     }

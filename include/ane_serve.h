@@ -35,6 +35,9 @@ typedef struct ane_serve_config {
   int64_t hbm_budget_bytes; /* admission budget for resident sessions */
   int32_t max_inflight;     /* requests on the device at once, 1..256 */
   int32_t num_streams;      /* HIP streams the requests are spread over, 1..16 */
+  int32_t capture_graphs;   /* 1: each update is captured into the session's HIP graph
+                               (updated in place, one launch per request); 0: direct launches */
+  int32_t reserved;         /* 0 */
 } ane_serve_config;
 
 typedef struct ane_serve_result {

@@ -100,7 +100,8 @@ int main(void) {
     L = native.load()
     h = ctypes.c_void_p()
     for cfg in (serving.ServeConfig(0, 4, 2), serving.ServeConfig(1 << 30, 0, 2),
-                serving.ServeConfig(1 << 30, 4, 0), serving.ServeConfig(1 << 30, 4, 17)):
+                serving.ServeConfig(1 << 30, 4, 0), serving.ServeConfig(1 << 30, 4, 17),
+                serving.ServeConfig(1 << 30, 4, 2, 2, 0), serving.ServeConfig(1 << 30, 4, 2, 1, 7)):
         assert L.ane_serve_open(ctypes.byref(cfg), ctypes.byref(h)) == 1
         assert not h.value
 
