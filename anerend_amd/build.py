@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libane_ddgi.so")
-SOURCES = ["api.hip", "bvh.hip", "trace.hip", "conv.hip", "translate.hip", "serve.hip"]
+SOURCES = ["api.hip", "bvh.hip", "trace.hip", "conv.hip", "translate.hip", "serve.hip", "sungrid.hip"]
 
 # Float contract: no contraction, correctly rounded divide/sqrt, denormals kept.
 FLAGS = [

@@ -246,6 +246,7 @@ struct ane_scene {
   std::vector<int32_t> level_begin;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t depth = 0;
+  uint64_t version = 0;  // changes with every build or refit (keys derived data such as sun grids)
   mutable float build_ms = 0.0f;
   mutable bool build_ms_pending = false;  // refit timing not yet read back
   // pinned double-buffered staging of instance records and inverse transforms:
@@ -334,6 +335,7 @@ static int upload_instances(ane_scene* s, const ane_instance* inst, hipStream_t 
 // full LBVH build from the uploaded instances
 static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
 {
+  ++s->version;
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
   const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
@@ -359,6 +361,7 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
 
 static int scene_refit_dev(ane_scene* s, hipStream_t st)
 {
+  ++s->version;
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
   const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
@@ -591,7 +594,20 @@ struct ane_ddgi_ctx {
   int32_t rad_begin = -1, rad_count = 0;  // layers held in the radiance buffer
   bool rad_bake = false;
   int32_t last_begin = 0, last_count = 0;
-  bool counting = false;
+  int counting = 0;  // 1: counting kernels, shadow rays traverse; 2: counting kernels, sun grid
+  // sun grid of the probe shadow rays, rebuilt when the scene or the sun changes
+  SunGridParams* d_sg_params = nullptr;
+  uint32_t* d_sg_acc = nullptr;   // [6] bounds reduction
+  uint32_t* d_sg_start = nullptr; // [cells + 1] counts, scanned into starts
+  uint32_t* d_sg_cur = nullptr;   // [cells] scatter cursors
+  uint32_t* d_sg_list = nullptr;  // [cap] candidates
+  uint32_t* d_sg_scan = nullptr;  // scan scratch
+  int sg_g = 0;
+  uint32_t sg_cap = 0;
+  const ane_scene* sg_scene = nullptr;
+  uint64_t sg_version = 0;
+  uint32_t sg_dir[3] = {0, 0, 0};
+  bool sg_built = false;
   ane_ddgi_stats stats{};
   int64_t bytes = 0;
   hipGraphExec_t gexec = nullptr;  // the last captured update (request loop)
@@ -615,6 +631,8 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
   dfree(c->d_tiles);
+  dfree(c->d_sg_params); dfree(c->d_sg_acc); dfree(c->d_sg_start); dfree(c->d_sg_cur); dfree(c->d_sg_list);
+  dfree(c->d_sg_scan);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
   dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
@@ -796,6 +814,56 @@ static int ensure_spill(ane_ddgi_ctx* c, int32_t depth)
   return ANE_OK;
 }
 
+// Sun-grid sizes for a scene: about one cell per triangle, at most 4096 x 4096;
+// capacity 16 entries per triangle (the grid falls back to traversal beyond it).
+static int sun_grid_alloc(ane_ddgi_ctx* c, const ane_scene* s)
+{
+  int g = (int)std::ceil(std::sqrt((double)std::max(s->ntri, 1)));
+  g = std::min(std::max(g, 16), 4096);
+  const int64_t capl = std::min<int64_t>((int64_t)16 * s->ntri + 1024, (int64_t)1 << 30);
+  const uint32_t cap = (uint32_t)capl;
+  if (g <= c->sg_g && cap <= c->sg_cap) return ANE_OK;
+  dfree(c->d_sg_params); dfree(c->d_sg_acc); dfree(c->d_sg_start); dfree(c->d_sg_cur); dfree(c->d_sg_list);
+  dfree(c->d_sg_scan);
+  c->sg_g = 0;
+  c->sg_cap = 0;
+  c->sg_built = false;
+  const int64_t ncell = (int64_t)g * g;
+  ANE_TRY(dalloc(&c->d_sg_params, 1, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_acc, 8, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_start, ncell + 1, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_cur, ncell, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_list, (int64_t)cap, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_scan, (ncell + 1) / 1024 + 256, nullptr));
+  c->sg_g = g;
+  c->sg_cap = cap;
+  return ANE_OK;
+}
+
+static bool sun_grid_wanted(const ane_ddgi_ctx* c) { return c->counting != 1 && trace_grid_supported(); }
+
+// (Re)builds the sun grid on `st` unless the one for this scene, scene version
+// and sun direction is current.
+static int sun_grid_update(ane_ddgi_ctx* c, const ane_scene* s, const float* sun3, hipStream_t st)
+{
+  ANE_TRY(sun_grid_alloc(c, s));
+  uint32_t bits[3];
+  std::memcpy(bits, sun3, sizeof(bits));
+  if (c->sg_built && c->sg_scene == s && c->sg_version == s->version && bits[0] == c->sg_dir[0] &&
+      bits[1] == c->sg_dir[1] && bits[2] == c->sg_dir[2])
+    return ANE_OK;
+  SunGridParams g;
+  sun_grid_setup(g, sun3, c->sg_g, c->sg_g);
+  launch_sun_grid_build(s->view, g, c->d_sg_params, c->d_sg_acc, c->d_sg_start, c->d_sg_cur, c->d_sg_list,
+                        c->sg_cap, c->d_sg_scan, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  c->sg_built = true;
+  c->sg_scene = s;
+  c->sg_version = s->version;
+  std::memcpy(c->sg_dir, bits, sizeof(bits));
+  return ANE_OK;
+}
+
 static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& fk)
 {
   TraceArgs a{};
@@ -847,6 +915,12 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   ANE_TRY(c->timers[kStSetup].end(st));
   TraceArgs a = trace_args(c, s, fk);
   a.work = c->d_work;
+  if (sun_grid_wanted(c)) {
+    ANE_TRY(sun_grid_update(c, s, a.sun, st));
+    a.sg.params = c->d_sg_params;
+    a.sg.start = c->d_sg_start;
+    a.sg.list = c->d_sg_list;
+  }
   // primary and shadow rays in one persistent launch (the trace stage)
   ANE_TRY(c->timers[kStTrace].begin(st));
   launch_trace(a, false, false, c->counting, st);
@@ -910,6 +984,7 @@ int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame*
   FrameK fk;
   ANE_TRY(frame_to_k(c, f, &fk));
   ANE_TRY(ensure_spill(c, s->depth));
+  if (sun_grid_wanted(c)) ANE_TRY(sun_grid_alloc(c, s));  // allocation outside the capture
   if (c->counting) return ane_ddgi_update(c, s, f, st);
   for (auto& t : c->timers) t.off = true;
   hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
@@ -1194,7 +1269,8 @@ extern "C" int ane_ddgi_debug_rays(ane_ddgi_ctx* c, ane_hit* hits, uint8_t* shad
 extern "C" int ane_ddgi_set_counting(ane_ddgi_ctx* c, int32_t enable)
 {
   if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
-  c->counting = enable != 0;
+  if (enable < 0 || enable > 2) return fail(ANE_ERR_INVALID_ARG, "counting mode %d outside 0..2", enable);
+  c->counting = enable;
   return ANE_OK;
 }
 

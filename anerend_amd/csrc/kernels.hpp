@@ -32,6 +32,33 @@ int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, 
 void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
                 const std::vector<int32_t>& level_begin, hipStream_t s);
 // trace.hip
+// Sun grid of the probe shadow rays (sungrid.hip): projected along the sun
+// direction, cell (cx, cy) of a point = (sun_cell(X, x0, icw, gx), sun_cell(Y, y0, ich, gy))
+struct SunGridParams {
+  float x0, y0, icw, ich;  // grid origin and inverse cell sizes
+  float sx, sy, margin;    // shear of the direction; widening of the triangle rectangles
+  int32_t kx, ky, kz;      // axis permutation (the watertight test's)
+  int32_t gx, gy;          // cells
+  uint32_t valid;          // 0: the traversal takes the shadow rays
+};
+struct SunGridK {
+  const SunGridParams* params;  // device
+  const uint32_t* start;        // [gx * gy + 1] first candidate of each cell
+  const uint32_t* list;         // candidates (sorted triangle positions); null: no grid
+};
+__host__ __device__ __forceinline__ int sun_cell(float v, float o, float inv, int n)
+{
+  float f = floorf((v - o) * inv);
+  f = fminf(fmaxf(f, 0.0f), (float)(n - 1));  // also NaN -> 0
+  return (int)f;
+}
+void sun_grid_setup(SunGridParams& g, const float* dir3, int gx, int gy);
+void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
+                           uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
+                           hipStream_t s);
+// probe traces can take a sun grid (needs leaf batching)
+bool trace_grid_supported();
+
 struct TraceArgs {
   SceneK sc;
   GridK g;
@@ -57,6 +84,7 @@ struct TraceArgs {
   const float* ray_orig;
   const float* ray_dir;
   float t_min, t_max;
+  SunGridK sg;             // probe rays: candidates of the shadow rays (list null: traversal)
 };
 
 int trace_lanes_per_launch();

@@ -61,6 +61,8 @@ static constexpr int kLeafBatchProbe = ANE_TRACE_LEAFBATCH;
 static constexpr int kLeafBatchRays = ANE_TRACE_LEAFBATCH_RAYS;
 static constexpr int kShadeUnroll = ANE_SHADE_UNROLL;
 
+bool trace_grid_supported() { return kLeafBatchProbe > 0; }
+
 int trace_stack_lds() { return kSS; }
 
 // Direction-only part of the ray setup (RayPre without the origin), 48 bytes:
@@ -256,6 +258,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   constexpr int32_t kDone = 0x7FFFFFFF;  // node sentinel: the lane's traversal has ended
   constexpr int32_t kPop = 0x7FFFFFFE;   // step result: continue from the stack
   constexpr int32_t kWait = 0x7FFFFFFD;  // stack empty, the last leaf still waits for its test
+  constexpr int32_t kGridRay = 0x7FFFFFFC;  // shadow ray walking its sun-grid cell (sp: cursor, steps_left: end)
   // (internal nodes are 0 .. ntri - 2 < kWait; leaves are ~position < 0)
   // Leaf batching: a lane that reaches a leaf parks it in `pend`, pops its next
   // node and waits; the triangle tests of all parked leaves run together once
@@ -267,6 +270,9 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   // closest hit).
   constexpr bool SPEC = ANE_TRACE_SPEC && !RAYS && !COUNT;
   constexpr int kLeafBatch = RAYS ? kLeafBatchRays : kLeafBatchProbe;
+  // probe shadow rays test the triangles of their sun-grid cell (parked one at a
+  // time as batch leaves) instead of traversing the hierarchy
+  const bool sgrid = !RAYS && kLeafBatch > 0 && a.sg.list != nullptr && a.sg.params->valid != 0u;
   __shared__ int32_t stk[kSS][kTB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -310,9 +316,21 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         sph = true;
         tbest = tmax0;
         bid = -1;
-        sp = 0;
-        steps_left = budget;
-        node = root;
+        if (sgrid) {
+          const SunGridParams* gq = a.sg.params;
+          const float X = comp(p, gq->kx) - gq->sx * comp(p, gq->kz);
+          const float Y = comp(p, gq->ky) - gq->sy * comp(p, gq->kz);
+          const int64_t cell = (int64_t)sun_cell(Y, gq->y0, gq->ich, gq->gy) * gq->gx +
+                               sun_cell(X, gq->x0, gq->icw, gq->gx);
+          sp = (int32_t)a.sg.start[cell];
+          steps_left = a.sg.start[cell + 1];
+          node = (uint32_t)sp < steps_left ? kGridRay : kDone;
+          if (COUNT) ++sn;  // one cell lookup
+        } else {
+          sp = 0;
+          steps_left = budget;
+          node = root;
+        }
       } else {
         if (!RAYS) a.shadow[out] = sph ? (uint8_t)(bid >= 0 ? 1 : 0) : (uint8_t)255;
         ray = -1;
@@ -374,8 +392,20 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
 #pragma unroll
     for (int u = 0; u < ANE_TRACE_UNROLL; ++u) {
     const bool go = ray >= 0 && node != kDone &&
-                    (kLeafBatch == 0 || (node != kWait && (SPEC ? !(node < 0 && pend != 0) : pend == 0)));
-    if (go) {
+                    (kLeafBatch == 0 || (node != kWait && ((SPEC && node != kGridRay) ? !(node < 0 && pend != 0)
+                                                                                         : pend == 0)));
+    if (!RAYS && go && node == kGridRay) {
+      // the next candidate of the shadow ray's cell waits for the batch
+      const uint32_t pos = a.sg.list[(uint32_t)sp];
+      ++sp;
+      if (pos >= (uint32_t)sc.ntri) {
+        *a.overflow = 1u;
+        node = kDone;
+      } else {
+        pend = ~(int32_t)pos;
+        node = (uint32_t)sp < steps_left ? kGridRay : kWait;
+      }
+    } else if (go) {
       int32_t next = kPop;
       // guards: a malformed hierarchy ends the ray and raises the error flag
       // instead of reading out of bounds or looping forever
@@ -454,7 +484,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
       const uint64_t pm = __ballot(pend != 0);
       if (pm != 0ull) {
         const bool can = ray >= 0 && node != kDone && node != kWait &&
-                         (SPEC ? !(node < 0 && pend != 0) : pend == 0);
+                         ((SPEC && node != kGridRay) ? !(node < 0 && pend != 0) : pend == 0);
         if (__popcll(pm) >= kLeafBatch || __ballot(can) == 0ull) {
           if (pend != 0) {
             if (COUNT) {
