@@ -608,6 +608,11 @@ struct ane_ddgi_ctx {
   uint64_t sg_version = 0;
   uint32_t sg_dir[3] = {0, 0, 0};
   bool sg_built = false;
+  // the key of the last update that found no current grid (built on its second sight)
+  const ane_scene* sg_req_scene = nullptr;
+  uint64_t sg_req_version = 0;
+  uint32_t sg_req_dir[3] = {0, 0, 0};
+  bool sg_req = false;
   ane_ddgi_stats stats{};
   int64_t bytes = 0;
   hipGraphExec_t gexec = nullptr;  // the last captured update (request loop)
@@ -842,16 +847,30 @@ static int sun_grid_alloc(ane_ddgi_ctx* c, const ane_scene* s)
 
 static bool sun_grid_wanted(const ane_ddgi_ctx* c) { return c->counting != 1 && trace_grid_supported(); }
 
-// (Re)builds the sun grid on `st` unless the one for this scene, scene version
-// and sun direction is current.
-static int sun_grid_update(ane_ddgi_ctx* c, const ane_scene* s, const float* sun3, hipStream_t st)
+// Sets *use when a sun grid for this scene, scene version and sun direction is
+// current after the call.  A key seen for the first time is only remembered: a
+// sun or a scene that changes with every update keeps the traversal and never
+// pays a build; the second update with the same key builds the grid on `st`.
+static int sun_grid_update(ane_ddgi_ctx* c, const ane_scene* s, const float* sun3, hipStream_t st, bool* use)
 {
-  ANE_TRY(sun_grid_alloc(c, s));
+  *use = false;
   uint32_t bits[3];
   std::memcpy(bits, sun3, sizeof(bits));
-  if (c->sg_built && c->sg_scene == s && c->sg_version == s->version && bits[0] == c->sg_dir[0] &&
-      bits[1] == c->sg_dir[1] && bits[2] == c->sg_dir[2])
+  auto same = [&](const ane_scene* ks, uint64_t kv, const uint32_t* kd) {
+    return ks == s && kv == s->version && kd[0] == bits[0] && kd[1] == bits[1] && kd[2] == bits[2];
+  };
+  if (c->sg_built && same(c->sg_scene, c->sg_version, c->sg_dir)) {
+    *use = true;
     return ANE_OK;
+  }
+  if (!(c->sg_req && same(c->sg_req_scene, c->sg_req_version, c->sg_req_dir))) {
+    c->sg_req = true;
+    c->sg_req_scene = s;
+    c->sg_req_version = s->version;
+    std::memcpy(c->sg_req_dir, bits, sizeof(bits));
+    return ANE_OK;
+  }
+  ANE_TRY(sun_grid_alloc(c, s));
   SunGridParams g;
   sun_grid_setup(g, sun3, c->sg_g, c->sg_g);
   launch_sun_grid_build(s->view, g, c->d_sg_params, c->d_sg_acc, c->d_sg_start, c->d_sg_cur, c->d_sg_list,
@@ -861,6 +880,7 @@ static int sun_grid_update(ane_ddgi_ctx* c, const ane_scene* s, const float* sun
   c->sg_scene = s;
   c->sg_version = s->version;
   std::memcpy(c->sg_dir, bits, sizeof(bits));
+  *use = true;
   return ANE_OK;
 }
 
@@ -916,10 +936,13 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   TraceArgs a = trace_args(c, s, fk);
   a.work = c->d_work;
   if (sun_grid_wanted(c)) {
-    ANE_TRY(sun_grid_update(c, s, a.sun, st));
-    a.sg.params = c->d_sg_params;
-    a.sg.start = c->d_sg_start;
-    a.sg.list = c->d_sg_list;
+    bool use = false;
+    ANE_TRY(sun_grid_update(c, s, a.sun, st, &use));
+    if (use) {
+      a.sg.params = c->d_sg_params;
+      a.sg.start = c->d_sg_start;
+      a.sg.list = c->d_sg_list;
+    }
   }
   // primary and shadow rays in one persistent launch (the trace stage)
   ANE_TRY(c->timers[kStTrace].begin(st));

@@ -114,28 +114,61 @@ __global__ void k_sun_params(SunGridParams g, const uint32_t* acc, SunGridParams
 
 struct CellRange {
   int cx0, cx1, cy0, cy1;
+  float px[3], py[3];  // projected vertices
 };
 
 // cells of triangle p's widened projected rectangle (inclusive)
 __device__ __forceinline__ CellRange sun_tri_cells(const SunGridParams& g, const float4* tris, int64_t p)
 {
+  CellRange c;
   float xl = 0.0f, xh = 0.0f, yl = 0.0f, yh = 0.0f;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const float4 v = tris[3 * p + j];
     float X, Y;
     sun_project(g, mk(v.x, v.y, v.z), X, Y);
+    c.px[j] = X;
+    c.py[j] = Y;
     xl = j ? fminf(xl, X) : X;
     xh = j ? fmaxf(xh, X) : X;
     yl = j ? fminf(yl, Y) : Y;
     yh = j ? fmaxf(yh, Y) : Y;
   }
-  CellRange c;
   c.cx0 = sun_cell(xl - g.margin, g.x0, g.icw, g.gx);
   c.cx1 = sun_cell(xh + g.margin, g.x0, g.icw, g.gx);
   c.cy0 = sun_cell(yl - g.margin, g.y0, g.ich, g.gy);
   c.cy1 = sun_cell(yh + g.margin, g.y0, g.ich, g.gy);
   return c;
+}
+
+// Whether the projected triangle can come within the margin of cell (x, y):
+// separating axes are the triangle's edge normals (the rectangle's own axes hold
+// by the cell range).  The cell rectangle is widened by the margin, and border
+// cells reach to infinity on their outer side (points outside the grid clamp to
+// them); the tolerance margin * (|ex| + |ey|) exceeds the rounding of the dot
+// products.  Degenerate edges separate nothing.
+__device__ __forceinline__ bool sun_tri_overlaps_cell(const SunGridParams& g, const CellRange& c, int x, int y)
+{
+  const float big = 3.0e38f;
+  const float cw = 1.0f / g.icw, ch = 1.0f / g.ich;
+  const float rx0 = x == 0 ? -big : g.x0 + (float)x * cw - g.margin;
+  const float rx1 = x == g.gx - 1 ? big : g.x0 + (float)(x + 1) * cw + g.margin;
+  const float ry0 = y == 0 ? -big : g.y0 + (float)y * ch - g.margin;
+  const float ry1 = y == g.gy - 1 ? big : g.y0 + (float)(y + 1) * ch + g.margin;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int k = j == 2 ? 0 : j + 1, o = j == 0 ? 2 : j - 1;
+    const float ex = c.px[k] - c.px[j], ey = c.py[k] - c.py[j];
+    const float nx = -ey, ny = ex;
+    const float side = nx * (c.px[o] - c.px[j]) + ny * (c.py[o] - c.py[j]);
+    if (!(side != 0.0f)) continue;  // degenerate (or NaN): no separation
+    const float s = side > 0.0f ? 1.0f : -1.0f;
+    // the rectangle corner furthest toward the triangle's side of this edge
+    const float cx = (s * nx > 0.0f) ? rx1 : rx0, cy = (s * ny > 0.0f) ? ry1 : ry0;
+    const float reach = s * (nx * (cx - c.px[j]) + ny * (cy - c.py[j]));
+    if (reach < -g.margin * (fabsf(ex) + fabsf(ey))) return false;
+  }
+  return true;
 }
 
 __global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t ntri, SunGridParams* gp,
@@ -151,7 +184,8 @@ __global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t n
     return;
   }
   for (int y = c.cy0; y <= c.cy1; ++y)
-    for (int x = c.cx0; x <= c.cx1; ++x) atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
+    for (int x = c.cx0; x <= c.cx1; ++x)
+      if (sun_tri_overlaps_cell(g, c, x, y)) atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
 }
 
 // after the scan: start[g.gx * g.gy] entries in total; the cursors begin at the starts
@@ -171,10 +205,11 @@ __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t
   if (p >= ntri || !g.valid) return;
   const CellRange c = sun_tri_cells(g, tris, p);
   for (int y = c.cy0; y <= c.cy1; ++y)
-    for (int x = c.cx0; x <= c.cx1; ++x) {
-      const uint32_t k = atomicAdd(&cur[(int64_t)y * g.gx + x], 1u);
-      if (k < cap) list[k] = (uint32_t)p;
-    }
+    for (int x = c.cx0; x <= c.cx1; ++x)
+      if (sun_tri_overlaps_cell(g, c, x, y)) {
+        const uint32_t k = atomicAdd(&cur[(int64_t)y * g.gx + x], 1u);
+        if (k < cap) list[k] = (uint32_t)p;
+      }
 }
 
 void sun_grid_setup(SunGridParams& g, const float* dir3, int gx, int gy)

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from anerend_amd import native, scenegen
+from anerend_amd.types import Config
 from oracle import oracle as O
 from tests.conftest import bits, rel_err
 
@@ -72,3 +73,37 @@ def test_room_updates_parity(room):
         print(f"  radiance rel err {e_rad:.3e}  atlas rel err {e_atl:.3e}", flush=True)
         assert e_rad <= RTOL
         assert e_atl <= RTOL
+
+
+@pytest.mark.parametrize("family", ["room", "terrain"])
+def test_sun_grid_shadow_rays_match_oracle(family):
+    """Probe shadow rays through the sun grid: the grid of a scene and sun is built
+    on the second update that sees them, so updates 0 and 3 (a new sun) traverse
+    the hierarchy and updates 1, 2 and 4 test the triangles of the sun-grid cells.
+    Hits, shadow decisions, radiance and atlas equal the oracle's at every update."""
+    if family == "room":
+        scene, cfg, _ = scenegen.workload("room8")
+        cam = (0.5, 1.5, 0.5)
+    else:
+        scene = scenegen.terrain_scene(9, cells=48, num_instances=200, spread=18.0)
+        cfg = Config.make((8, 4, 8), step=(1, 2, 1), rays=256)
+        cam = (0.5, 9.0, 0.5)
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    lit = shadowed = 0
+    for it in range(5):
+        fr = scenegen.make_frame(6, camera=cam, config=cfg, iteration=it)
+        if it >= 3:
+            fr.light_dir[0] += 0.25   # a new sun direction from update 3 on
+        print(f"sun grid {family} update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), f"hit ids differ at update {it}"
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.radiance(fr), rado) <= RTOL
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        lit += int((sho == 0).sum())
+        shadowed += int((sho == 1).sum())
+    assert lit > 0 and shadowed > 0
