@@ -597,7 +597,7 @@ struct ane_ddgi_ctx {
   int counting = 0;  // 1: counting kernels, shadow rays traverse; 2: counting kernels, sun grid
   // sun grid of the probe shadow rays, rebuilt when the scene or the sun changes
   SunGridParams* d_sg_params = nullptr;
-  uint32_t* d_sg_acc = nullptr;   // [6] bounds reduction
+  uint32_t* d_sg_acc = nullptr;   // [6][256] per-block bounds
   uint32_t* d_sg_start = nullptr; // [cells + 1] counts, scanned into starts
   uint32_t* d_sg_cur = nullptr;   // [cells] scatter cursors
   uint32_t* d_sg_list = nullptr;  // [cap] candidates
@@ -835,7 +835,7 @@ static int sun_grid_alloc(ane_ddgi_ctx* c, const ane_scene* s)
   c->sg_built = false;
   const int64_t ncell = (int64_t)g * g;
   ANE_TRY(dalloc(&c->d_sg_params, 1, nullptr));
-  ANE_TRY(dalloc(&c->d_sg_acc, 8, nullptr));
+  ANE_TRY(dalloc(&c->d_sg_acc, 6 * 256, nullptr));  // per-block bounds of k_sun_bounds
   ANE_TRY(dalloc(&c->d_sg_start, ncell + 1, nullptr));
   ANE_TRY(dalloc(&c->d_sg_cur, ncell, nullptr));
   ANE_TRY(dalloc(&c->d_sg_list, (int64_t)cap, nullptr));

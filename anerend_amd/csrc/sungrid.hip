@@ -44,9 +44,12 @@ __device__ __forceinline__ void sun_project(const SunGridParams& g, f3 p, float&
   Y = comp(p, g.ky) - g.sy * comp(p, g.kz);
 }
 
-// acc: [0] min X, [1] min Y, [2] max X, [3] max Y, [4] max |coordinate|, [5] non-finite seen
+static constexpr int kBoundBlocks = 256;
+
+// per block, [6][kBoundBlocks]: min X, min Y, max X, max Y, max |coordinate|,
+// non-finite seen (ordered encodings); k_sun_params reduces the blocks
 __global__ __launch_bounds__(256) void k_sun_bounds(const float4* tris, int32_t ntri, SunGridParams g,
-                                                    uint32_t* acc)
+                                                    uint32_t* part)
 {
   __shared__ uint32_t red[6][256];
   uint32_t mnx = 0xFFFFFFFFu, mny = 0xFFFFFFFFu, mxx = 0u, mxy = 0u, mr = 0u, bad = 0u;
@@ -80,22 +83,29 @@ __global__ __launch_bounds__(256) void k_sun_bounds(const float4* tris, int32_t 
     }
     __syncthreads();
   }
-  if (t == 0) {
-    atomicMin(&acc[0], red[0][0]);
-    atomicMin(&acc[1], red[1][0]);
-    atomicMax(&acc[2], red[2][0]);
-    atomicMax(&acc[3], red[3][0]);
-    atomicMax(&acc[4], red[4][0]);
-    atomicMax(&acc[5], red[5][0]);
-  }
+  if (t < 6) part[t * kBoundBlocks + blockIdx.x] = red[t][0];
 }
 
 // placement: margin 2e-5 of the largest coordinate magnitude (the rounding of
 // the projection and of the watertight test is below 1e-6 of it); cells of the
 // widened bounds
-__global__ void k_sun_params(SunGridParams g, const uint32_t* acc, SunGridParams* out)
+__global__ __launch_bounds__(kBoundBlocks) void k_sun_params(SunGridParams g, const uint32_t* part,
+                                                            SunGridParams* out)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ uint32_t red[6][kBoundBlocks];
+  const int t = threadIdx.x;
+  for (int k = 0; k < 6; ++k) red[k][t] = part[k * kBoundBlocks + t];
+  __syncthreads();
+  for (int s = kBoundBlocks / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = min(red[0][t], red[0][t + s]);
+      red[1][t] = min(red[1][t], red[1][t + s]);
+      for (int k = 2; k < 6; ++k) red[k][t] = max(red[k][t], red[k][t + s]);
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const uint32_t acc[6] = {red[0][0], red[1][0], red[2][0], red[3][0], red[4][0], red[5][0]};
   SunGridParams r = g;
   const bool empty = acc[0] == 0xFFFFFFFFu;
   const float R = empty ? 0.0f : ord2f(acc[4]);
@@ -233,15 +243,9 @@ void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridPara
                            hipStream_t s)
 {
   const int64_t ncell = (int64_t)g.gx * g.gy;
-  fill_u32(acc, 0xFFFFFFFFu, 2, s);
-  fill_u32(acc + 2, 0u, 4, s);
   fill_u32(cnt, 0u, ncell + 1, s);
-  const int64_t nv = (int64_t)sc.ntri * 3;
-  int blocks = (int)((nv + 255) / 256);
-  if (blocks > 2048) blocks = 2048;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_sun_bounds, dim3(blocks), dim3(256), 0, s, sc.tris, sc.ntri, g, acc);
-  hipLaunchKernelGGL(k_sun_params, dim3(1), dim3(64), 0, s, g, (const uint32_t*)acc, d_params);
+  hipLaunchKernelGGL(k_sun_bounds, dim3(kBoundBlocks), dim3(256), 0, s, sc.tris, sc.ntri, g, acc);
+  hipLaunchKernelGGL(k_sun_params, dim3(1), dim3(kBoundBlocks), 0, s, g, (const uint32_t*)acc, d_params);
   const unsigned tb = (unsigned)((sc.ntri + 255) / 256);
   if (sc.ntri > 0)
     hipLaunchKernelGGL(k_sun_count, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri, d_params, cnt);

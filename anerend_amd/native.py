@@ -363,8 +363,9 @@ class ProbeGrid:
         check(load().ane_ddgi_debug_rays(self._h, ptr(hits), ptr(sh), None))
         return hits, sh
 
-    def set_counting(self, on=True):
-        check(load().ane_ddgi_set_counting(self._h, int(on)))
+    def set_counting(self, mode=1):
+        """0 off; 1 (or True) every ray traverses; 2 shadow rays take the sun grid when current."""
+        check(load().ane_ddgi_set_counting(self._h, int(mode)))
 
     def stats(self):
         s = Stats()

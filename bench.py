@@ -595,13 +595,21 @@ def main():
         return scenegen.make_frame(2, camera=tuple(frame.camera_pos[:3]),
                                    config=cfg, iteration=i)
 
-    # traversal work counts (separate, counting kernels; outside the timed region)
-    grid.set_counting(True)
+    # work counts (counting kernels, outside the timed region).  Mode 1: every ray
+    # traverses the hierarchy (the counts the oracle reproduces).  Mode 2: the
+    # update's own shadow path; its second update with this scene and sun builds
+    # the sun grid, after which shadow rays count cell lookups and candidate tests.
+    grid.set_counting(1)
     grid.update(ds, frame_at(0))
     work = grid.stats()
-    grid.set_counting(False)
+    grid.set_counting(2)
+    grid.update(ds, frame_at(0))
+    grid.update(ds, frame_at(0))
+    gwork = grid.stats()
+    grid.set_counting(0)
     log(f"[bench] work: primary nodes {work['primary_nodes']} tris {work['primary_tris']}, "
         f"shadow nodes {work['shadow_nodes']} tris {work['shadow_tris']}")
+    log(f"[bench] sun grid: shadow-ray cell lookups {gwork['shadow_nodes']}, candidate tests {gwork['shadow_tris']}")
 
     for i in range(args.warmup):
         grid.update(ds, frame_at(1 + i))
@@ -632,10 +640,12 @@ def main():
     # roofline of the dominant kernel (algorithmic bytes per launch / mean launch time)
     rays = P * N
     # the trace stage is one launch: primary rays and, in the same lanes, the
-    # shadow ray of every hit; per ray 16 B hit record + 1 B shadow flag out
+    # shadow ray of every hit against the triangles of its sun-grid cell (8 B of
+    # cell bounds per lookup, a 4 B list entry and the 48 B triangle per candidate);
+    # per ray 16 B hit record + 1 B shadow flag out
     algo = {
-        "trace": (work["primary_nodes"] + work["shadow_nodes"]) * NODE_BYTES
-                 + (work["primary_tris"] + work["shadow_tris"]) * TRI_BYTES + rays * (HIT_BYTES + 1),
+        "trace": work["primary_nodes"] * NODE_BYTES + work["primary_tris"] * TRI_BYTES
+                 + gwork["shadow_nodes"] * 8 + gwork["shadow_tris"] * (4 + TRI_BYTES) + rays * (HIT_BYTES + 1),
     }
     dom = max(("trace", "shade", "convolve"), key=lambda k: mean[k])
     roof = None
@@ -690,6 +700,11 @@ def main():
                    "lbvh_build_ms": round(info["build_ms"], 3),
                    "atlas_bytes": atlas_bytes},
         "stages_ms": {k: round(v, 4) for k, v in mean.items() if k not in ("translate", "shadow")},
+        "work": {"primary_node_visits": work["primary_nodes"], "primary_triangle_tests": work["primary_tris"],
+                 "shadow_rays_traversing": {"node_visits": work["shadow_nodes"],
+                                            "triangle_tests": work["shadow_tris"]},
+                 "shadow_rays_sun_grid": {"cell_lookups": gwork["shadow_nodes"],
+                                          "candidate_tests": gwork["shadow_tris"]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "conv_study": conv,

@@ -203,7 +203,13 @@ int ane_ddgi_destroy(ane_ddgi_ctx* ctx);
 int ane_ddgi_translate(ane_ddgi_ctx* ctx, const float* camera_pos3, int32_t* shifted,
                        void* stream);
 /* IrradianceProbeRT: trace probe rays of layers [layer_begin, +layer_count),
- * shade hits (shadow ray, direct + indirect + emissive) and store radiance. */
+ * shade hits (shadow ray, direct + indirect + emissive) and store radiance.
+ * Shadow rays all point at the sun: from the second trace that sees the same
+ * scene (no refit or rebuild in between) and the same sun direction, the context
+ * keeps a grid of the triangles projected along the sun and a shadow ray tests
+ * the triangles of its cell instead of traversing; the decisions are identical.
+ * The grid takes about 28 bytes per triangle plus 4 per cell (one cell per
+ * triangle) and is rebuilt on the stream when that scene or sun recurs. */
 int ane_ddgi_trace(ane_ddgi_ctx* ctx, const ane_scene* scene, const ane_ddgi_frame* frame,
                    void* stream);
 /* IrradianceProbeConvolve: cosine-weighted octahedral convolution of the stored
@@ -265,8 +271,11 @@ int ane_ddgi_radiance_get(ane_ddgi_ctx* ctx, float* host, void* stream);
 /* ---- parity / diagnostics ---- */
 /* Per-ray hits [probe][ray] and shadow flags (0 lit, 1 shadowed, 255 miss) of the last trace. */
 int ane_ddgi_debug_rays(ane_ddgi_ctx* ctx, ane_hit* hits, uint8_t* shadowed, void* stream);
-/* Count traversal work (nodes/triangles) in the next traces (slower kernels). */
-int ane_ddgi_set_counting(ane_ddgi_ctx* ctx, int32_t enable);
+/* Count traversal work in the next traces (slower kernels).  mode 0: off;
+ * 1: every ray traverses the hierarchy (node visits, triangle tests); 2: the
+ * shadow rays take the sun grid when it is current, and their counters hold
+ * cell lookups and candidate triangle tests instead. */
+int ane_ddgi_set_counting(ane_ddgi_ctx* ctx, int32_t mode);
 int ane_ddgi_stats_get(const ane_ddgi_ctx* ctx, ane_ddgi_stats* out);
 int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
 /* LBVH export in sorted order: keys, triangle ids, internal children (>= 0
