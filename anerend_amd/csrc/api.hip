@@ -211,6 +211,100 @@ enum { kStSetup = 0, kStTrace, kStShadow, kStShade, kStConv, kStTrans, kStages }
 using namespace ane;
 
 // ===========================================================================
+// sun grids (sungrid.hip): the triangles of a scene binned along one direction,
+// for any-hit rays that all share it, and the key they were built for.  A key
+// is first only remembered; the second request with it builds the grid on the
+// request's stream, so a direction or scene that changes with every call keeps
+// the traversal and never pays a build.
+// ===========================================================================
+#ifndef ANE_SUNGRID_DENSITY
+#define ANE_SUNGRID_DENSITY 1.0  // cells per triangle
+#endif
+#ifndef ANE_SUNGRID_CAPACITY
+#define ANE_SUNGRID_CAPACITY 16  // entries per triangle before the grid falls back to traversal
+#endif
+
+struct SunGridCache {
+  SunGridParams* params = nullptr;
+  uint32_t* acc = nullptr;    // [6][256] per-block bounds
+  uint32_t* start = nullptr;  // [cells + 1] counts, scanned into starts
+  uint32_t* cur = nullptr;    // [cells] scatter cursors
+  uint32_t* list = nullptr;   // [cap] candidates
+  uint32_t* scan = nullptr;   // scan scratch
+  int g = 0;
+  uint32_t cap = 0;
+  struct Key {
+    const void* scene = nullptr;
+    uint64_t version = 0;
+    uint32_t dir[3] = {0, 0, 0};
+    bool set = false;
+    bool operator==(const Key& o) const
+    {
+      return set && o.set && scene == o.scene && version == o.version && dir[0] == o.dir[0] &&
+             dir[1] == o.dir[1] && dir[2] == o.dir[2];
+    }
+  } built, seen;
+
+  void release()
+  {
+    dfree(params); dfree(acc); dfree(start); dfree(cur); dfree(list); dfree(scan);
+    g = 0;
+    cap = 0;
+    built = Key{};
+  }
+  // device buffers for a scene of ntri triangles: about ANE_SUNGRID_DENSITY cells
+  // per triangle (at most 4096 x 4096), ANE_SUNGRID_CAPACITY entries per triangle
+  int alloc(int32_t ntri)
+  {
+    int gg = (int)std::ceil(std::sqrt((double)std::max(ntri, 1) * ANE_SUNGRID_DENSITY));
+    gg = std::min(std::max(gg, 16), 4096);
+    const int64_t capl = std::min<int64_t>((int64_t)ANE_SUNGRID_CAPACITY * ntri + 1024, (int64_t)1 << 30);
+    if (gg <= g && capl <= (int64_t)cap) return ANE_OK;
+    release();
+    const int64_t ncell = (int64_t)gg * gg;
+    ANE_TRY(dalloc(&params, 1, nullptr));
+    ANE_TRY(dalloc(&acc, 6 * 256, nullptr));
+    ANE_TRY(dalloc(&start, ncell + 1, nullptr));
+    ANE_TRY(dalloc(&cur, ncell, nullptr));
+    ANE_TRY(dalloc(&list, capl, nullptr));
+    ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, nullptr));
+    g = gg;
+    cap = (uint32_t)capl;
+    return ANE_OK;
+  }
+  // *use: the grid for (scene, version, dir) is current after the call
+  int update(const SceneK& view, int32_t ntri, const void* scene, uint64_t version, const float* dir3,
+             hipStream_t st, bool* use)
+  {
+    *use = false;
+    Key k;
+    k.scene = scene;
+    k.version = version;
+    std::memcpy(k.dir, dir3, sizeof(k.dir));
+    k.set = true;
+    if (built == k) {
+      *use = true;
+      return ANE_OK;
+    }
+    if (!(seen == k)) {
+      seen = k;
+      return ANE_OK;
+    }
+    ANE_TRY(alloc(ntri));
+    SunGridParams p;
+    sun_grid_setup(p, dir3, g, g);
+    launch_sun_grid_build(view, p, params, acc, start, cur, list, cap, scan, st);
+    ANE_CHECK_HIP(hipGetLastError());
+    built = k;
+    *use = true;
+    return ANE_OK;
+  }
+  SunGridK view() const { return SunGridK{params, start, list}; }
+  // device address of the validity flag (the traversal launches skip when it is set)
+  const uint32_t* valid_flag() const { return &params->valid; }
+};
+
+// ===========================================================================
 // scene
 // ===========================================================================
 struct ane_scene {
@@ -258,6 +352,7 @@ struct ane_scene {
   int up_slot = 0;
   int64_t bytes = 0;
   SceneK view{};
+  SunGridCache sgc;  // the shadow mask's rays along -light_dir
   // shadow-mask workspace (grows to the largest image seen)
   int64_t shadow_px = 0;
   float* d_sh_orig = nullptr;
@@ -280,6 +375,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
+  s->sgc.release();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (int k = 0; k < 2; ++k) {
@@ -595,24 +691,7 @@ struct ane_ddgi_ctx {
   bool rad_bake = false;
   int32_t last_begin = 0, last_count = 0;
   int counting = 0;  // 1: counting kernels, shadow rays traverse; 2: counting kernels, sun grid
-  // sun grid of the probe shadow rays, rebuilt when the scene or the sun changes
-  SunGridParams* d_sg_params = nullptr;
-  uint32_t* d_sg_acc = nullptr;   // [6][256] per-block bounds
-  uint32_t* d_sg_start = nullptr; // [cells + 1] counts, scanned into starts
-  uint32_t* d_sg_cur = nullptr;   // [cells] scatter cursors
-  uint32_t* d_sg_list = nullptr;  // [cap] candidates
-  uint32_t* d_sg_scan = nullptr;  // scan scratch
-  int sg_g = 0;
-  uint32_t sg_cap = 0;
-  const ane_scene* sg_scene = nullptr;
-  uint64_t sg_version = 0;
-  uint32_t sg_dir[3] = {0, 0, 0};
-  bool sg_built = false;
-  // the key of the last update that found no current grid (built on its second sight)
-  const ane_scene* sg_req_scene = nullptr;
-  uint64_t sg_req_version = 0;
-  uint32_t sg_req_dir[3] = {0, 0, 0};
-  bool sg_req = false;
+  SunGridCache sgc;  // probe and mirror shadow rays (normalize(-light_dir))
   ane_ddgi_stats stats{};
   int64_t bytes = 0;
   hipGraphExec_t gexec = nullptr;  // the last captured update (request loop)
@@ -636,8 +715,7 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
   dfree(c->d_tiles);
-  dfree(c->d_sg_params); dfree(c->d_sg_acc); dfree(c->d_sg_start); dfree(c->d_sg_cur); dfree(c->d_sg_list);
-  dfree(c->d_sg_scan);
+  c->sgc.release();
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
   dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
@@ -819,70 +897,7 @@ static int ensure_spill(ane_ddgi_ctx* c, int32_t depth)
   return ANE_OK;
 }
 
-// Sun-grid sizes for a scene: about one cell per triangle, at most 4096 x 4096;
-// capacity 16 entries per triangle (the grid falls back to traversal beyond it).
-static int sun_grid_alloc(ane_ddgi_ctx* c, const ane_scene* s)
-{
-  int g = (int)std::ceil(std::sqrt((double)std::max(s->ntri, 1)));
-  g = std::min(std::max(g, 16), 4096);
-  const int64_t capl = std::min<int64_t>((int64_t)16 * s->ntri + 1024, (int64_t)1 << 30);
-  const uint32_t cap = (uint32_t)capl;
-  if (g <= c->sg_g && cap <= c->sg_cap) return ANE_OK;
-  dfree(c->d_sg_params); dfree(c->d_sg_acc); dfree(c->d_sg_start); dfree(c->d_sg_cur); dfree(c->d_sg_list);
-  dfree(c->d_sg_scan);
-  c->sg_g = 0;
-  c->sg_cap = 0;
-  c->sg_built = false;
-  const int64_t ncell = (int64_t)g * g;
-  ANE_TRY(dalloc(&c->d_sg_params, 1, nullptr));
-  ANE_TRY(dalloc(&c->d_sg_acc, 6 * 256, nullptr));  // per-block bounds of k_sun_bounds
-  ANE_TRY(dalloc(&c->d_sg_start, ncell + 1, nullptr));
-  ANE_TRY(dalloc(&c->d_sg_cur, ncell, nullptr));
-  ANE_TRY(dalloc(&c->d_sg_list, (int64_t)cap, nullptr));
-  ANE_TRY(dalloc(&c->d_sg_scan, (ncell + 1) / 1024 + 256, nullptr));
-  c->sg_g = g;
-  c->sg_cap = cap;
-  return ANE_OK;
-}
-
 static bool sun_grid_wanted(const ane_ddgi_ctx* c) { return c->counting != 1 && trace_grid_supported(); }
-
-// Sets *use when a sun grid for this scene, scene version and sun direction is
-// current after the call.  A key seen for the first time is only remembered: a
-// sun or a scene that changes with every update keeps the traversal and never
-// pays a build; the second update with the same key builds the grid on `st`.
-static int sun_grid_update(ane_ddgi_ctx* c, const ane_scene* s, const float* sun3, hipStream_t st, bool* use)
-{
-  *use = false;
-  uint32_t bits[3];
-  std::memcpy(bits, sun3, sizeof(bits));
-  auto same = [&](const ane_scene* ks, uint64_t kv, const uint32_t* kd) {
-    return ks == s && kv == s->version && kd[0] == bits[0] && kd[1] == bits[1] && kd[2] == bits[2];
-  };
-  if (c->sg_built && same(c->sg_scene, c->sg_version, c->sg_dir)) {
-    *use = true;
-    return ANE_OK;
-  }
-  if (!(c->sg_req && same(c->sg_req_scene, c->sg_req_version, c->sg_req_dir))) {
-    c->sg_req = true;
-    c->sg_req_scene = s;
-    c->sg_req_version = s->version;
-    std::memcpy(c->sg_req_dir, bits, sizeof(bits));
-    return ANE_OK;
-  }
-  ANE_TRY(sun_grid_alloc(c, s));
-  SunGridParams g;
-  sun_grid_setup(g, sun3, c->sg_g, c->sg_g);
-  launch_sun_grid_build(s->view, g, c->d_sg_params, c->d_sg_acc, c->d_sg_start, c->d_sg_cur, c->d_sg_list,
-                        c->sg_cap, c->d_sg_scan, st);
-  ANE_CHECK_HIP(hipGetLastError());
-  c->sg_built = true;
-  c->sg_scene = s;
-  c->sg_version = s->version;
-  std::memcpy(c->sg_dir, bits, sizeof(bits));
-  *use = true;
-  return ANE_OK;
-}
 
 static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& fk)
 {
@@ -937,12 +952,8 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   a.work = c->d_work;
   if (sun_grid_wanted(c)) {
     bool use = false;
-    ANE_TRY(sun_grid_update(c, s, a.sun, st, &use));
-    if (use) {
-      a.sg.params = c->d_sg_params;
-      a.sg.start = c->d_sg_start;
-      a.sg.list = c->d_sg_list;
-    }
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use));
+    if (use) a.sg = c->sgc.view();
   }
   // primary and shadow rays in one persistent launch (the trace stage)
   ANE_TRY(c->timers[kStTrace].begin(st));
@@ -1007,7 +1018,7 @@ int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame*
   FrameK fk;
   ANE_TRY(frame_to_k(c, f, &fk));
   ANE_TRY(ensure_spill(c, s->depth));
-  if (sun_grid_wanted(c)) ANE_TRY(sun_grid_alloc(c, s));  // allocation outside the capture
+  if (sun_grid_wanted(c)) ANE_TRY(c->sgc.alloc(s->ntri));  // allocation outside the capture
   if (c->counting) return ane_ddgi_update(c, s, f, st);
   for (auto& t : c->timers) t.off = true;
   hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
@@ -1174,6 +1185,17 @@ extern "C" int ane_ddgi_mirror_reflections(ane_ddgi_ctx* c, const ane_scene* s, 
   const f3 sun = normalize(mk(-fk.light_dir[0], -fk.light_dir[1], -fk.light_dir[2]));
   const float sun3[3] = {sun.x, sun.y, sun.z};
   launch_mirror_shadow_rays(s->view, c->d_mr_hits[0], n, sun3, c->d_mr_orig[1], c->d_mr_dir[1], st);
+  const uint32_t* skip = nullptr;
+  if (!counters4 && sun_grid_wanted(c)) {
+    bool use = false;
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, sun3, st, &use));
+    if (use) {
+      launch_anyhit_grid(c->d_mr_orig[1], c->d_mr_dir[1], n, c->sgc.view(), s->view, 0.1f, 50.0f, c->d_mr_hits[1],
+                         st);
+      skip = c->sgc.valid_flag();
+    }
+  }
+  a.skip_if_set = skip;
   a.hits = c->d_mr_hits[1];
   a.work = c->d_mr_work + 1;
   a.stats = c->d_mr_stats + 2;
@@ -1371,7 +1393,19 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
     ANE_TRY(dalloc(&s->d_sh_spill, sd * lanes, nullptr));
     s->sh_spill_words = sd * lanes;
   }
-  launch_shadow_rays(depth, width, height, ivp16, light3, s->d_sh_orig, s->d_sh_dir, st);
+  // without counters the rays go through the scene's grid along -light_dir once it
+  // is current; the traversal launches below then do nothing (device flag)
+  const uint32_t* skip = nullptr;
+  if (!counters2) {
+    const float dir[3] = {-light3[0], -light3[1], -light3[2]};
+    bool use = false;
+    ANE_TRY(s->sgc.update(s->view, s->ntri, s, s->version, dir, st, &use));
+    if (use) {
+      launch_shadow_mask_grid(depth, width, height, ivp16, light3, s->sgc.view(), s->view, out_vis, st);
+      skip = s->sgc.valid_flag();
+    }
+  }
+  launch_shadow_rays(depth, width, height, ivp16, light3, s->d_sh_orig, s->d_sh_dir, st, skip);
   fill_u32(s->d_sh_work, 0u, 1, st);
   fill_u32(reinterpret_cast<uint32_t*>(s->d_sh_stats), 0u, 4, st);
   fill_u32(s->d_sh_ov, 0u, 1, st);
@@ -1389,8 +1423,9 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
   a.ray_dir = s->d_sh_dir;
   a.t_min = 0.1f;  // shadow.rgen:66-67
   a.t_max = 50.0f;
+  a.skip_if_set = skip;
   launch_trace(a, true, true, counters2 != nullptr, st);
-  launch_shadow_resolve(depth, s->d_sh_hits, n, out_vis, st);
+  launch_shadow_resolve(depth, s->d_sh_hits, n, out_vis, st, skip);
   ANE_CHECK_HIP(hipGetLastError());
   if (counters2) {
     unsigned long long c2[2] = {0, 0};

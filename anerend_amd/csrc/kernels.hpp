@@ -53,6 +53,38 @@ __host__ __device__ __forceinline__ int sun_cell(float v, float o, float inv, in
   return (int)f;
 }
 void sun_grid_setup(SunGridParams& g, const float* dir3, int gx, int gy);
+
+// Any hit of the ray r (origin o, the grid's direction) among the triangles of
+// its cell: the watertight test in list order, the first hit ends it.
+__device__ __forceinline__ bool grid_any_hit(const SunGridK& sg, const SunGridParams& g, const float4* tris,
+                                             int32_t ntri, const RayPre& r, f3 o, float tmin, float tmax,
+                                             ane_hit* h)
+{
+  const float X = comp(o, g.kx) - g.sx * comp(o, g.kz);
+  const float Y = comp(o, g.ky) - g.sy * comp(o, g.kz);
+  const int64_t cell = (int64_t)sun_cell(Y, g.y0, g.ich, g.gy) * g.gx + sun_cell(X, g.x0, g.icw, g.gx);
+  const uint32_t b = sg.start[cell], e = sg.start[cell + 1];
+  for (uint32_t k = b; k < e; ++k) {
+    const uint32_t p = sg.list[k];
+    if (p >= (uint32_t)ntri) continue;
+    const float4 v0 = tris[3 * (int64_t)p], v1 = tris[3 * (int64_t)p + 1], v2 = tris[3 * (int64_t)p + 2];
+    float t, u, v;
+    if (tri_test(r, mk(v0.x, v0.y, v0.z), mk(v1.x, v1.y, v1.z), mk(v2.x, v2.y, v2.z), tmin, tmax, t, u, v)) {
+      h->tri = __float_as_int(v0.w);
+      h->t = t;
+      h->u = u;
+      h->v = v;
+      return true;
+    }
+  }
+  return false;
+}
+// rays sharing the grid's direction (zero or non-finite direction: no ray); kernels
+// do nothing unless the grid is valid
+void launch_shadow_mask_grid(const float* depth, int W, int H, const float* ivp16, const float* light3,
+                             const SunGridK& sg, const SceneK& sc, float* vis, hipStream_t s);
+void launch_anyhit_grid(const float* orig, const float* dirs, int64_t n, const SunGridK& sg, const SceneK& sc,
+                        float tmin, float tmax, ane_hit* hits, hipStream_t s);
 void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
                            uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
                            hipStream_t s);
@@ -85,6 +117,7 @@ struct TraceArgs {
   const float* ray_dir;
   float t_min, t_max;
   SunGridK sg;             // probe rays: candidates of the shadow rays (list null: traversal)
+  const uint32_t* skip_if_set; // caller rays: the launch does nothing when *skip_if_set != 0
 };
 
 int trace_lanes_per_launch();
@@ -122,8 +155,9 @@ void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const
 
 // RT hard shadows: per-pixel rays (orig, dirs: W*H*3 floats) and visibility
 void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
-                        float* orig, float* dirs, hipStream_t s);
-void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s);
+                        float* orig, float* dirs, hipStream_t s, const uint32_t* skip_if_set = nullptr);
+void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s,
+                           const uint32_t* skip_if_set = nullptr);
 // mirror reflections: reflection rays, the shadow rays of their hits, shading
 void launch_mirror_rays(const float* depth, const float* normal4, int W, int H, const float* ivp16,
                         const float* cam3, float* orig, float* dirs, hipStream_t s);

@@ -274,6 +274,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   // time as batch leaves) instead of traversing the hierarchy
   const bool sgrid = !RAYS && kLeafBatch > 0 && a.sg.list != nullptr && a.sg.params->valid != 0u;
   __shared__ int32_t stk[kSS][kTB];
+  if (RAYS && a.skip_if_set && *a.skip_if_set != 0u) return;  // a sun grid answered these rays
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int gl = blockIdx.x * kTB + tid;
@@ -992,10 +993,11 @@ __device__ __forceinline__ f3 pixel_world_pos(int64_t i, int W, int H, float d, 
 }
 
 __global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, int H, Mat4 ivp, f3 dir,
-                                                     float* orig, float* dirs)
+                                                     float* orig, float* dirs, const uint32_t* skip_if_set)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)W * H) return;
+  if (skip_if_set && *skip_if_set != 0u) return;
   const float d = depth[i];
   const f3 p = pixel_world_pos(i, W, H, d, ivp);
   orig[3 * i + 0] = p.x;
@@ -1009,15 +1011,16 @@ __global__ __launch_bounds__(256) void k_shadow_rays(const float* depth, int W, 
 }
 
 __global__ __launch_bounds__(256) void k_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n,
-                                                        float* vis)
+                                                        float* vis, const uint32_t* skip_if_set)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (skip_if_set && *skip_if_set != 0u) return;
   vis[i] = (depth[i] == 1.0f || hits[i].tri < 0) ? 1.0f : 0.0f;
 }
 
 void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
-                        float* orig, float* dirs, hipStream_t s)
+                        float* orig, float* dirs, hipStream_t s, const uint32_t* skip_if_set)
 {
   Mat4 m;
   for (int k = 0; k < 16; ++k) m.m[k] = ivp16[k];
@@ -1025,13 +1028,82 @@ void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, co
   const int64_t n = (int64_t)W * H;
   if (n <= 0) return;
   hipLaunchKernelGGL(k_shadow_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, W, H, m, dir,
-                     orig, dirs);
+                     orig, dirs, skip_if_set);
 }
 
-void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s)
+void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s,
+                           const uint32_t* skip_if_set)
 {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_shadow_resolve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, hits, n, vis);
+  hipLaunchKernelGGL(k_shadow_resolve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, hits, n, vis,
+                     skip_if_set);
+}
+
+// RT hard shadows through a sun grid built for the rays' direction: one thread
+// per pixel, the world position and ray setup of k_shadow_rays and the trace,
+// the cell's candidates in the watertight test (the any-hit answer of the
+// traversal); nothing unless the grid is valid
+__global__ __launch_bounds__(256) void k_shadow_mask_grid(const float* depth, int W, int H, Mat4 ivp, f3 dir,
+                                                          SunGridK sg, SceneK sc, float* vis)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)W * H) return;
+  const SunGridParams g = *sg.params;
+  if (!g.valid) return;
+  const float d = depth[i];
+  const bool none = d == 1.0f || (dir.x == 0.0f && dir.y == 0.0f && dir.z == 0.0f) ||
+                    !__builtin_isfinite(dir.x) || !__builtin_isfinite(dir.y) || !__builtin_isfinite(dir.z);
+  if (none) {
+    vis[i] = 1.0f;
+    return;
+  }
+  const f3 p = pixel_world_pos(i, W, H, d, ivp);
+  RayPre r;
+  ray_setup(r, p, dir);
+  ane_hit h;
+  vis[i] = grid_any_hit(sg, g, sc.tris, sc.ntri, r, p, 0.1f, 50.0f, &h) ? 0.0f : 1.0f;  // shadow.rgen:66-67
+}
+
+// any hit of caller rays along the grid's direction (hits: tri -1 on a miss)
+__global__ __launch_bounds__(256) void k_anyhit_grid(const float* orig, const float* dirs, int64_t n, SunGridK sg,
+                                                     SceneK sc, float tmin, float tmax, ane_hit* hits)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SunGridParams g = *sg.params;
+  if (!g.valid) return;
+  const f3 o = ld3(orig + 3 * i), d = ld3(dirs + 3 * i);
+  ane_hit h;
+  h.tri = -1;
+  h.t = h.u = h.v = 0.0f;
+  const bool none = (d.x == 0.0f && d.y == 0.0f && d.z == 0.0f) || !__builtin_isfinite(d.x) ||
+                    !__builtin_isfinite(d.y) || !__builtin_isfinite(d.z);
+  if (!none) {
+    RayPre r;
+    ray_setup(r, o, d);
+    (void)grid_any_hit(sg, g, sc.tris, sc.ntri, r, o, tmin, tmax, &h);
+  }
+  hits[i] = h;
+}
+
+void launch_shadow_mask_grid(const float* depth, int W, int H, const float* ivp16, const float* light3,
+                             const SunGridK& sg, const SceneK& sc, float* vis, hipStream_t s)
+{
+  Mat4 m;
+  for (int k = 0; k < 16; ++k) m.m[k] = ivp16[k];
+  const f3 dir = mk(-light3[0], -light3[1], -light3[2]);
+  const int64_t n = (int64_t)W * H;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_shadow_mask_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, depth, W, H, m, dir,
+                     sg, sc, vis);
+}
+
+void launch_anyhit_grid(const float* orig, const float* dirs, int64_t n, const SunGridK& sg, const SceneK& sc,
+                        float tmin, float tmax, ane_hit* hits, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_anyhit_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, orig, dirs, n, sg, sc,
+                     tmin, tmax, hits);
 }
 
 // ---------------------------------------------------------------------------

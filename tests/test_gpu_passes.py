@@ -312,3 +312,38 @@ def test_mirror_reflections_matches_oracle():
         assert rel_err(og[..., :3], oo[..., :3]) <= RTOL
         hits = (depth != 1.0) & (oo[..., 3] < 1e6)
         assert 0 < hits.sum() < (depth != 1.0).sum()
+
+
+def test_shadow_mask_and_mirror_through_sun_grid_match_oracle():
+    """Without counters, the second call with the same scene and light builds a grid
+    of the triangles along the light direction and later calls use it (the
+    traversal launches then do nothing).  Every call's visibility, and the mirror
+    pass's distances and radiance, equal the oracle's."""
+    from tests.test_oracle_shading import _view_proj
+    scene = scenegen.terrain_scene(9, cells=48, num_instances=200, spread=18.0)
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    cfg = Config.make((16, 4, 16), step=(2, 2, 2), rays=64)
+    grid = native.ProbeGrid(cfg)
+    rng = np.random.default_rng(21)
+    atlas = rng.uniform(0.05, 1.5, cfg.atlas_shape).astype(np.float32)
+    grid.atlas = atlas
+    cam = (2.0, 14.0, 18.0)
+    _, ivp = _view_proj(cam, (0.0, 0.0, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    h, w = 90, 160
+    depth = rng.uniform(0.97, 0.9995, (h, w)).astype(np.float32)
+    depth[:6] = 1.0
+    normal4 = rng.normal(size=(h, w, 4)).astype(np.float32)
+    light = np.array([0.35, -0.8, -0.3], np.float32)
+    vo, _ = osc.shadow_mask(depth, ivp16, light)
+    fr = scenegen.make_frame(4, camera=cam, config=cfg, flags=FLAG_DDGI)
+    fr.light_dir[:3] = light
+    oo, _ = osc.mirror_reflections(cfg, fr, atlas, depth, normal4, ivp16)
+    for call in range(3):
+        print(f"[sun grid] screen-space call {call}", flush=True)
+        vg = ds.shadow_mask(depth, ivp16, light)
+        assert np.array_equal(vg, vo), f"shadow mask differs at call {call}"
+        og = grid.mirror_reflections(ds, fr, depth, normal4, ivp16)
+        assert np.array_equal(bits(og[..., 3]), bits(oo[..., 3])), f"mirror distances differ at call {call}"
+        assert rel_err(og[..., :3], oo[..., :3]) <= RTOL
+    assert 0 < (vo == 0).sum() < vo.size
