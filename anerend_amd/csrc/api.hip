@@ -223,6 +223,9 @@ using namespace ane;
 #ifndef ANE_SUNGRID_CAPACITY
 #define ANE_SUNGRID_CAPACITY 16  // entries per triangle before the grid falls back to traversal
 #endif
+#ifndef ANE_SUNGRID_EAGER
+#define ANE_SUNGRID_EAGER 0  // 1: build on the first request with a key
+#endif
 
 struct SunGridCache {
   SunGridParams* params = nullptr;
@@ -231,6 +234,7 @@ struct SunGridCache {
   uint32_t* cur = nullptr;    // [cells] scatter cursors
   uint32_t* list = nullptr;   // [cap] candidates
   uint32_t* scan = nullptr;   // scan scratch
+  float* est = nullptr;       // [2 * 2048] candidate-estimate partials
   int g = 0;
   uint32_t cap = 0;
   struct Key {
@@ -247,7 +251,7 @@ struct SunGridCache {
 
   void release()
   {
-    dfree(params); dfree(acc); dfree(start); dfree(cur); dfree(list); dfree(scan);
+    dfree(params); dfree(acc); dfree(start); dfree(cur); dfree(list); dfree(scan); dfree(est);
     g = 0;
     cap = 0;
     built = Key{};
@@ -268,6 +272,7 @@ struct SunGridCache {
     ANE_TRY(dalloc(&cur, ncell, nullptr));
     ANE_TRY(dalloc(&list, capl, nullptr));
     ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, nullptr));
+    ANE_TRY(dalloc(&est, 2 * 2048, nullptr));
     g = gg;
     cap = (uint32_t)capl;
     return ANE_OK;
@@ -286,14 +291,14 @@ struct SunGridCache {
       *use = true;
       return ANE_OK;
     }
-    if (!(seen == k)) {
+    if (!ANE_SUNGRID_EAGER && !(seen == k)) {
       seen = k;
       return ANE_OK;
     }
     ANE_TRY(alloc(ntri));
     SunGridParams p;
     sun_grid_setup(p, dir3, g, g);
-    launch_sun_grid_build(view, p, params, acc, start, cur, list, cap, scan, st);
+    launch_sun_grid_build(view, p, params, acc, start, cur, list, cap, scan, est, st);
     ANE_CHECK_HIP(hipGetLastError());
     built = k;
     *use = true;
@@ -1337,6 +1342,14 @@ extern "C" int ane_ddgi_stats_get(const ane_ddgi_ctx* cc, ane_ddgi_stats* out)
     unsigned long long v[4] = {0, 0, 0, 0};
     ANE_CHECK_HIP(hipMemcpy(v, c->d_stats, sizeof(v), hipMemcpyDeviceToHost));
     s.primary_nodes = v[0]; s.primary_tris = v[1]; s.shadow_nodes = v[2]; s.shadow_tris = v[3];
+  }
+  s.sun_grid_state = 0;
+  s.sun_grid_candidates = 0.0f;
+  if (c->sgc.built.set) {
+    SunGridParams p;
+    ANE_CHECK_HIP(hipMemcpy(&p, c->sgc.params, sizeof(p), hipMemcpyDeviceToHost));
+    s.sun_grid_state = p.valid ? 1 : 2;
+    s.sun_grid_candidates = p.est;
   }
   uint32_t ov = 0;
   ANE_CHECK_HIP(hipMemcpy(&ov, c->d_overflow, 4, hipMemcpyDeviceToHost));

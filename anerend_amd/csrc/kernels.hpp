@@ -40,6 +40,8 @@ struct SunGridParams {
   int32_t kx, ky, kz;      // axis permutation (the watertight test's)
   int32_t gx, gy;          // cells
   uint32_t valid;          // 0: the traversal takes the shadow rays
+  float est;               // mean candidates of a ray from a triangle (over triangles)
+  float max_est;           // above it the grid is not used (traversal is cheaper)
 };
 struct SunGridK {
   const SunGridParams* params;  // device
@@ -85,9 +87,10 @@ void launch_shadow_mask_grid(const float* depth, int W, int H, const float* ivp1
                              const SunGridK& sg, const SceneK& sc, float* vis, hipStream_t s);
 void launch_anyhit_grid(const float* orig, const float* dirs, int64_t n, const SunGridK& sg, const SceneK& sc,
                         float tmin, float tmax, ane_hit* hits, hipStream_t s);
+// est: [2 * 2048] partial sums of the candidate estimate
 void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
                            uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
-                           hipStream_t s);
+                           float* est, hipStream_t s);
 // probe traces can take a sun grid (needs leaf batching)
 bool trace_grid_supported();
 

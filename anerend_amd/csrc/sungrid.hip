@@ -26,6 +26,9 @@
 namespace ane {
 
 static constexpr int kMaxCellsPerTri = 1 << 16;
+#ifndef ANE_SUNGRID_MAX_CANDIDATES
+#define ANE_SUNGRID_MAX_CANDIDATES 32.0f  // estimated candidates per shadow ray above which traversal is kept
+#endif
 
 // ordered encoding of floats for unsigned atomicMin/atomicMax
 __device__ __forceinline__ uint32_t f2ord(float f)
@@ -222,6 +225,78 @@ __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t
       }
 }
 
+static constexpr int kEstBlocks = 2048;
+
+// Candidate estimate: a shadow ray starts on a triangle and tests the
+// candidates of the cell its origin falls in, one of the triangle's cells; per
+// triangle the mean count of its cells, then the mean over the triangles.
+__global__ __launch_bounds__(256) void k_sun_estimate(const float4* tris, int32_t ntri, const SunGridParams* gp,
+                                                      const uint32_t* start, float* part)
+{
+  __shared__ float rs[256], rn[256];
+  const SunGridParams g = *gp;
+  float sum = 0.0f, num = 0.0f;
+  if (g.valid) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ntri;
+         p += (int64_t)gridDim.x * blockDim.x) {
+      const CellRange c = sun_tri_cells(g, tris, p);
+      float s = 0.0f, k = 0.0f;
+      for (int y = c.cy0; y <= c.cy1; ++y)
+        for (int x = c.cx0; x <= c.cx1; ++x)
+          if (sun_tri_overlaps_cell(g, c, x, y)) {
+            const int64_t cell = (int64_t)y * g.gx + x;
+            s += (float)(start[cell + 1] - start[cell]);
+            k += 1.0f;
+          }
+      if (k > 0.0f) {
+        sum += s / k;
+        num += 1.0f;
+      }
+    }
+  }
+  const int t = threadIdx.x;
+  rs[t] = sum;
+  rn[t] = num;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      rs[t] += rs[t + h];
+      rn[t] += rn[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    part[2 * blockIdx.x] = rs[0];
+    part[2 * blockIdx.x + 1] = rn[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sun_judge(const float* part, SunGridParams* gp)
+{
+  __shared__ float rs[256], rn[256];
+  const int t = threadIdx.x;
+  float s = 0.0f, n = 0.0f;
+  for (int b = t; b < kEstBlocks; b += 256) {
+    s += part[2 * b];
+    n += part[2 * b + 1];
+  }
+  rs[t] = s;
+  rn[t] = n;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      rs[t] += rs[t + h];
+      rn[t] += rn[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const float est = rn[0] > 0.0f ? rs[0] / rn[0] : 0.0f;
+    gp->est = est;
+    if (!(est <= gp->max_est)) gp->valid = 0u;
+  }
+}
+
 void sun_grid_setup(SunGridParams& g, const float* dir3, int gx, int gy)
 {
   // the watertight test's axis permutation and shear for the direction (ray_setup)
@@ -236,11 +311,13 @@ void sun_grid_setup(SunGridParams& g, const float* dir3, int gx, int gy)
   g.gy = gy;
   g.x0 = g.y0 = g.icw = g.ich = g.margin = 0.0f;
   g.valid = 0u;
+  g.est = 0.0f;
+  g.max_est = ANE_SUNGRID_MAX_CANDIDATES;
 }
 
 void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
                            uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
-                           hipStream_t s)
+                           float* est, hipStream_t s)
 {
   const int64_t ncell = (int64_t)g.gx * g.gy;
   fill_u32(cnt, 0u, ncell + 1, s);
@@ -252,6 +329,9 @@ void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridPara
   exclusive_scan(cnt, ncell + 1, scan_scratch, s);
   hipLaunchKernelGGL(k_sun_cursor, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
                      (const uint32_t*)cnt, ncell, cap, d_params, cur);
+  hipLaunchKernelGGL(k_sun_estimate, dim3(kEstBlocks), dim3(256), 0, s, sc.tris, sc.ntri,
+                     (const SunGridParams*)d_params, (const uint32_t*)cnt, est);
+  hipLaunchKernelGGL(k_sun_judge, dim3(1), dim3(256), 0, s, (const float*)est, d_params);
   if (sc.ntri > 0)
     hipLaunchKernelGGL(k_sun_scatter, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri,
                        (const SunGridParams*)d_params, cur, list, cap);

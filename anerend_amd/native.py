@@ -38,7 +38,8 @@ class Stats(ctypes.Structure):
                 ("calls", ctypes.c_uint64 * 6),
                 ("primary_nodes", ctypes.c_uint64), ("primary_tris", ctypes.c_uint64),
                 ("shadow_nodes", ctypes.c_uint64), ("shadow_tris", ctypes.c_uint64),
-                ("rays", ctypes.c_uint64), ("device_bytes", ctypes.c_int64)]
+                ("rays", ctypes.c_uint64), ("device_bytes", ctypes.c_int64),
+                ("sun_grid_state", ctypes.c_int32), ("sun_grid_candidates", ctypes.c_float)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_[3:]}

@@ -704,7 +704,9 @@ def main():
                  "shadow_rays_traversing": {"node_visits": work["shadow_nodes"],
                                             "triangle_tests": work["shadow_tris"]},
                  "shadow_rays_sun_grid": {"cell_lookups": gwork["shadow_nodes"],
-                                          "candidate_tests": gwork["shadow_tris"]}},
+                                          "candidate_tests": gwork["shadow_tris"],
+                                          "state": st["sun_grid_state"],
+                                          "estimated_candidates": round(st["sun_grid_candidates"], 2)}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "conv_study": conv,

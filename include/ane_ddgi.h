@@ -146,6 +146,11 @@ typedef struct ane_ddgi_stats {
   uint64_t primary_nodes, primary_tris, shadow_nodes, shadow_tris;
   uint64_t rays;          /* primary rays of the last trace */
   int64_t device_bytes;   /* HBM held by the context */
+  /* the context's sun grid of the probe shadow rays: 0 none built, 1 in use,
+   * 2 built but not used (over its capacity, or more estimated candidates per
+   * ray than traversal pays for); the estimate of its last build */
+  int32_t sun_grid_state;
+  float sun_grid_candidates;
 } ane_ddgi_stats;
 
 /* One G-buffer texel for the deferred GI lookup (deferred_tiled.comp:264-298 inputs):
