@@ -277,11 +277,18 @@ struct SunGridCache {
     cap = (uint32_t)capl;
     return ANE_OK;
   }
-  // *use: the grid for (scene, version, dir) is current after the call
+  // *use: the grid for (scene, version, dir) is current after the call.  A
+  // direction with a zero (or non-finite) component never gets one: its rays
+  // can lie exactly in a box face, where the traversal's slab test (zero
+  // components take the reciprocal of +-1e-30) culls boxes whose triangles the
+  // watertight test would touch on an edge, so a grid's answer could differ
+  // from the traversal's there.
   int update(const SceneK& view, int32_t ntri, const void* scene, uint64_t version, const float* dir3,
              hipStream_t st, bool* use)
   {
     *use = false;
+    for (int k = 0; k < 3; ++k)
+      if (!(dir3[k] != 0.0f) || !std::isfinite(dir3[k])) return ANE_OK;
     Key k;
     k.scene = scene;
     k.version = version;

@@ -107,3 +107,65 @@ def test_sun_grid_shadow_rays_match_oracle(family):
         lit += int((sho == 0).sum())
         shadowed += int((sho == 1).sum())
     assert lit > 0 and shadowed > 0
+
+
+@pytest.mark.parametrize("light", [(0.0, -1.0, 0.0), (1.0, 0.0, 0.0), (0.0, -0.6, 0.8)])
+def test_sun_grid_axis_aligned_suns(light):
+    """Suns along an axis or in a coordinate plane.  Their rays can lie exactly
+    in the room's box faces, where the traversal's slab test (zero components)
+    culls triangles the watertight test touches on an edge; such directions keep
+    the traversal (no sun grid), so probe updates and the shadow mask equal the
+    oracle update by update."""
+    from tests.test_oracle_shading import _view_proj
+    scene, cfg, _ = scenegen.workload("room8")
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    _, ivp = _view_proj((0.3, 1.8, 4.2), (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    rng = np.random.default_rng(3)
+    depth = rng.uniform(0.95, 0.9995, (30, 40)).astype(np.float32)
+    depth[rng.uniform(size=depth.shape) < 0.1] = 1.0
+    l3 = np.array(light, np.float32)
+    vo, _ = osc.shadow_mask(depth, ivp16, l3)
+    for it in range(3):
+        fr = scenegen.make_frame(6, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        fr.light_dir[:3] = light
+        print(f"axis sun {light} update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"])
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        assert np.array_equal(ds.shadow_mask(depth, ivp16, l3), vo), f"shadow mask differs at call {it}"
+    assert grid.stats()["sun_grid_state"] == 0
+
+
+def test_sun_grid_near_axis_sun():
+    """A sun a hair off the vertical (no zero component) uses the grid; the room's
+    walls then project to thin slivers.  Updates and the shadow mask equal the
+    oracle."""
+    from tests.test_oracle_shading import _view_proj
+    scene, cfg, _ = scenegen.workload("room8")
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    _, ivp = _view_proj((0.3, 1.8, 4.2), (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    rng = np.random.default_rng(5)
+    depth = rng.uniform(0.95, 0.9995, (30, 40)).astype(np.float32)
+    light = (1e-3, -1.0, -2e-3)
+    l3 = np.array(light, np.float32)
+    vo, _ = osc.shadow_mask(depth, ivp16, l3)
+    for it in range(3):
+        fr = scenegen.make_frame(6, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        fr.light_dir[:3] = light
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"])
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        assert np.array_equal(ds.shadow_mask(depth, ivp16, l3), vo), f"shadow mask differs at call {it}"
+    assert grid.stats()["sun_grid_state"] == 1
