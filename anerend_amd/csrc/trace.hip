@@ -8,13 +8,19 @@
 //   k_trace          persistent wave-wide BVH traversal, one ray per lane: each wave
 //                    claims chunks of consecutive (direction-coherent) rays and refills
 //                    finished lanes by ballot between traversal steps; short stack in
-//                    LDS with a global spill area.  A probe ray's closest hit
-//                    (rgen:90-106) is followed in the same lane by its any-hit shadow
-//                    ray from the hit position (rchit:30-66)
+//                    LDS with a global spill area; leaves tested in wave-wide batches.
+//                    A probe ray's closest hit (rgen:90-106) is followed in the same
+//                    lane by its any-hit shadow ray from the hit position (rchit:30-66),
+//                    against the triangles of its sun-grid cell when the context has a
+//                    current grid (sungrid.hip), else through the hierarchy
 //   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
 //                    indirect light from the previous atlas, emissive, sky on miss
 //                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
 //                    context precision (FP8: per-probe scale from the block maximum)
+//   k_gi_lookup      the probe term of deferred_tiled.comp:264-298 per G-buffer texel
+//   k_shadow_*       RT hard shadows of a depth buffer (shadow.rgen), through the
+//                    scene's sun grid (k_shadow_mask_grid) or caller-ray traversal
+//   k_mirror_*       mirror reflections (mirror_reflections.rgen/.rchit/.rmiss)
 #include "common.hpp"
 #include "kernels.hpp"
 
