@@ -169,3 +169,44 @@ def test_sun_grid_near_axis_sun():
         assert rel_err(grid.atlas, atlas) <= RTOL
         assert np.array_equal(ds.shadow_mask(depth, ivp16, l3), vo), f"shadow mask differs at call {it}"
     assert grid.stats()["sun_grid_state"] == 1
+
+
+def test_sun_grid_rebuilt_after_refit():
+    """A refit changes the scene version: the context's sun grid of the old
+    geometry is not used again (the next update traverses, the one after builds
+    a grid of the moved instances).  Every update and the shadow mask equal the
+    oracle, which refits the same way."""
+    from tests.test_oracle_shading import _view_proj
+    scene = scenegen.room_scene(1)
+    cfg = scenegen.small_room_config()
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    _, ivp = _view_proj((0.3, 1.8, 4.2), (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    depth = np.random.default_rng(2).uniform(0.95, 0.9995, (30, 40)).astype(np.float32)
+    inst = scene.instances.copy()
+    rng = np.random.default_rng(10)
+    light = np.array([0.3, -0.85, 0.4], np.float32)
+    states = []
+    for it in range(6):
+        if it == 3:   # move every box; the room stays
+            inst["transform"][1:, 3] += rng.uniform(-0.6, 0.6, len(inst) - 1).astype(np.float32)
+            inst["transform"][1:, 11] += rng.uniform(-0.6, 0.6, len(inst) - 1).astype(np.float32)
+            ds.refit(inst)
+            osc.refit(inst)
+        fr = scenegen.make_frame(8, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        fr.light_dir[:3] = light
+        print(f"refit sun grid update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), f"hits differ at update {it}"
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        vo, _ = osc.shadow_mask(depth, ivp16, light)
+        assert np.array_equal(ds.shadow_mask(depth, ivp16, light), vo), f"shadow mask differs at update {it}"
+        states.append(grid.stats()["sun_grid_state"])
+    # built at update 1, used at 2; stale after the refit at 3 (the state still
+    # reports the last build); rebuilt at 4
+    assert states[1] == 1 and states[4] == 1
