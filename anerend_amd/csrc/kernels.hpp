@@ -135,7 +135,7 @@ struct RayDiv {
   uint64_t magic;
   int32_t shift;
 };
-// exact x / n for 0 <= x < nrays (nrays < 2^31, 1 <= n <= 4096)
+// exact x / n for 0 <= x < nrays (nrays < 2^31, n >= 1): 2^shift > nrays * n
 RayDiv ray_div_magic(int64_t nrays, int n);
 void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float4* probe_org,
                   const ane_hit* hits,
