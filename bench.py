@@ -259,7 +259,8 @@ def camera_gbuffer(ds, width, height):
     hit = hits["tri"] >= 0
     pos = eye + d.astype(float) * hits["t"][:, None].astype(float)
     ndc = np.concatenate([pos, np.ones((len(pos), 1))], 1) @ VP.T
-    depth = np.where(hit, ndc[:, 2] / ndc[:, 3], 1.0).astype(np.float32).reshape(height, width)
+    with np.errstate(divide="ignore", invalid="ignore"):   # misses: w is 0 or inf, masked below
+        depth = np.where(hit, ndc[:, 2] / ndc[:, 3], 1.0).astype(np.float32).reshape(height, width)
     p = pos.reshape(height, width, 3)
     gx = np.gradient(p, axis=1)
     gy = np.gradient(p, axis=0)
