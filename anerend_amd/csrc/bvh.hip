@@ -52,7 +52,10 @@ void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// flatten: world-space triangles in input order + centroid bounds
+// flatten: world-space triangles in input order + centroid bounds.  The first
+// build finds each triangle's instance, mesh, indices and vertices and writes its
+// 64-byte record (object-space positions, vertex indices, instance); rebuilds and
+// refits keep every instance's mesh (checked on the host) and read that record.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32_t* tri_inst,
                                                  float4* tri_rec, uint32_t* cbounds,
@@ -62,26 +65,46 @@ __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32
   if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
   __syncthreads();
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, cmax[3] = {0u, 0u, 0u};
   if (t < sc.ntri) {
-    // instance k: last k with inst_tri_off[k] <= t
-    int lo = 0, hi = sc.ninst - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (sc.inst_tri_off[mid] <= (uint32_t)t) lo = mid; else hi = mid - 1;
+    int lo;
+    f3 po[3];
+    float4* r = tri_rec + 4 * (int64_t)t;
+    if (write_inst) {
+      // instance k: last k with inst_tri_off[k] <= t
+      lo = 0;
+      int hi = sc.ninst - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sc.inst_tri_off[mid] <= (uint32_t)t) lo = mid; else hi = mid - 1;
+      }
+      const ane_mesh& m = sc.meshes[mesh_of(sc.inst[lo].custom_index_and_mask)];
+      uint32_t prim = (uint32_t)t - sc.inst_tri_off[lo];
+      uint32_t vi[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        uint32_t idx = sc.indices[m.index_offset + 3 * prim + j];
+        vi[j] = m.vertex_offset + idx;
+        po[j] = ld3(sc.vertices + (int64_t)(m.vertex_offset + idx) * 24);
+      }
+      tri_inst[t] = (uint32_t)lo;
+      // one 64-byte record per triangle: what a hit needs, in one cache line
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r[j] = make_float4(po[j].x, po[j].y, po[j].z, __uint_as_float(vi[j]));
+      r[3] = make_float4(__uint_as_float((uint32_t)lo), 0.0f, 0.0f, 0.0f);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float4 q = r[j];
+        po[j] = mk(q.x, q.y, q.z);
+      }
+      lo = (int)__float_as_uint(r[3].x);
     }
     const ane_instance& in = sc.inst[lo];
-    const ane_mesh& m = sc.meshes[mesh_of(in.custom_index_and_mask)];
-    uint32_t prim = (uint32_t)t - sc.inst_tri_off[lo];
     float w[9];
-    uint32_t vi[3];
-    f3 po[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      uint32_t idx = sc.indices[m.index_offset + 3 * prim + j];
-      vi[j] = m.vertex_offset + idx;
-      f3 p = ld3(sc.vertices + (int64_t)(m.vertex_offset + idx) * 24);
-      po[j] = p;
-      f3 q = xform_point(in.transform, p);
+      f3 q = xform_point(in.transform, po[j]);
       w[3 * j + 0] = q.x;
       w[3 * j + 1] = q.y;
       w[3 * j + 2] = q.z;
@@ -89,21 +112,28 @@ __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32
     float* dst = world + (int64_t)9 * t;
 #pragma unroll
     for (int j = 0; j < 9; ++j) dst[j] = w[j];
-    if (write_inst) {
-      tri_inst[t] = (uint32_t)lo;
-      // one 64-byte record per triangle: what a hit needs, in one cache line
-      float4* r = tri_rec + 4 * (int64_t)t;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) r[j] = make_float4(po[j].x, po[j].y, po[j].z, __uint_as_float(vi[j]));
-      r[3] = make_float4(__uint_as_float((uint32_t)lo), 0.0f, 0.0f, 0.0f);
-    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       float l = fminf(fminf(w[a], w[3 + a]), w[6 + a]);
       float h = fmaxf(fmaxf(w[a], w[3 + a]), w[6 + a]);
       float c = (l + h) * 0.5f;
-      atomicMin(&sb[a], f2o(c));
-      atomicMax(&sb[3 + a], f2o(c));
+      cmin[a] = f2o(c);
+      cmax[a] = cmin[a];
+    }
+  }
+  // wave-wide min / max first: one LDS atomic per wave and bound, not one per thread
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      cmin[a] = min(cmin[a], (uint32_t)__shfl_xor((int)cmin[a], off, 64));
+      cmax[a] = max(cmax[a], (uint32_t)__shfl_xor((int)cmax[a], off, 64));
+    }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&sb[a], cmin[a]);
+      atomicMax(&sb[3 + a], cmax[a]);
     }
   }
   __syncthreads();
