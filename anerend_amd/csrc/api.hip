@@ -102,6 +102,35 @@ static void fibonacci_dirs(int n, float* out)
   }
 }
 
+// Weights of light_shadow_sum.comp:45-61, wt[ray][texel] for the 1024 rays of a
+// light (light_shadow_helpers.glsl:1) and the 64 texels of its octahedral tile.
+// The direction image is RGBA8_UNORM (LightShadowRayTracingPass.cpp:53): the
+// rgen stores (dir + 1) * 0.5 (light_shadow.rgen:83) and the sum reads it back as
+// c / 255 * 2 - 1.  Contract: clamp to [0, 1], times 255, round to nearest even.
+// pow(x, 50) is the binary32 product chain x^2, x^4, x^8, x^16, x^32, then
+// (x^32 * x^16) * x^2 (GLSL pow precision is implementation-defined).  Weights not
+// above 0.001 are stored as 0 (sum.comp:59).  The rounded directions are not
+// renormalized, so a weight can exceed 1, as in the shader.
+static void light_weights(const float* dirs, float* wt)
+{
+  for (int r = 0; r < kLightRays; ++r) {
+    float q[3];
+    for (int k = 0; k < 3; ++k) {
+      float v = (dirs[3 * r + k] + 1.0f) * 0.5f;
+      v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+      q[k] = std::rint(v * 255.0f) / 255.0f * 2.0f - 1.0f;
+    }
+    for (int t = 0; t < 64; ++t) {
+      const f3 o = texel_dir(t);
+      float c = q[0] * o.x + q[1] * o.y + q[2] * o.z;
+      c = c > 0.0f ? c : 0.0f;
+      const float x2 = c * c, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16;
+      const float w = (x32 * x16) * x2;
+      wt[r * 64 + t] = w > 0.001f ? w : 0.0f;
+    }
+  }
+}
+
 // Processing order of the N ray indices: sorted by the 2-D Morton code of the
 // octahedral encoding of their (unrotated) direction, so that the 64 lanes of a
 // wave trace a compact cone of directions from one probe.  Every probe shares
@@ -375,6 +404,25 @@ struct ane_scene {
   uint32_t* d_sh_ov = nullptr;
   int32_t* d_sh_spill = nullptr;
   int64_t sh_spill_words = 0;
+  // point-light shadow workspace (grows to the most lights seen); the sum's
+  // weights and the 1024 ray directions are made once per scene
+  int32_t lt_cap = 0;
+  float* d_lt_lights = nullptr;
+  float* d_lt_orig = nullptr;
+  float* d_lt_dir = nullptr;
+  float* d_lt_tmax = nullptr;
+  ane_hit* d_lt_hits = nullptr;
+  float* d_lt_dist = nullptr;
+  float* d_lt_dirs = nullptr;
+  float* d_lt_wt = nullptr;       // [lt_kmax][64] weights of each texel's rays, summation order
+  int32_t* d_lt_idx = nullptr;    // [lt_kmax][64] their ray indices
+  uint32_t* d_lt_order = nullptr; // [1024] ray traced in each slot (direction-coherent order)
+  int32_t lt_kmax = 0;
+  uint32_t* d_lt_work = nullptr;
+  unsigned long long* d_lt_stats = nullptr;
+  uint32_t* d_lt_ov = nullptr;
+  int32_t* d_lt_spill = nullptr;
+  int64_t lt_spill_words = 0;
 };
 
 static void scene_free(ane_scene* s)
@@ -387,6 +435,10 @@ static void scene_free(ane_scene* s)
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
+  dfree(s->d_lt_lights); dfree(s->d_lt_orig); dfree(s->d_lt_dir); dfree(s->d_lt_tmax); dfree(s->d_lt_hits);
+  dfree(s->d_lt_dist); dfree(s->d_lt_dirs); dfree(s->d_lt_wt); dfree(s->d_lt_idx); dfree(s->d_lt_order); dfree(s->d_lt_work);
+  dfree(s->d_lt_stats);
+  dfree(s->d_lt_ov); dfree(s->d_lt_spill);
   s->sgc.release();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -1452,6 +1504,123 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
     uint32_t ov = 0;
     ANE_CHECK_HIP(hipMemcpyAsync(c2, s->d_sh_stats, 16, hipMemcpyDeviceToHost, st));
     ANE_CHECK_HIP(hipMemcpyAsync(&ov, s->d_sh_ov, 4, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    counters2[0] = c2[0];
+    counters2[1] = c2[1];
+    if (ov) return fail(ANE_ERR_DEVICE, "traversal stack overflow");
+  }
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_light_shadows(ane_scene* s, const ane_light* lights, int32_t num_lights,
+                                       float* dist_img, float* moments_img, uint64_t* counters2,
+                                       void* stream)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "NULL scene");
+  if (num_lights < 0 || num_lights > ANE_MAX_LIGHTS)
+    return fail(ANE_ERR_INVALID_ARG, "num_lights must be in [0, 1024]");
+  if (num_lights == 0) return ANE_OK;
+  if (!lights || !moments_img) return fail(ANE_ERR_INVALID_ARG, "NULL lights or moments image");
+  for (int32_t l = 0; l < num_lights; ++l) {
+    if (!(lights[l].color[3] >= 0.5f)) continue;  // disabled (NaN counts as disabled, as in the shader)
+    for (int k = 0; k < 4; ++k)
+      if (!std::isfinite(lights[l].world_pos[k]))
+        return fail(ANE_ERR_INVALID_ARG, "non-finite position or range of an enabled light");
+  }
+  hipStream_t st = S(stream);
+  const int64_t m = (int64_t)num_lights * kLightRays;
+  if (!s->d_lt_wt) {
+    std::vector<float> dirs((size_t)kLightRays * 3), wt((size_t)kLightRays * 64);
+    fibonacci_dirs(kLightRays, dirs.data());  // worldDirFromFibonacciIndex, light_shadow.rgen:27-38
+    light_weights(dirs.data(), wt.data());
+    // each texel's rays with a weight above 0.001, in the shader's order (x outer,
+    // y inner over the 32 x 32 rays, light_shadow_sum.comp:50-52)
+    std::vector<std::vector<std::pair<float, int32_t>>> list(64);
+    for (int x = 0; x < 32; ++x)
+      for (int y = 0; y < 32; ++y) {
+        const int idx = y * 32 + x;
+        for (int t = 0; t < 64; ++t)
+          if (wt[(size_t)idx * 64 + t] > 0.001f) list[t].push_back({wt[(size_t)idx * 64 + t], idx});
+      }
+    int kmax = 1;
+    for (const auto& e : list) kmax = std::max(kmax, (int)e.size());
+    std::vector<float> lw((size_t)kmax * 64, 0.0f);
+    std::vector<int32_t> li((size_t)kmax * 64, 0);
+    for (int t = 0; t < 64; ++t)
+      for (size_t k = 0; k < list[t].size(); ++k) {
+        lw[k * 64 + t] = list[t][k].first;
+        li[k * 64 + t] = list[t][k].second;
+      }
+    std::vector<uint32_t> order;
+    ray_order(kLightRays, dirs.data(), &order);
+    ANE_TRY(dalloc(&s->d_lt_order, kLightRays, nullptr));
+    ANE_CHECK_HIP(hipMemcpy(s->d_lt_order, order.data(), sizeof(uint32_t) * kLightRays, hipMemcpyHostToDevice));
+    ANE_TRY(dalloc(&s->d_lt_dirs, 3 * kLightRays, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_wt, 64 * (int64_t)kmax, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_idx, 64 * (int64_t)kmax, nullptr));
+    ANE_CHECK_HIP(hipMemcpy(s->d_lt_dirs, dirs.data(), sizeof(float) * dirs.size(), hipMemcpyHostToDevice));
+    ANE_CHECK_HIP(hipMemcpy(s->d_lt_wt, lw.data(), sizeof(float) * lw.size(), hipMemcpyHostToDevice));
+    ANE_CHECK_HIP(hipMemcpy(s->d_lt_idx, li.data(), sizeof(int32_t) * li.size(), hipMemcpyHostToDevice));
+    s->lt_kmax = kmax;
+    ANE_TRY(dalloc(&s->d_lt_work, 1, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_stats, 2, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_ov, 1, nullptr));
+  }
+  if (num_lights > s->lt_cap) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_lt_lights); dfree(s->d_lt_orig); dfree(s->d_lt_dir); dfree(s->d_lt_tmax);
+    dfree(s->d_lt_hits); dfree(s->d_lt_dist);
+    s->lt_cap = 0;
+    ANE_TRY(dalloc(&s->d_lt_lights, 8 * (int64_t)num_lights, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_orig, 3 * m, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_dir, 3 * m, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_tmax, m, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_hits, m, nullptr));
+    ANE_TRY(dalloc(&s->d_lt_dist, m, nullptr));
+    s->lt_cap = num_lights;
+  }
+  const int64_t lanes = trace_lanes_per_launch();
+  const int64_t sd = s->depth - trace_stack_lds() + 1 > 0 ? s->depth - trace_stack_lds() + 1 : 0;
+  if (sd * lanes > s->lt_spill_words) {
+    ANE_CHECK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_lt_spill);
+    s->lt_spill_words = 0;
+    ANE_TRY(dalloc(&s->d_lt_spill, sd * lanes, nullptr));
+    s->lt_spill_words = sd * lanes;
+  }
+  // the light records are read on the device (LightBuffer, bindless.glsl:129-131)
+  ANE_CHECK_HIP(hipMemcpyAsync(s->d_lt_lights, lights, sizeof(ane_light) * (size_t)num_lights,
+                               hipMemcpyHostToDevice, st));
+  launch_light_rays(s->d_lt_lights, num_lights, s->d_lt_dirs, s->d_lt_order, s->d_lt_orig, s->d_lt_dir,
+                    s->d_lt_tmax, st);
+  fill_u32(s->d_lt_work, 0u, 1, st);
+  fill_u32(reinterpret_cast<uint32_t*>(s->d_lt_stats), 0u, 4, st);
+  fill_u32(s->d_lt_ov, 0u, 1, st);
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g.rays = 1;
+  a.hits = s->d_lt_hits;
+  a.work = s->d_lt_work;
+  a.stats = s->d_lt_stats;
+  a.overflow = s->d_lt_ov;
+  a.spill = s->d_lt_spill;
+  a.spill_depth = (int32_t)sd;
+  a.nrays = (int32_t)m;
+  a.ray_orig = s->d_lt_orig;
+  a.ray_dir = s->d_lt_dir;
+  a.t_min = 0.1f;          // light_shadow.rgen:65
+  a.t_max = 0.0f;          // per ray: the light's range (rgen:66)
+  a.ray_tmax = s->d_lt_tmax;
+  launch_trace(a, false, true, counters2 != nullptr, st);  // closest hit: rchit reports gl_HitTEXT
+  launch_light_dist(s->d_lt_lights, num_lights, s->d_lt_order, s->d_lt_hits, s->d_lt_dist, dist_img, st);
+  launch_light_moments(s->d_lt_lights, num_lights, s->d_lt_dist, s->d_lt_wt, s->d_lt_idx, s->lt_kmax,
+                       moments_img, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  if (counters2) {
+    unsigned long long c2[2] = {0, 0};
+    uint32_t ov = 0;
+    ANE_CHECK_HIP(hipMemcpyAsync(c2, s->d_lt_stats, 16, hipMemcpyDeviceToHost, st));
+    ANE_CHECK_HIP(hipMemcpyAsync(&ov, s->d_lt_ov, 4, hipMemcpyDeviceToHost, st));
     ANE_CHECK_HIP(hipStreamSynchronize(st));
     counters2[0] = c2[0];
     counters2[1] = c2[1];

@@ -119,6 +119,7 @@ struct TraceArgs {
   const float* ray_orig;
   const float* ray_dir;
   float t_min, t_max;
+  const float* ray_tmax;   // caller rays: per-ray t_max (null: t_max for every ray)
   SunGridK sg;             // probe rays: candidates of the shadow rays (list null: traversal)
   const uint32_t* skip_if_set; // caller rays: the launch does nothing when *skip_if_set != 0
 };
@@ -159,6 +160,17 @@ void launch_gi_lookup(const GridK& g, const FrameK& f, const float* atlas, const
 // RT hard shadows: per-pixel rays (orig, dirs: W*H*3 floats) and visibility
 void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,
                         float* orig, float* dirs, hipStream_t s, const uint32_t* skip_if_set = nullptr);
+// point-light shadow moments: lights = n x 8 floats {x, y, z, range, r, g, b, enabled};
+// 1024 rays per light; lw / lidx [kmax][64 texels] = each texel's (weight, ray) pairs
+// with weight above 0.001 in the summation order, padded with weight 0
+static constexpr int kLightRays = 1024;
+// order[j]: the ray traced in slot j of each light (a permutation of 0..1023)
+void launch_light_rays(const float* lights, int n, const float* dirs, const uint32_t* order, float* orig,
+                       float* rdir, float* tmax, hipStream_t s);
+void launch_light_dist(const float* lights, int n, const uint32_t* order, const ane_hit* hits, float* dist,
+                       float* dist_img, hipStream_t s);
+void launch_light_moments(const float* lights, int n, const float* dist, const float* lw,
+                          const int32_t* lidx, int kmax, float* mom, hipStream_t s);
 void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, float* vis, hipStream_t s,
                            const uint32_t* skip_if_set = nullptr);
 // mirror reflections: reflection rays, the shadow rays of their hits, shading

@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
             sph = false;
           }
           tbest = tmax0;
+          if (RAYS && a.ray_tmax) tbest = a.ray_tmax[gidx];
           bid = -1;
           sp = 0;
           steps_left = budget;
@@ -1023,6 +1024,112 @@ __global__ __launch_bounds__(256) void k_shadow_resolve(const float* depth, cons
   if (i >= n) return;
   if (skip_if_set && *skip_if_set != 0u) return;
   vis[i] = (depth[i] == 1.0f || hits[i].tri < 0) ? 1.0f : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// point-light shadow moments (LightShadowRayTracingPass.cpp:17-100 with
+// light_shadow.rgen/.rchit/.rmiss; LightShadowSumPass.cpp:16-87 with
+// light_shadow_sum.comp; constants of light_shadow_helpers.glsl:1-4)
+// ---------------------------------------------------------------------------
+static constexpr int kLightSide = 32;  // LIGHT_DIR_DIST_PIX_SIZE: ray y * 32 + x at pixel (x, y)
+
+// light_shadow.rgen:40-81: ray r of light l leaves the light's position along
+// Fibonacci direction r over [0.1, range]; a disabled light (color.w < 0.5) casts
+// no ray (zero direction).  Slot j of a light traces ray order[j], so that the
+// lanes of a wave follow neighbouring directions.
+__global__ __launch_bounds__(256) void k_light_rays(const float* lights, int n, const float* dirs,
+                                                    const uint32_t* order, float* orig, float* rdir,
+                                                    float* tmax)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * kLightRays) return;
+  const int l = (int)(i / kLightRays), r = (int)order[i % kLightRays];
+  const float* L = lights + 8 * (int64_t)l;
+  const bool on = L[7] >= 0.5f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    orig[3 * i + k] = L[k];
+    rdir[3 * i + k] = on ? dirs[3 * r + k] : 0.0f;
+  }
+  tmax[i] = L[3];
+}
+
+// light_shadow.rchit:29-32 (hit distance), light_shadow.rmiss:12-15 (the range),
+// light_shadow.rgen:54-58 (disabled light: 0), stored at rgen:43-49, 84
+__global__ __launch_bounds__(256) void k_light_dist(const float* lights, int n, const uint32_t* order,
+                                                    const ane_hit* hits, float* dist, float* dist_img)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * kLightRays) return;
+  const int l = (int)(i / kLightRays), r = (int)order[i % kLightRays];
+  const float* L = lights + 8 * (int64_t)l;
+  const float d = L[7] < 0.5f ? 0.0f : (hits[i].tri >= 0 ? hits[i].t : L[3]);
+  dist[(int64_t)l * kLightRays + r] = d;
+  if (dist_img)
+    dist_img[(int64_t)(r / kLightSide) * kLightSide * n + (int64_t)kLightSide * l + r % kLightSide] = d;
+}
+
+// light_shadow_sum.comp:16-71: one thread per texel of one light's 8x8 tile.  The
+// shader visits the 1024 rays x outer, y inner and adds those whose weight
+// pow(max(0, dot(dir, octDir)), 50) is above 0.001 (63-69 of them per texel).
+// lw / lidx [k][texel] list exactly those (weight, ray) pairs of each texel in that
+// order, padded with weight 0, so the sums are the shader's, operation for
+// operation, whatever the launch shape.  Texel (tx, ty) of light l lands at
+// (10 l + 1 + tx, ty) of the RG32F image; the border columns are not written.
+__global__ __launch_bounds__(64) void k_light_moments(const float* lights, int n, const float* dist,
+                                                      const float* lw, const int32_t* lidx, int kmax,
+                                                      float* mom)
+{
+  __shared__ float ds[kLightRays];
+  const int l = blockIdx.x, t = threadIdx.x;
+  for (int k = t; k < kLightRays; k += 64) ds[k] = dist[(int64_t)l * kLightRays + k];
+  __syncthreads();
+  float ar = 0.0f, ag = 0.0f, aw = 0.0f;
+  if (lights[8 * (int64_t)l + 7] >= 0.5f) {
+#pragma unroll 8
+    for (int k = 0; k < kmax; ++k) {
+      const float w = lw[k * 64 + t];
+      if (w > 0.001f) {
+        const float d = ds[lidx[k * 64 + t]];
+        ar += w * d;
+        ag += w * d * d;
+        aw += w;
+      }
+    }
+    if (aw > 1e-6f) {
+      ar = ar / aw;
+      ag = ag / aw;
+    }
+  }
+  float* o = mom + 2 * ((int64_t)(t >> 3) * (10 * (int64_t)n) + 10 * (int64_t)l + 1 + (t & 7));
+  o[0] = ar;
+  o[1] = ag;
+}
+
+void launch_light_rays(const float* lights, int n, const float* dirs, const uint32_t* order, float* orig,
+                       float* rdir, float* tmax, hipStream_t s)
+{
+  const int64_t m = (int64_t)n * kLightRays;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_light_rays, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, lights, n, dirs,
+                     order, orig, rdir, tmax);
+}
+
+void launch_light_dist(const float* lights, int n, const uint32_t* order, const ane_hit* hits, float* dist,
+                       float* dist_img, hipStream_t s)
+{
+  const int64_t m = (int64_t)n * kLightRays;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_light_dist, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, lights, n, order,
+                     hits, dist, dist_img);
+}
+
+void launch_light_moments(const float* lights, int n, const float* dist, const float* lw,
+                          const int32_t* lidx, int kmax, float* mom, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_light_moments, dim3((unsigned)n), dim3(64), 0, s, lights, n, dist, lw, lidx, kmax,
+                     mom);
 }
 
 void launch_shadow_rays(const float* depth, int W, int H, const float* ivp16, const float* light3,

@@ -85,6 +85,7 @@ _SIGS = [
     ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
     ("ane_scene_shadow_mask", ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
+    ("ane_scene_light_shadows", ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     ("ane_scene_trace_rays", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
     ("ane_ddgi_ray_directions", ctypes.c_int, [_i32, _vp, _vp]),
 ]
@@ -227,6 +228,28 @@ class DeviceScene:
             src.close()
             dst.close()
         return (vis, ctr) if count else vis
+
+    def light_shadows(self, lights, count=False, moments_fill=0.0):
+        """Point-light shadow moments of ``lights`` [n][8] = {x, y, z, range, r, g, b,
+        enabled} -> (distance image [32][32 n], moments image [8][10 n][2]) (and
+        counters).  Border texels of the moments image keep ``moments_fill``."""
+        L = np.ascontiguousarray(lights, np.float32).reshape(-1, 8)
+        n = len(L)
+        dist = np.zeros((32, 32 * n), np.float32)
+        mom = np.full((8, 10 * n, 2), moments_fill, np.float32)
+        ctr = np.zeros(2, np.uint64)
+        dd, dm = DeviceBuffer(max(dist.nbytes, 4)), DeviceBuffer(max(mom.nbytes, 4))
+        try:
+            dm.upload(mom)
+            check(load().ane_scene_light_shadows(self._h, ptr(L), n, dd.ptr, dm.ptr,
+                                                 ptr(ctr) if count else None, None))
+            synchronize()
+            dist = dd.download(dist)
+            mom = dm.download(mom)
+        finally:
+            dd.close()
+            dm.close()
+        return (dist, mom, ctr) if count else (dist, mom)
 
     def trace_rays(self, orig, dirs, tmin=0.1, tmax=50.0, any_hit=False, count=False, stream=None):
         orig = np.ascontiguousarray(orig, np.float32)

@@ -362,6 +362,52 @@ def mirror_study(grid, ds, frame, width=1920, height=1080, reps=10):
             "shadow_traversal_triangle_tests": int(ctr[3])}
 
 
+def light_shadow_study(ds, num_lights=1024, reps=10, seed=21):
+    """Point-light shadow moments (light_shadow.rgen + light_shadow_sum.comp) of
+    ``num_lights`` seeded lights (the reference's full light budget) over the
+    scene: positions within +-200 m in x, z and 1-12 m above the terrain's base,
+    ranges U[5, 30] m.  Timed with HIP events over ``reps`` calls (upload of the
+    light records, 1024 rays per light, distances, 8x8 moments per light)."""
+    import ctypes
+    from anerend_amd import native
+    rng = np.random.default_rng(seed)
+    n = num_lights
+    L = np.zeros((n, 8), np.float32)
+    L[:, 0] = rng.uniform(-200.0, 200.0, n)
+    L[:, 1] = rng.uniform(1.0, 12.0, n)
+    L[:, 2] = rng.uniform(-200.0, 200.0, n)
+    L[:, 3] = rng.uniform(5.0, 30.0, n)
+    L[:, 4:7] = 1.0
+    L[:, 7] = 1.0
+    dist, _, ctr = ds.light_shadows(L, count=True)
+    dd, dm = native.DeviceBuffer(4 * 32 * 32 * n), native.DeviceBuffer(4 * 8 * 10 * n * 2)
+    hip = native.DeviceBuffer._hip
+    lib = native.load()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    try:
+        lp = L.ctypes.data_as(ctypes.c_void_p)
+        native.check(lib.ane_scene_light_shadows(ds.handle, lp, n, dd.ptr, dm.ptr, None, None))
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            native.check(lib.ane_scene_light_shadows(ds.handle, lp, n, dd.ptr, dm.ptr, None, None))
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        t = ms.value / reps
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        dd.close()
+        dm.close()
+    rays = n * 1024
+    return {"lights": n, "rays": rays, "ms": round(t, 5), "rays_per_s": round(rays / (t * 1e-3), 1),
+            "hit_fraction": round(float((dist < L[:, 3].repeat(32)[None, :]).mean()), 4),
+            "node_visits": int(ctr[0]), "triangle_tests": int(ctr[1])}
+
+
 def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
     """Error of one convolution of ``grid_conv`` (its precision) against a float64
     GEMM of the FP32 radiance of the same frame traced by ``grid_ref`` (FP32) from
@@ -671,6 +717,10 @@ def main():
     mirror = mirror_study(grid, ds, frame_at(0))
     log(f"[bench] mirror reflections {json.dumps(mirror)}")
 
+    log("[bench] point-light shadow moments, 1024 lights")
+    lights = light_shadow_study(ds)
+    log(f"[bench] light shadows {json.dumps(lights)}")
+
     conv = None
     if not args.no_conv_study:
         log("[bench] convolution study (FP32 / BF16 / FP8 vs FP64)")
@@ -715,6 +765,7 @@ def main():
         "gi_lookup": gi,
         "shadow_mask": shadows,
         "mirror_reflections": mirror,
+        "light_shadows": lights,
     }
     print(json.dumps(out), flush=True)
 

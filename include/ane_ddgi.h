@@ -125,6 +125,14 @@ typedef struct ane_hit {
   float t, u, v; /* hit distance; barycentric weights of vertices 1 and 2 */
 } ane_hit;
 
+/* A point light as the ray-tracing passes read it: GPULight (GpuBuffers.h:69-72,
+ * bindless.glsl:21-25), 32 bytes. */
+#define ANE_MAX_LIGHTS 1024 /* MAX_NUM_LIGHTS, light_shadow_helpers.glsl:4 */
+typedef struct ane_light {
+  float world_pos[4]; /* xyz position, w range */
+  float color[4];     /* rgb colour, w enabled (>= 0.5) */
+} ane_light;
+
 typedef struct ane_scene_info {
   int64_t num_triangles;
   int32_t num_instances;
@@ -305,6 +313,27 @@ int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const flo
 int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, int32_t height,
                           const float* inv_view_proj16, const float* light_dir3, float* out_visibility,
                           uint64_t* counters2, void* stream);
+
+/* Point-light shadow moments (replaces LightShadowRT, LightShadowRayTracingPass.cpp:72-99
+ * with light_shadow.rgen:40-86, .rchit:29-32, .rmiss:12-15; and LightShadowSum,
+ * LightShadowSumPass.cpp:65-87 with light_shadow_sum.comp:16-71).  Each of the
+ * num_lights (<= 1024) lights casts 1024 rays from its position along the
+ * unrotated Fibonacci directions (rgen:27-38) over [0.1, range]; a ray's distance is
+ * its closest hit, or the range on a miss, or 0 for a disabled light (color[3] < 0.5).
+ *   dist_img     (DEVICE, nullable) R32F 32 x (32 num_lights) image:
+ *                ray y * 32 + x of light l at pixel (32 l + x, y);
+ *   moments_img  (DEVICE) RG32F 8 x (10 num_lights) image: for texel (tx, ty) of
+ *                light l's 8x8 octahedral tile, at pixel (10 l + 1 + tx, ty), the
+ *                mean and mean square distance weighted by pow(max(0, dot(dir,
+ *                octDir)), 50) over the rays whose weight is above 0.001, dir being
+ *                rounded through RGBA8 as the reference's direction image; the border
+ *                columns 10 l and 10 l + 9 are not written.
+ * The lights are read from HOST memory during the call; the images are ordered on
+ * `stream`.  counters2 (host, nullable) receives node visits and triangle tests
+ * (the stream is then synchronized).  Enabled lights need a finite position and
+ * range. */
+int ane_scene_light_shadows(ane_scene* scene, const ane_light* lights, int32_t num_lights,
+                            float* dist_img, float* moments_img, uint64_t* counters2, void* stream);
 
 /* ---- host-only helpers (no device work) ---- */
 /* Fibonacci-lattice ray directions rotated by mat3(rot16) (rgen:35-46,86-88). */

@@ -685,6 +685,109 @@ void ora_shadow_mask(const ora_scene* s, const float* depth, int32_t w, int32_t 
   if (counters) { counters[0] += cn; counters[1] += ct; }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Point-light shadow moments: light_shadow.rgen:27-86, light_shadow.rchit:   */
+/* 29-32, light_shadow.rmiss:12-15 (LightShadowRayTracingPass.cpp:17-100),    */
+/* light_shadow_sum.comp:16-71 (LightShadowSumPass.cpp:16-87), constants of   */
+/* light_shadow_helpers.glsl:1-4.                                             */
+/* ------------------------------------------------------------------------ */
+
+/* pow(x, 50.0) of light_shadow_sum.comp:58.  Contract (GLSL pow precision is
+ * implementation-defined): the binary32 product chain x^2, x^4, x^8, x^16,
+ * x^32, then (x^32 * x^16) * x^2. */
+static float pow50(float x)
+{
+  float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16;
+  return (x32 * x16) * x2;
+}
+
+/* The direction image is RGBA8_UNORM (LightShadowRayTracingPass.cpp:53): the
+ * store of (dir + 1) * 0.5 (light_shadow.rgen:83-85) and the load
+ * (light_shadow_sum.comp:56-57).  Contract: clamp to [0, 1], scale by 255,
+ * round to nearest even; the load is c / 255. */
+static float unorm8_round_trip(float v)
+{
+  float c = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  return rintf(c * 255.0f) / 255.0f;
+}
+
+void ora_light_shadow_weights(float* w)
+{
+  float dirs[3 * ORA_LIGHT_RAYS], td[64 * 3];
+  ora_fibonacci_dirs(ORA_LIGHT_RAYS, dirs);  /* worldDirFromFibonacciIndex, rgen:27-38 */
+  ora_texel_dirs(td);                        /* octUv / octDecode, sum.comp:45-47 */
+  for (int r = 0; r < ORA_LIGHT_RAYS; ++r) {
+    f3 q;
+    q.x = unorm8_round_trip((dirs[3 * r + 0] + 1.0f) * 0.5f) * 2.0f - 1.0f;
+    q.y = unorm8_round_trip((dirs[3 * r + 1] + 1.0f) * 0.5f) * 2.0f - 1.0f;
+    q.z = unorm8_round_trip((dirs[3 * r + 2] + 1.0f) * 0.5f) * 2.0f - 1.0f;
+    for (int t = 0; t < 64; ++t) {
+      float c = dot(q, ld3(td + 3 * t));
+      float wt = pow50(c > 0.0f ? c : 0.0f);  /* pow(max(0.0, dot(dir, octDir)), 50.0) */
+      w[t * ORA_LIGHT_RAYS + r] = wt > 0.001f ? wt : 0.0f;  /* sum.comp:59 */
+    }
+  }
+}
+
+void ora_light_shadows(const ora_scene* s, const float* lights, int32_t n, float* dist_img,
+                       float* moments_img, uint64_t* counters)
+{
+  float dirs[3 * ORA_LIGHT_RAYS];
+  ora_fibonacci_dirs(ORA_LIGHT_RAYS, dirs);
+  float* dist = (float*)malloc(sizeof(float) * (size_t)n * ORA_LIGHT_RAYS);
+  float* w = (float*)malloc(sizeof(float) * 64 * ORA_LIGHT_RAYS);
+  ora_light_shadow_weights(w);
+  uint64_t cn = 0, ct = 0;
+  const int64_t total = (int64_t)n * ORA_LIGHT_RAYS;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : cn, ct)
+  for (int64_t i = 0; i < total; ++i) {
+    const int32_t l = (int32_t)(i / ORA_LIGHT_RAYS), r = (int32_t)(i % ORA_LIGHT_RAYS);
+    const float* L = lights + 8 * (int64_t)l;  /* {worldPos (w = range), color (w = enabled)} */
+    float d;
+    if (L[7] < 0.5f) {
+      d = 0.0f;  /* disabled light: rgen:54-58 */
+    } else {
+      ora_hit h;
+      trace_one(s, mk(L[0], L[1], L[2]), ld3(dirs + 3 * r), 0.1f, L[3], 0, &h, &cn, &ct);
+      d = h.tri >= 0 ? h.t : L[3];  /* rchit: gl_HitTEXT; rmiss: maxDist = range */
+    }
+    dist[i] = d;
+    /* rayIndex = y * 32 + x at pixel (32 * l + x, y) of the distance image (rgen:43-49) */
+    if (dist_img)
+      dist_img[(int64_t)(r / ORA_LIGHT_SIDE) * ORA_LIGHT_SIDE * n + ORA_LIGHT_SIDE * l + r % ORA_LIGHT_SIDE] = d;
+  }
+  const int64_t row = 10 * (int64_t)n;  /* (LIGHT_SHADOW_OCT_SIZE + 2) texels per light */
+  for (int32_t l = 0; l < n; ++l)
+    for (int t = 0; t < 64; ++t) {
+      const int tx = t & 7, ty = t >> 3;
+      float acc_r = 0.0f, acc_g = 0.0f, acc_w = 0.0f;
+      if (lights[8 * (int64_t)l + 7] >= 0.5f) {
+        for (int x = 0; x < ORA_LIGHT_SIDE; ++x)      /* sum.comp:50-63: x outer, y inner */
+          for (int y = 0; y < ORA_LIGHT_SIDE; ++y) {
+            const int idx = y * ORA_LIGHT_SIDE + x;
+            const float wt = w[t * ORA_LIGHT_RAYS + idx];
+            if (wt > 0.001f) {
+              const float d = dist[(int64_t)l * ORA_LIGHT_RAYS + idx];
+              acc_r += wt * d;
+              acc_g += wt * d * d;
+              acc_w += wt;
+            }
+          }
+        if (acc_w > 1e-6f) {
+          acc_r = acc_r / acc_w;
+          acc_g = acc_g / acc_w;
+        }
+      }
+      /* texel (10 * l + 1 + tx, ty) of the RG32F moments image (sum.comp:29-31, 69) */
+      float* o = moments_img + 2 * ((int64_t)ty * row + 10 * (int64_t)l + 1 + tx);
+      o[0] = acc_r;
+      o[1] = acc_g;
+    }
+  free(w);
+  free(dist);
+  if (counters) { counters[0] += cn; counters[1] += ct; }
+}
+
 void ora_trace_bruteforce(const ora_scene* s, const float* orig, const float* dir,
                           int64_t nrays, float tmin, float tmax, int32_t any_hit, ora_hit* out)
 {

@@ -36,6 +36,9 @@
  *   ora_convolve_blend     float64 GEMM, uniform radiance => E = L, fixed point,
  *                          geometric series over k updates
  *   ora_translate          independent per-texel restatement of the two copies
+ *   ora_light_shadows      analytic distances from a light inside a closed box,
+ *                          all-miss case (mean = range), disabled lights,
+ *                          float64 weights and moments from the distances
  *
  * All arithmetic is IEEE binary32 with contraction off (-ffp-contract=off).
  */
@@ -140,6 +143,18 @@ void ora_scene_world_tris(const ora_scene* s, float* out);
  * pixel of a w x h depth buffer; ivp = column-major inverse view-projection */
 void ora_shadow_mask(const ora_scene* s, const float* depth, int32_t w, int32_t h, const float* ivp,
                      const float* light, float* out, uint64_t* counters);
+/* Point-light shadow moments (light_shadow.rgen/.rchit/.rmiss, light_shadow_sum.comp):
+ * lights = n x 8 floats {x, y, z, range, r, g, b, enabled (>= 0.5)}; 1024 rays per light.
+ * dist_img (nullable): R32F image 32 x (32 n), ray y * 32 + x of light l at (32 l + x, y).
+ * moments_img: RG32F image 8 x (10 n); texel (tx, ty) of light l at (10 l + 1 + tx, ty);
+ * the border columns are not written.  counters as ora_trace. */
+#define ORA_LIGHT_RAYS 1024
+#define ORA_LIGHT_SIDE 32
+void ora_light_shadows(const ora_scene* s, const float* lights, int32_t n, float* dist_img,
+                       float* moments_img, uint64_t* counters);
+/* the sum's weights: w[t][r] = pow(max(0, dot(dir_r, octDir_t)), 50), 0 unless > 0.001,
+ * with dir_r rounded through RGBA8_UNORM; [64][1024] */
+void ora_light_shadow_weights(float* w);
 void ora_scene_bvh(const ora_scene* s, uint64_t* keys, uint32_t* sorted_ids, int32_t* left,
                    int32_t* right, int32_t* parent, float* node_boxes);
 

@@ -42,6 +42,8 @@ def lib():
         L.ora_scene_bvh.argtypes = [vp] * 7
         L.ora_gi_lookup.argtypes = [vp, vp, vp, vp, i64, vp]
         L.ora_shadow_mask.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp]
+        L.ora_light_shadows.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.ora_light_shadow_weights.argtypes = [vp]
         L.ora_mirror_reflections.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp]
         L.ora_trace.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp, vp]
         L.ora_trace_bruteforce.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp]
@@ -126,6 +128,19 @@ class OracleScene:
         lib().ora_shadow_mask(self.h, ptr(d), w, h, ptr(ivp), ptr(light), ptr(out), ptr(ctr))
         return out, ctr
 
+    def light_shadows(self, lights, moments_fill=0.0):
+        """Point-light shadow moments (light_shadow.rgen, light_shadow_sum.comp) of
+        ``lights`` [n][8] = {x, y, z, range, r, g, b, enabled} -> (distance image
+        [32][32 n], moments image [8][10 n][2], counters[2]).  Border texels of the
+        moments image keep ``moments_fill``."""
+        L = np.ascontiguousarray(lights, np.float32).reshape(-1, 8)
+        n = len(L)
+        dist = np.zeros((32, 32 * n), np.float32)
+        mom = np.full((8, 10 * n, 2), moments_fill, np.float32)
+        ctr = np.zeros(2, np.uint64)
+        lib().ora_light_shadows(self.h, ptr(L), n, ptr(dist), ptr(mom), ptr(ctr))
+        return dist, mom, ctr
+
     def mirror_reflections(self, cfg, frame, atlas, depth, normal4, inv_view_proj16):
         """Mirror reflections of a depth + normal buffer -> ([h][w][4] (rgb, dist), counters[4])."""
         d = np.ascontiguousarray(depth, np.float32)
@@ -171,6 +186,13 @@ class OracleScene:
         lib().ora_shade_hit(self.h, _ref(cfg), _ref(frame), ptr(atlas), ptr(o), ptr(h), ptr(out),
                             ctypes.byref(shadowed), None)
         return out, shadowed.value
+
+
+def light_shadow_weights():
+    """The sum's [64][1024] weights (light_shadow_sum.comp:56-61)."""
+    out = np.zeros((64, 1024), np.float32)
+    lib().ora_light_shadow_weights(ptr(out))
+    return out
 
 
 def fibonacci_dirs(n):
