@@ -44,7 +44,7 @@ def parse():
     p.add_argument("--no-graphs", action="store_true", help="serve64: direct launches instead of captured graphs")
     p.add_argument("--streams", type=int, default=4, help="serve64: HIP streams of the server")
     p.add_argument("--frames", type=int, default=300, help="frames of the dynamic4m workload")
-    p.add_argument("--rebuild-every", type=int, default=1,
+    p.add_argument("--rebuild-every", type=int, default=4,
                    help="dynamic4m: full LBVH rebuild every K frames, refit in between (0 = refit only)")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU oracle baseline: repeat whole updates for at least this long")
