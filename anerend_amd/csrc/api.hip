@@ -374,6 +374,7 @@ struct ane_scene {
   uint32_t* d_arrive = nullptr;
   float* d_node_box = nullptr;
   uint32_t* d_cbounds = nullptr;
+  uint32_t* d_cpart = nullptr;  // per-block centroid-bound partials of a build
   uint32_t* d_scratch = nullptr;
   int32_t* d_order = nullptr;   // internal nodes bucketed by depth
   uint64_t* d_lvl_key = nullptr;
@@ -431,7 +432,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_tri_rec); dfree(s->d_inst_off);
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
-  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_scratch);
+  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_scratch);
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
@@ -498,9 +499,7 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   ++s->version;
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
-  const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-  ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
-  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cbounds, write_inst, st);
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cpart, s->d_cbounds, write_inst, st);
   launch_morton(s->d_world, n, s->d_cbounds, s->d_keys, s->d_ids, st);
   radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
   launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, st);
@@ -524,9 +523,8 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   ++s->version;
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
-  const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-  ANE_CHECK_HIP(hipMemcpyAsync(s->d_cbounds, init, sizeof(init), hipMemcpyHostToDevice, st));
-  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cbounds, false, st);
+  // a refit keeps the Morton order: no centroid bounds
+  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, nullptr, nullptr, false, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
   ANE_CHECK_HIP(hipGetLastError());
@@ -604,6 +602,7 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_arrive, n);
   ANE_A(d_node_box, (n > 1 ? n - 1 : 1) * 6);
   ANE_A(d_cbounds, 6);
+  ANE_A(d_cpart, flatten_partial_words((int)n));
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
   ANE_A(d_order, n);
   ANE_A(d_lvl_key, n);

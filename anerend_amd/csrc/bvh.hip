@@ -58,7 +58,7 @@ void fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
 // refits keep every instance's mesh (checked on the host) and read that record.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32_t* tri_inst,
-                                                 float4* tri_rec, uint32_t* cbounds,
+                                                 float4* tri_rec, uint32_t* part,
                                                  int32_t write_inst)
 {
   __shared__ uint32_t sb[6];
@@ -137,8 +137,35 @@ __global__ __launch_bounds__(256) void k_flatten(SceneK sc, float* world, uint32
     }
   }
   __syncthreads();
-  if (threadIdx.x < 3) atomicMin(&cbounds[threadIdx.x], sb[threadIdx.x]);
-  else if (threadIdx.x < 6) atomicMax(&cbounds[threadIdx.x], sb[threadIdx.x]);
+  // one partial per block (k_bounds_reduce combines them): a device-wide atomic per
+  // block on the same six words would serialize the whole launch
+  if (part && threadIdx.x < 6) part[threadIdx.x * gridDim.x + blockIdx.x] = sb[threadIdx.x];
+}
+
+// centroid bounds of the scene from the per-block partials of k_flatten: min of
+// part[0..2][*], max of part[3..5][*] (order-preserving uint encoding)
+__global__ __launch_bounds__(256) void k_bounds_reduce(const uint32_t* part, int nblocks, uint32_t* cbounds)
+{
+  __shared__ uint32_t red[6][256];
+  const int t = threadIdx.x;
+  uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  for (int i = t; i < nblocks; i += 256)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const uint32_t x = part[a * nblocks + i];
+      v[a] = a < 3 ? min(v[a], x) : max(v[a], x);
+    }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) red[a][t] = v[a];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+        red[a][t] = a < 3 ? min(red[a][t], red[a][t + s]) : max(red[a][t], red[a][t + s]);
+    __syncthreads();
+  }
+  if (t < 6) cbounds[t] = red[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -533,12 +560,16 @@ __global__ __launch_bounds__(1024) void k_fit_top(const float4* tris, NodeBuild*
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec,
+int flatten_partial_words(int ntri) { return 6 * (int)blocks_for(ntri > 0 ? ntri : 1, 256); }
+
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec, uint32_t* part,
                     uint32_t* cbounds, bool write_inst, hipStream_t s)
 {
   if (sc.ntri <= 0) return;
-  hipLaunchKernelGGL(k_flatten, dim3(blocks_for(sc.ntri, 256)), dim3(256), 0, s, sc, world,
-                     tri_inst, tri_rec, cbounds, (int32_t)write_inst);
+  const int nb = (int)blocks_for(sc.ntri, 256);
+  hipLaunchKernelGGL(k_flatten, dim3(nb), dim3(256), 0, s, sc, world, tri_inst, tri_rec,
+                     cbounds ? part : (uint32_t*)nullptr, (int32_t)write_inst);
+  if (cbounds) hipLaunchKernelGGL(k_bounds_reduce, dim3(1), dim3(256), 0, s, (const uint32_t*)part, nb, cbounds);
 }
 
 void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,

@@ -14,7 +14,10 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t s)
 int64_t radix_scratch_words(int64_t n);
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2, int64_t n,
                       uint32_t* scratch, hipStream_t s);
-void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec,
+// world-space triangles; with cbounds (a rebuild), also the centroid bounds through
+// per-block partials in part (flatten_partial_words(ntri) words)
+int flatten_partial_words(int ntri);
+void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* tri_rec, uint32_t* part,
                     uint32_t* cbounds, bool write_inst, hipStream_t s);
 void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
                    uint32_t* ids, hipStream_t s);
