@@ -375,6 +375,8 @@ struct ane_scene {
   float* d_node_box = nullptr;
   uint32_t* d_cbounds = nullptr;
   uint32_t* d_cpart = nullptr;  // per-block centroid-bound partials of a build
+  uint4* d_qnodes = nullptr;    // quantized traversal nodes (allocated when first used)
+  int32_t node_format = ANE_NODES_AUTO;
   uint32_t* d_scratch = nullptr;
   int32_t* d_order = nullptr;   // internal nodes bucketed by depth
   uint64_t* d_lvl_key = nullptr;
@@ -432,7 +434,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_tri_rec); dfree(s->d_inst_off);
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
-  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_scratch);
+  dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_qnodes); dfree(s->d_scratch);
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
@@ -493,6 +495,28 @@ static int upload_instances(ane_scene* s, const ane_instance* inst, hipStream_t 
   return ANE_OK;
 }
 
+// The traversal layout after a build or refit: quantized 32-byte nodes when asked
+// for, or by default for scenes of at least ANE_NODES_QUANTIZED_MIN_TRIS triangles,
+// whose 64-byte nodes outgrow the Infinity Cache; else the 64-byte nodes.
+static bool scene_wants_qnodes(const ane_scene* s)
+{
+  if (s->ntri < 2) return false;
+  if (s->node_format == ANE_NODES_QUANTIZED) return true;
+  return s->node_format == ANE_NODES_AUTO && (int64_t)s->ntri >= ANE_NODES_QUANTIZED_MIN_TRIS;
+}
+
+static int scene_apply_node_format(ane_scene* s, hipStream_t st)
+{
+  if (!scene_wants_qnodes(s)) {
+    s->view.qnodes = nullptr;
+    return ANE_OK;
+  }
+  if (!s->d_qnodes) ANE_TRY(dalloc(&s->d_qnodes, 2 * (int64_t)(s->ntri - 1), &s->bytes));
+  launch_quantize_nodes(s->d_nodes, s->ntri, s->d_qnodes, st);
+  s->view.qnodes = s->d_qnodes;
+  return ANE_OK;
+}
+
 // full LBVH build from the uploaded instances
 static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
 {
@@ -508,6 +532,7 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
                     s->d_hist, &s->level_begin, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
+  ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
   ANE_CHECK_HIP(hipStreamSynchronize(st));
@@ -527,6 +552,7 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, nullptr, nullptr, false, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
+  ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
   // the refit stays asynchronous; its device time is read when it is asked for
@@ -677,6 +703,19 @@ extern "C" int ane_scene_info_get(const ane_scene* s, ane_scene_info* out)
     s->build_ms_pending = false;
   }
   out->build_ms = s->build_ms;
+  out->node_format = s->view.qnodes ? ANE_NODES_QUANTIZED : ANE_NODES_FULL;
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_set_node_format(ane_scene* s, int32_t format, void* stream)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "NULL scene");
+  if (format != ANE_NODES_AUTO && format != ANE_NODES_FULL && format != ANE_NODES_QUANTIZED)
+    return fail(ANE_ERR_INVALID_ARG, "unknown node format");
+  s->node_format = format;
+  hipStream_t st = S(stream);
+  ANE_TRY(scene_apply_node_format(s, st));
+  ANE_CHECK_HIP(hipGetLastError());
   return ANE_OK;
 }
 

@@ -502,6 +502,58 @@ __device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, f
   nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
 }
 
+// Quantized nodes (layout next to NodeBuild): one thread per internal node, from
+// the fitted child boxes.  The same arithmetic restated on the CPU gives the same
+// bytes, so a traversal over these nodes visits what the CPU's does.
+__device__ int quant_exp(double extent)
+{
+  int e = -126;
+  if (extent > 0.0) {
+    int k;
+    (void)frexp(extent, &k);  // extent < 2^k
+    e = k - 9;                // 254 * 2^(k-9) < 2^k
+    while (254.0 * ldexp(1.0, e) < extent) ++e;
+    if (e < -126) e = -126;
+  }
+  return e;
+}
+
+__global__ __launch_bounds__(256) void k_quantize_nodes(const NodeBuild* nodes, int n, uint4* q)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const NodeBuild nd = nodes[i];
+  const float l0[3] = {nd.a.x, nd.a.z, nd.c.x}, h0[3] = {nd.a.y, nd.a.w, nd.c.y};
+  const float l1[3] = {nd.b.x, nd.b.z, nd.c.z}, h1[3] = {nd.b.y, nd.b.w, nd.c.w};
+  uint32_t by[12], eb = 0u, ob[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float o = l0[a] < l1[a] ? l0[a] : l1[a];
+    const float hi = h0[a] > h1[a] ? h0[a] : h1[a];
+    const int e = quant_exp((double)hi - (double)o);
+    const double step = ldexp(1.0, e);
+    by[0 + a] = (uint32_t)floor(((double)l0[a] - (double)o) / step);
+    by[3 + a] = (uint32_t)ceil(((double)h0[a] - (double)o) / step);
+    by[6 + a] = (uint32_t)floor(((double)l1[a] - (double)o) / step);
+    by[9 + a] = (uint32_t)ceil(((double)h1[a] - (double)o) / step);
+    eb |= (uint32_t)(e + 128) << (8 * a);
+    ob[a] = __float_as_uint(o);
+  }
+  const int L = nd.d.x, R = nd.d.y;
+  eb |= (L < 0 ? 1u : 0u) << 24 | (R < 0 ? 1u : 0u) << 25;
+  const uint32_t g = (uint32_t)(L >= 0 ? L : ~L);
+  q[2 * (int64_t)i] = make_uint4(ob[0], ob[1], ob[2], eb);
+  q[2 * (int64_t)i + 1] = make_uint4(by[0] | by[1] << 8 | by[2] << 16 | by[3] << 24,
+                                     by[4] | by[5] << 8 | by[6] << 16 | by[7] << 24,
+                                     by[8] | by[9] << 8 | by[10] << 16 | by[11] << 24, g);
+}
+
+void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t s)
+{
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_quantize_nodes, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, s, nodes, n, q);
+}
+
 // Depth (root = 0) of every internal node as a radix key, node id as value.
 __global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int n, uint64_t* key,
                                                     uint32_t* val)

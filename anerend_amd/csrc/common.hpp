@@ -94,6 +94,7 @@ struct SceneK {
   const float4* tri_rec;        // [ntri][4] of triangle id: (object-space p_j, vertex index j) j = 0..2, (instance)
   const uint32_t* inst_tri_off; // [ninst + 1]
   const float4* nodes;          // [ntri - 1][4] traversal nodes
+  const uint4* qnodes;          // [ntri - 1][2] quantized traversal nodes, or null (use nodes)
   const float4* tris;           // [ntri][3] sorted world triangles (v0.w = id bits)
   int32_t ntri;
   int32_t ninst;
@@ -108,6 +109,19 @@ struct NodeBuild {
   float4 a, b, c;
   int4 d;
 };
+
+// Quantized traversal node, 32 bytes (2 x uint4), for scenes whose 64-byte nodes
+// outgrow the caches:
+//   q0 = (o.x, o.y, o.z as bits, (ex+128) | (ey+128) << 8 | (ez+128) << 16 | leaf0 << 24 | leaf1 << 25)
+//   q1 = (12 bound bytes c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz, lowest byte first; split g)
+// o is the node box's lower corner; per axis 2^e is the smallest step (e >= -126)
+// with 254 * 2^e >= the node box extent.  Lower bounds are floor((v - o) / 2^e),
+// upper bounds ceil((v - o) / 2^e), exact in binary64; a bound decodes as
+// o + q * 2^e in binary32 (exact product, one rounding), which is monotone, so a
+// decoded box contains its exact child box.  The children are g and g + 1 (the
+// Karras split), leaves (~position) where the leaf flag is set.
+ANE_HD float qnode_step(uint32_t ebyte) { return __builtin_bit_cast(float, (ebyte - 128u + 127u) << 23); }
+ANE_HD float qnode_bound(float o, uint32_t q, float step) { return o + (float)q * step; }
 
 ANE_HD uint32_t mesh_of(uint32_t custom) { return (custom >> 10) & 0x3FFFu; }
 ANE_HD uint32_t mat_of(uint32_t custom) { return custom & 0x3FFu; }

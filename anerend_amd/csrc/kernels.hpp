@@ -14,6 +14,8 @@ void exclusive_scan(uint32_t* data, int64_t n, uint32_t* scratch, hipStream_t s)
 int64_t radix_scratch_words(int64_t n);
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2, int64_t n,
                       uint32_t* scratch, hipStream_t s);
+// quantized traversal nodes [n-1][2] from the fitted nodes (layout next to NodeBuild)
+void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t s);
 // world-space triangles; with cbounds (a rebuild), also the centroid bounds through
 // per-block partials in part (flatten_partial_words(ntri) words)
 int flatten_partial_words(int ntri);

@@ -258,7 +258,7 @@ __device__ __forceinline__ f3 hit_geom_pos(const SceneK& sc, int32_t id, float u
 //     shadow ray from the hit position toward the sun (any hit, rchit:44-66).
 //   RAYS = true: caller rays, closest hit or (ANY) any hit.
 // ---------------------------------------------------------------------------
-template <bool ANY, bool RAYS, bool COUNT>
+template <bool ANY, bool RAYS, bool COUNT, bool Q>
 __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_WAVES))) void k_trace(TraceArgs a)
 {
   constexpr int32_t kDone = 0x7FFFFFFF;  // node sentinel: the lane's traversal has ended
@@ -431,12 +431,36 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         if (COUNT) {
           if (sph) ++sn; else ++cn;
         }
-        const float4* nd = sc.nodes + 4 * (int64_t)node;
-        const float4 na = nd[0], nb = nd[1], nc = nd[2], nn = nd[3];
+        float4 na, nb, nc;
+        int32_t L, R;
+        if (Q) {
+          // quantized node: decode the two child boxes (layout next to NodeBuild)
+          const uint4* nq = sc.qnodes + 2 * (int64_t)node;
+          const uint4 q0 = nq[0], q1 = nq[1];
+          const float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
+          const float sx = qnode_step(q0.w & 0xFFu), sy = qnode_step((q0.w >> 8) & 0xFFu);
+          const float sz = qnode_step((q0.w >> 16) & 0xFFu);
+          na = make_float4(qnode_bound(ox, q1.x & 0xFFu, sx), qnode_bound(ox, q1.x >> 24, sx),
+                           qnode_bound(oy, (q1.x >> 8) & 0xFFu, sy), qnode_bound(oy, q1.y & 0xFFu, sy));
+          nb = make_float4(qnode_bound(ox, (q1.y >> 16) & 0xFFu, sx), qnode_bound(ox, (q1.z >> 8) & 0xFFu, sx),
+                           qnode_bound(oy, q1.y >> 24, sy), qnode_bound(oy, (q1.z >> 16) & 0xFFu, sy));
+          nc = make_float4(qnode_bound(oz, (q1.x >> 16) & 0xFFu, sz), qnode_bound(oz, (q1.y >> 8) & 0xFFu, sz),
+                           qnode_bound(oz, q1.z & 0xFFu, sz), qnode_bound(oz, q1.z >> 24, sz));
+          const int32_t g = (int32_t)q1.w;
+          L = (q0.w >> 24) & 1u ? ~g : g;
+          R = (q0.w >> 25) & 1u ? ~(g + 1) : g + 1;
+        } else {
+          const float4* nd = sc.nodes + 4 * (int64_t)node;
+          na = nd[0];
+          nb = nd[1];
+          nc = nd[2];
+          const float4 nn = nd[3];
+          L = __float_as_int(nn.x);
+          R = __float_as_int(nn.y);
+        }
         float tn0, tn1;
         const bool b0 = box_test(r, na.x, na.y, na.z, na.w, nc.x, nc.y, tmin, tbest, tn0);
         const bool b1 = box_test(r, nb.x, nb.y, nb.z, nb.w, nc.z, nc.w, tmin, tbest, tn1);
-        const int32_t L = __float_as_int(nn.x), R = __float_as_int(nn.y);
         const bool sw = b0 && b1 && tn1 < tn0;
         next = (b0 || b1) ? ((b0 && !sw) ? L : R) : kPop;
         if (b0 && b1) {
@@ -542,7 +566,7 @@ int trace_lanes_per_launch()
   int dev = 0, cus = 0, per = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<false, false, false>, kTB, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<false, false, false, false>, kTB, 0);
   if (cus <= 0) cus = 256;
   if (per <= 0) per = 8;
   g_lanes = cus * per * kTB;
@@ -554,7 +578,11 @@ void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipSt
   const int blocks = trace_lanes_per_launch() / kTB;
   TraceArgs b = a;
   b.total_lanes = blocks * kTB;
-#define ANE_TR(S, R, C) hipLaunchKernelGGL((k_trace<S, R, C>), dim3(blocks), dim3(kTB), 0, s, b)
+#define ANE_TR(S, R, C)                                                                             \
+  do {                                                                                              \
+    if (b.sc.qnodes) hipLaunchKernelGGL((k_trace<S, R, C, true>), dim3(blocks), dim3(kTB), 0, s, b);  \
+    else hipLaunchKernelGGL((k_trace<S, R, C, false>), dim3(blocks), dim3(kTB), 0, s, b);             \
+  } while (0)
   if (rays) {
     if (any_hit) { if (count) ANE_TR(true, true, true); else ANE_TR(true, true, false); }
     else { if (count) ANE_TR(false, true, true); else ANE_TR(false, true, false); }

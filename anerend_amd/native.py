@@ -27,7 +27,7 @@ class AneError(RuntimeError):
 class SceneInfo(ctypes.Structure):
     _fields_ = [("num_triangles", ctypes.c_int64), ("num_instances", ctypes.c_int32),
                 ("bvh_depth", ctypes.c_int32), ("device_bytes", ctypes.c_int64),
-                ("build_ms", ctypes.c_float)]
+                ("build_ms", ctypes.c_float), ("node_format", ctypes.c_int32)]
 
 
 STAGES = ("setup", "trace", "shadow", "shade", "convolve", "translate")
@@ -86,6 +86,7 @@ _SIGS = [
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
     ("ane_scene_shadow_mask", ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("ane_scene_light_shadows", ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    ("ane_scene_set_node_format", ctypes.c_int, [_vp, _i32, _vp]),
     ("ane_scene_trace_rays", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
     ("ane_ddgi_ray_directions", ctypes.c_int, [_i32, _vp, _vp]),
 ]
@@ -185,6 +186,12 @@ class DeviceScene:
         s = SceneInfo()
         check(load().ane_scene_info_get(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    NODES_AUTO, NODES_FULL, NODES_QUANTIZED = 0, 1, 2
+
+    def set_node_format(self, fmt, stream=None):
+        """Traversal node layout: NODES_AUTO, NODES_FULL or NODES_QUANTIZED."""
+        check(load().ane_scene_set_node_format(self._h, int(fmt), stream))
 
     def refit(self, instances, stream=None):
         inst = np.ascontiguousarray(instances)

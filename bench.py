@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--frames", type=int, default=300, help="frames of the dynamic4m workload")
     p.add_argument("--rebuild-every", type=int, default=4,
                    help="dynamic4m: full LBVH rebuild every K frames, refit in between (0 = refit only)")
+    p.add_argument("--node-format", default="auto", choices=["auto", "full", "quantized"],
+                   help="traversal node layout (auto: quantized from 2M triangles up)")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU oracle baseline: repeat whole updates for at least this long")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -453,6 +455,7 @@ def bench_dynamic(args):
     insts = [scenegen.dynamic_instances(scene, vel, f) for f in range(frames + 1)]
     frs = [scenegen.dynamic_frame(5, f, cfg) for f in range(frames + 1)]
     ds = native.DeviceScene(scene)
+    ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[args.node_format])
     info = ds.info()
     log(f"[bench] {scene.num_tris} triangles, LBVH {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
     grid = native.ProbeGrid(cfg)
@@ -498,6 +501,7 @@ def bench_dynamic(args):
                    "rebuild_every": K,
                    "probes": P, "rays_per_probe": cfg.rays_per_probe,
                    "triangles": scene.num_tris, "conv_precision": "fp8",
+                   "bvh_nodes": "quantized 32 B" if ds.info()["node_format"] == 2 else "full 64 B",
                    "probe_rays_per_s": round(P * cfg.rays_per_probe / (ms / 1000.0), 1)},
         "translations": moves,
         "bvh_update": (f"rebuild every {K} frames, refit in between" if K > 1 else
@@ -633,6 +637,7 @@ def main():
     cfg.precision = prec
     log(f"[bench] scene: {scene.num_tris} triangles, {len(scene.instances)} instances")
     ds = native.DeviceScene(scene)
+    ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[args.node_format])
     info = ds.info()
     log(f"[bench] LBVH on GPU: {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
     grid = native.ProbeGrid(cfg)
@@ -748,6 +753,7 @@ def main():
         "config": {"workload": "terrain1m: 32x16x32 probes x 512 rays, 512^2 value-noise terrain "
                                "+ 10k box/icosphere instances",
                    "probes": P, "rays_per_probe": N, "triangles": scene.num_tris,
+                   "bvh_nodes": "quantized 32 B" if ds.info()["node_format"] == 2 else "full 64 B",
                    "conv_precision": args.precision, "bvh_depth": info["bvh_depth"],
                    "lbvh_build_ms": round(info["build_ms"], 3),
                    "atlas_bytes": atlas_bytes},

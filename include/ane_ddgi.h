@@ -139,7 +139,14 @@ typedef struct ane_scene_info {
   int32_t bvh_depth;      /* max root-to-leaf edge count of the LBVH */
   int64_t device_bytes;   /* HBM held by the scene */
   float build_ms;         /* last build or refit, device time */
+  int32_t node_format;    /* traversal layout in use: ANE_NODES_FULL or ANE_NODES_QUANTIZED */
 } ane_scene_info;
+
+/* Traversal node layouts (ane_scene_set_node_format). */
+#define ANE_NODES_AUTO 0      /* quantized from ANE_NODES_QUANTIZED_MIN_TRIS triangles up (default) */
+#define ANE_NODES_FULL 1      /* 64 bytes: binary32 child boxes */
+#define ANE_NODES_QUANTIZED 2 /* 32 bytes: 8-bit child bounds, rounded outward */
+#define ANE_NODES_QUANTIZED_MIN_TRIS 2000000
 
 typedef struct ane_ddgi_stats {
   /* device time per stage from hipEvent pairs, ms: the last call and the mean
@@ -334,6 +341,18 @@ int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, i
  * range. */
 int ane_scene_light_shadows(ane_scene* scene, const ane_light* lights, int32_t num_lights,
                             float* dist_img, float* moments_img, uint64_t* counters2, void* stream);
+
+/* Traversal layout of a scene's hierarchy (no counterpart in the reference, whose
+ * acceleration structures are the driver's).  ANE_NODES_FULL keeps 64-byte nodes
+ * with binary32 child boxes.  ANE_NODES_QUANTIZED stores 32-byte nodes: each child
+ * bound is an 8-bit offset from the node box's lower corner in a power-of-two step,
+ * rounded outward, so decoded boxes contain the exact ones.  That halves the node
+ * bytes for about 2-3% more node visits, which pays once the nodes no longer fit
+ * the caches.  ANE_NODES_AUTO (the default) picks quantized nodes from
+ * ANE_NODES_QUANTIZED_MIN_TRIS triangles up.  Hits are identical in every layout;
+ * traversal counters count the visits of the layout in use.  The nodes are
+ * re-encoded on `stream`, and after every refit or rebuild. */
+int ane_scene_set_node_format(ane_scene* scene, int32_t format, void* stream);
 
 /* ---- host-only helpers (no device work) ---- */
 /* Fibonacci-lattice ray directions rotated by mat3(rot16) (rgen:35-46,86-88). */

@@ -188,6 +188,7 @@ struct ora_scene {
   float* box;          /* [ntri-1][6] internal node boxes */
   float* leaf_box;     /* [ntri][6] in sorted order */
   int32_t max_depth;
+  float* qbox;         /* [ntri-1][12] decoded quantized child boxes (null: exact boxes) */
 };
 
 static inline uint32_t mesh_of(const ora_instance* in) { return (in->custom_index_and_mask >> 10) & 0x3FFFu; }
@@ -336,6 +337,71 @@ static void child_box(const ora_scene* s, int32_t c, float* lo, float* hi)
   memcpy(hi, b + 3, 12);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Quantized nodes (the library's 32-byte node layout, see                   */
+/* ane_scene_set_node_format).  Node i keeps its box's lower corner o (binary32)  */
+/* and per axis the smallest step 2^e (e >= -126) with 254 * 2^e >= the box   */
+/* extent.  A child bound v is stored as q = floor((v - o) / 2^e) (lower) or   */
+/* ceil((v - o) / 2^e) (upper), both exact in binary64, 0 <= q <= 254, and is  */
+/* decoded in binary32 as o + q * 2^e (the product is exact, the sum rounds  */
+/* to nearest).  Rounding is monotone, so the decoded box contains the exact  */
+/* child box, and the traversal tests the decoded boxes.                      */
+/* ------------------------------------------------------------------------ */
+static int quant_exp(double extent)
+{
+  int e = -126;
+  if (extent > 0.0) {
+    int k;
+    (void)frexp(extent, &k);  /* extent < 2^k */
+    e = k - 9;                /* 254 * 2^(k-9) < 2^k */
+    while (254.0 * ldexp(1.0, e) < extent) ++e;
+    if (e < -126) e = -126;
+  }
+  return e;
+}
+
+static float quant_dec(float o, int q, int e) { return o + (float)q * ldexpf(1.0f, e); }
+
+static void quantize_node(const float* l0, const float* h0, const float* l1, const float* h1, float* out12)
+{
+  for (int a = 0; a < 3; ++a) {
+    const float o = l0[a] < l1[a] ? l0[a] : l1[a];
+    const float hi = h0[a] > h1[a] ? h0[a] : h1[a];
+    const int e = quant_exp((double)hi - (double)o);
+    const double step = ldexp(1.0, e);
+    out12[0 + a] = quant_dec(o, (int)floor(((double)l0[a] - (double)o) / step), e);
+    out12[3 + a] = quant_dec(o, (int)ceil(((double)h0[a] - (double)o) / step), e);
+    out12[6 + a] = quant_dec(o, (int)floor(((double)l1[a] - (double)o) / step), e);
+    out12[9 + a] = quant_dec(o, (int)ceil(((double)h1[a] - (double)o) / step), e);
+  }
+}
+
+static void quantize_nodes(ora_scene* s)
+{
+  if (!s->qbox || s->ntri < 2) return;
+  for (int64_t i = 0; i < s->ntri - 1; ++i) {
+    float l0[3], h0[3], l1[3], h1[3];
+    child_box(s, s->left[i], l0, h0);
+    child_box(s, s->right[i], l1, h1);
+    quantize_node(l0, h0, l1, h1, s->qbox + 12 * i);
+  }
+}
+
+void ora_scene_set_node32(ora_scene* s, int32_t on)
+{
+  free(s->qbox);
+  s->qbox = NULL;
+  if (on && s->ntri >= 2) {
+    s->qbox = (float*)malloc(sizeof(float) * 12 * (size_t)(s->ntri - 1));
+    quantize_nodes(s);
+  }
+}
+
+void ora_scene_qboxes(const ora_scene* s, float* out)
+{
+  if (s->qbox) memcpy(out, s->qbox, sizeof(float) * 12 * (size_t)(s->ntri - 1));
+}
+
 /* Bottom-up boxes: parent box = componentwise min/max of child boxes (exact). */
 static void fit_boxes(ora_scene* s)
 {
@@ -470,7 +536,7 @@ void ora_scene_destroy(ora_scene* s)
   free(s->vertices); free(s->indices); free(s->meshes); free(s->mats); free(s->inst);
   free(s->inv_xform); free(s->inst_tri_offset); free(s->tri_inst); free(s->world);
   free(s->keys); free(s->ids); free(s->left); free(s->right); free(s->parent);
-  free(s->leaf_parent); free(s->box); free(s->leaf_box);
+  free(s->leaf_parent); free(s->box); free(s->leaf_box); free(s->qbox);
   free(s);
 }
 
@@ -484,6 +550,7 @@ int ora_scene_refit(ora_scene* s, const ora_instance* inst, int32_t ninst)
   memcpy(s->inst, inst, sizeof(ora_instance) * (size_t)ninst);
   flatten(s);
   fit_boxes(s);
+  quantize_nodes(s);
   return 0;
 }
 
@@ -616,8 +683,13 @@ static void trace_one(const ora_scene* s, f3 o, f3 d, float tmin, float tmax, in
       ++cn;
       int32_t L = s->left[node], R = s->right[node];
       float l0[3], h0[3], l1[3], h1[3], tn0, tn1;
-      child_box(s, L, l0, h0);
-      child_box(s, R, l1, h1);
+      if (s->qbox) {
+        const float* q = s->qbox + 12 * (int64_t)node;
+        memcpy(l0, q, 12); memcpy(h0, q + 3, 12); memcpy(l1, q + 6, 12); memcpy(h1, q + 9, 12);
+      } else {
+        child_box(s, L, l0, h0);
+        child_box(s, R, l1, h1);
+      }
       int b0 = box_test(&r, l0, h0, tmin, best.t, &tn0);
       int b1 = box_test(&r, l1, h1, tmin, best.t, &tn1);
       if (b0 || b1) {

@@ -36,6 +36,8 @@
  *   ora_convolve_blend     float64 GEMM, uniform radiance => E = L, fixed point,
  *                          geometric series over k updates
  *   ora_translate          independent per-texel restatement of the two copies
+ *   ora_scene_set_node32   decoded boxes contain the exact child boxes and stay within
+ *                          one step of them; traversal with them == brute force
  *   ora_light_shadows      analytic distances from a light inside a closed box,
  *                          all-miss case (mean = range), disabled lights,
  *                          float64 weights and moments from the distances
@@ -155,6 +157,12 @@ void ora_light_shadows(const ora_scene* s, const float* lights, int32_t n, float
 /* the sum's weights: w[t][r] = pow(max(0, dot(dir_r, octDir_t)), 50), 0 unless > 0.001,
  * with dir_r rounded through RGBA8_UNORM; [64][1024] */
 void ora_light_shadow_weights(float* w);
+/* Quantized 32-byte nodes, as the library stores them for large scenes (8-bit child
+ * bounds from the node box's lower corner in power-of-two steps, rounded outward):
+ * on = 1 makes the traversal test the decoded boxes (refits keep them current).
+ * qboxes: [ntri-1][12] decoded boxes {child0 lo, hi, child1 lo, hi}. */
+void ora_scene_set_node32(ora_scene* s, int32_t on);
+void ora_scene_qboxes(const ora_scene* s, float* out);
 void ora_scene_bvh(const ora_scene* s, uint64_t* keys, uint32_t* sorted_ids, int32_t* left,
                    int32_t* right, int32_t* parent, float* node_boxes);
 

@@ -43,6 +43,8 @@ def lib():
         L.ora_gi_lookup.argtypes = [vp, vp, vp, vp, i64, vp]
         L.ora_shadow_mask.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp]
         L.ora_light_shadows.argtypes = [vp, vp, i32, vp, vp, vp]
+        L.ora_scene_set_node32.argtypes = [vp, i32]
+        L.ora_scene_qboxes.argtypes = [vp, vp]
         L.ora_light_shadow_weights.argtypes = [vp]
         L.ora_mirror_reflections.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp]
         L.ora_trace.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp, vp]
@@ -127,6 +129,16 @@ class OracleScene:
         ctr = np.zeros(2, np.uint64)
         lib().ora_shadow_mask(self.h, ptr(d), w, h, ptr(ivp), ptr(light), ptr(out), ptr(ctr))
         return out, ctr
+
+    def set_node32(self, on=True):
+        """Traverse the decoded boxes of the library's quantized 32-byte nodes."""
+        lib().ora_scene_set_node32(self.h, int(bool(on)))
+
+    def qboxes(self):
+        """Decoded quantized child boxes [ntri-1][2][2][3] (child, lo/hi, xyz)."""
+        out = np.zeros((max(self.num_tris - 1, 0), 12), np.float32)
+        lib().ora_scene_qboxes(self.h, ptr(out))
+        return out.reshape(-1, 2, 2, 3)
 
     def light_shadows(self, lights, moments_fill=0.0):
         """Point-light shadow moments (light_shadow.rgen, light_shadow_sum.comp) of

@@ -193,3 +193,62 @@ def test_degenerate_directions_are_no_rays(osc):
         assert np.all(h["tri"] == -1) and np.all(ctr == 0)
     h, ctr = osc.trace(o[:1], np.array([[0, -1, 0]], np.float32))
     assert ctr[0] > 0
+
+
+def _exact_child_boxes(osc):
+    """Exact boxes [n-1][2][2][3] of every internal node's two children."""
+    b = osc.bvh()
+    tris = osc.world_tris().reshape(-1, 3, 3)
+    leaf = np.stack([tris.min(1), tris.max(1)], 1)[b["ids"]]      # sorted order
+    node = b["boxes"].reshape(-1, 2, 3)
+    out = []
+    for side in ("left", "right"):
+        c = b[side]
+        out.append(np.where((c < 0)[:, None, None], leaf[np.where(c < 0, ~c, 0)], node[np.where(c >= 0, c, 0)]))
+    return np.stack(out, 1)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_quantized_nodes_are_conservative_and_tight(name):
+    osc = O.OracleScene(SCENES[name]())
+    osc.set_node32(True)
+    q = osc.qboxes().astype(np.float64)
+    ex = _exact_child_boxes(osc).astype(np.float64)
+    assert np.all(q[:, :, 0] <= ex[:, :, 0]) and np.all(q[:, :, 1] >= ex[:, :, 1])
+    # at most one step (extent / 127 of the node box, or the smallest step) plus one rounding
+    lo = np.minimum(ex[:, 0, 0], ex[:, 1, 0])
+    hi = np.maximum(ex[:, 0, 1], ex[:, 1, 1])
+    step = np.maximum((hi - lo) / 127.0, 2.0 ** -126)[:, None, :]
+    slack = step + np.abs(lo)[:, None, :] * 2.0 ** -23
+    assert np.all(ex[:, :, 0] - q[:, :, 0] <= slack) and np.all(q[:, :, 1] - ex[:, :, 1] <= slack)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_quantized_traversal_equals_brute_force(name):
+    osc = O.OracleScene(SCENES[name]())
+    o, d = _random_rays(6000, 11, [-5, -1, -5], [5, 4, 5])
+    _, c_exact = osc.trace(o, d)
+    osc.set_node32(True)
+    for any_hit in (False, True):
+        h, c = osc.trace(o, d, any_hit=any_hit)
+        hb = osc.trace(o, d, any_hit=any_hit, brute=True)
+        if any_hit:
+            assert np.array_equal(h["tri"] >= 0, hb["tri"] >= 0)
+        else:
+            assert np.array_equal(h, hb)
+            assert c[0] >= c_exact[0]      # looser boxes visit at least as many nodes
+
+
+def test_quantized_nodes_follow_a_refit():
+    scene = scenegen.room_scene(1, num_boxes=16)
+    osc = O.OracleScene(scene)
+    osc.set_node32(True)
+    inst = scene.instances.copy()
+    inst["transform"][1:, 3] += 0.25
+    osc.refit(inst)
+    q = osc.qboxes().astype(np.float64)
+    ex = _exact_child_boxes(osc).astype(np.float64)
+    assert np.all(q[:, :, 0] <= ex[:, :, 0]) and np.all(q[:, :, 1] >= ex[:, :, 1])
+    o, d = _random_rays(3000, 2, [-4, 0, -4], [4, 3, 4])
+    h, _ = osc.trace(o, d)
+    assert np.array_equal(h, osc.trace(o, d, brute=True))
