@@ -12,7 +12,9 @@
 //                    A probe ray's closest hit (rgen:90-106) is followed in the same
 //                    lane by its any-hit shadow ray from the hit position (rchit:30-66),
 //                    against the triangles of its sun-grid cell when the context has a
-//                    current grid (sungrid.hip), else through the hierarchy
+//                    current grid (sungrid.hip), else through the hierarchy.  Q = true
+//                    reads the scene's quantized 32-byte nodes (large scenes), else the
+//                    64-byte nodes; caller rays may carry a per-ray t_max
 //   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
 //                    indirect light from the previous atlas, emissive, sky on miss
 //                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
@@ -21,6 +23,8 @@
 //   k_shadow_*       RT hard shadows of a depth buffer (shadow.rgen), through the
 //                    scene's sun grid (k_shadow_mask_grid) or caller-ray traversal
 //   k_mirror_*       mirror reflections (mirror_reflections.rgen/.rchit/.rmiss)
+//   k_light_*        point-light shadow moments (light_shadow.rgen/.rchit/.rmiss and
+//                    light_shadow_sum.comp)
 #include "common.hpp"
 #include "kernels.hpp"
 
