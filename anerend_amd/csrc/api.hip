@@ -714,8 +714,12 @@ extern "C" int ane_scene_set_node_format(ane_scene* s, int32_t format, void* str
     return fail(ANE_ERR_INVALID_ARG, "unknown node format");
   s->node_format = format;
   hipStream_t st = S(stream);
+  // work already queued on the scene's streams must not see the nodes change under it,
+  // and work queued afterwards on any stream must see them complete
+  ANE_CHECK_HIP(hipDeviceSynchronize());
   ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
   return ANE_OK;
 }
 

@@ -350,8 +350,9 @@ int ane_scene_light_shadows(ane_scene* scene, const ane_light* lights, int32_t n
  * bytes for about 2-3% more node visits, which pays once the nodes no longer fit
  * the caches.  ANE_NODES_AUTO (the default) picks quantized nodes from
  * ANE_NODES_QUANTIZED_MIN_TRIS triangles up.  Hits are identical in every layout;
- * traversal counters count the visits of the layout in use.  The nodes are
- * re-encoded on `stream`, and after every refit or rebuild. */
+ * traversal counters count the visits of the layout in use.  The call waits for
+ * the device to be idle, re-encodes the nodes on `stream` and returns when they are
+ * complete; refits and rebuilds re-encode them on their own stream. */
 int ane_scene_set_node_format(ane_scene* scene, int32_t format, void* stream);
 
 /* ---- host-only helpers (no device work) ---- */
