@@ -27,14 +27,18 @@ def _sessions():
     return specs
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_mixed_equals_alone(graphs):
+@pytest.mark.parametrize("graphs,nodes", [(True, "full"), (False, "full"), (True, "quantized")])
+def test_mixed_equals_alone(graphs, nodes):
     """With captured graphs every session's graph is replayed (updated in place)
     once per request, with the eager per-scene updates of the other sessions and
     host synchronizations in between; each replay feeds the next update through
-    the atlas, so the final atlas checks them all."""
+    the atlas, so the final atlas checks them all.  The quantized case runs every
+    scene on 32-byte nodes."""
     specs = _sessions()
     scenes = [native.DeviceScene(s) for s, _, _ in specs]
+    if nodes == "quantized":
+        for ds in scenes:
+            ds.set_node_format(native.DeviceScene.NODES_QUANTIZED)
     srv = serving.Server(hbm_budget_bytes=64 << 30, max_inflight=4, num_streams=3, capture_graphs=graphs)
     ids = [srv.add_session(ds, cfg) for ds, (_, cfg, _) in zip(scenes, specs)]
     # interleaved requests: 3 full updates per session, submitted back to back
@@ -46,7 +50,7 @@ def test_mixed_equals_alone(graphs):
             fr = scenegen.make_frame(11 + k, camera=cam, config=cfg, iteration=it)
             frames.setdefault(k, []).append(fr)
             tickets.append(srv.submit(sid, fr))
-    print(f"serve (graphs {graphs}): {len(tickets)} requests submitted", flush=True)
+    print(f"serve (graphs {graphs}, {nodes} nodes): {len(tickets)} requests submitted", flush=True)
     res = [srv.poll(t, timeout_ms=-1) for t in tickets]
     assert all(r["status"] == 0 for r in res)
     st = srv.stats()
