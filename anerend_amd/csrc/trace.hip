@@ -69,12 +69,19 @@ static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a 
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 // The quantized-node traversal serves scenes whose nodes miss the caches: its
 // lanes wait on memory rather than on VALU issue, so it refills earlier and claims
-// larger chunks (4.04M-triangle scene: trace 10.0 -> 9.0 ms against 32 / 128)
+// larger chunks (4.04M-triangle scene: trace 10.0 -> 9.0 ms against 32 / 128), and
+// checks for refills every 7 steps (9.09 -> 8.98 ms against 5)
 #ifndef ANE_TRACE_CHUNK_Q
 #define ANE_TRACE_CHUNK_Q 256
 #endif
 #ifndef ANE_TRACE_REFILL_Q
 #define ANE_TRACE_REFILL_Q 16
+#endif
+#ifndef ANE_TRACE_UNROLL_Q
+#define ANE_TRACE_UNROLL_Q 7  // steps per refill check (quantized nodes)
+#endif
+#ifndef ANE_TRACE_LEAFBATCH_Q
+#define ANE_TRACE_LEAFBATCH_Q ANE_TRACE_LEAFBATCH
 #endif
 static constexpr int kChunkQ = ANE_TRACE_CHUNK_Q;
 static constexpr int kRefillQ = ANE_TRACE_REFILL_Q;
@@ -290,7 +297,8 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   // visiting internal nodes with the older best distance (more visits, the same
   // closest hit).
   constexpr bool SPEC = ANE_TRACE_SPEC && !RAYS && !COUNT;
-  constexpr int kLeafBatch = RAYS ? kLeafBatchRays : kLeafBatchProbe;
+  constexpr int kLeafBatch = RAYS ? kLeafBatchRays : (Q ? ANE_TRACE_LEAFBATCH_Q : kLeafBatchProbe);
+  constexpr int kUnroll = Q ? ANE_TRACE_UNROLL_Q : ANE_TRACE_UNROLL;
   // probe shadow rays test the triangles of their sun-grid cell (parked one at a
   // time as batch leaves) instead of traversing the hierarchy
   const bool sgrid = !RAYS && kLeafBatch > 0 && a.sg.list != nullptr && a.sg.params->valid != 0u;
@@ -414,7 +422,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
     }
     // ---- traversal steps: only node, sp, the best hit and the budget change ----
 #pragma unroll
-    for (int u = 0; u < ANE_TRACE_UNROLL; ++u) {
+    for (int u = 0; u < kUnroll; ++u) {
     const bool go = ray >= 0 && node != kDone &&
                     (kLeafBatch == 0 || (node != kWait && ((SPEC && node != kGridRay) ? !(node < 0 && pend != 0)
                                                                                          : pend == 0)));
