@@ -56,6 +56,28 @@ def parse():
     return p.parse_args()
 
 
+def dynamic_cpu_baseline(scene, vel, cfg, frames=1):
+    """The oracle on a bounded sample of the dynamic workload: ``frames`` whole frames
+    (instance refit + full FP32 probe update of all 131,072 probes), timed on this
+    host's cores.  The per-frame figure extrapolates to the 300-frame run."""
+    from anerend_amd import scenegen
+    from oracle import oracle as O
+    t0 = time.time()
+    osc = O.OracleScene(scene)
+    build_s = time.time() - t0
+    atlas = O.new_atlas(cfg)
+    t0 = time.time()
+    for f in range(frames):
+        osc.refit(scenegen.dynamic_instances(scene, vel, f))
+        osc.update(cfg, scenegen.dynamic_frame(5, f, cfg), atlas, want_rays=False)
+    dt = time.time() - t0
+    return {"value": round(dt * 1000.0 / frames, 1), "unit": "ms/frame", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"{frames} frame(s) of the same workload (refit + full probe update, "
+                      f"{frames * cfg.probes_x * cfg.probes_y * cfg.probes_z * cfg.rays_per_probe} rays) in "
+                      f"{dt:.2f} s, extrapolated per frame; CPU LBVH build {build_s:.2f} s not counted"}
+
+
 def cpu_baseline(scene, cfg, frame_at, min_seconds):
     """The oracle (C, OpenMP over rays and probes) on a bounded sample of the same
     workload: whole probe updates, repeated until ``min_seconds`` have passed."""
@@ -489,6 +511,11 @@ def bench_dynamic(args):
     ref_grid = native.ProbeGrid(scenegen.dynamic_config(PRECISION_FP32))
     err = fp64_conv_error(grid, ref_grid, ds, frs[frames], cfg)
     log(f"[bench] FP8 error vs FP64: {err}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle on one frame of the same workload")
+        cpu = dynamic_cpu_baseline(scene, vel, scenegen.dynamic_config(PRECISION_FP32))
+        log(f"[bench] cpu baseline {json.dumps(cpu)}")
     P = cfg.probes_x * cfg.probes_y * cfg.probes_z
     out = {
         "metric": "ms per frame (dynamic 4M-triangle scene: refit + translation + full probe update)",
@@ -509,6 +536,7 @@ def bench_dynamic(args):
         "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items() if k != "shadow"},
         "last_bvh_update_ms": round(refit_ms, 4),
         "fp8_error_vs_fp64": err,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
