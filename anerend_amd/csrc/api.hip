@@ -418,7 +418,7 @@ struct ane_scene {
   float* d_lt_dist = nullptr;
   float* d_lt_dirs = nullptr;
   float* d_lt_wt = nullptr;       // [lt_kmax][64] weights of each texel's rays, summation order
-  int32_t* d_lt_idx = nullptr;    // [lt_kmax][64] their ray indices
+  int32_t* d_lt_idx = nullptr;    // [lt_kmax][64] their ray indices (lt_kmax 0: not set up yet)
   uint32_t* d_lt_order = nullptr; // [1024] ray traced in each slot (direction-coherent order)
   int32_t lt_kmax = 0;
   uint32_t* d_lt_work = nullptr;
@@ -1253,7 +1253,8 @@ extern "C" int ane_ddgi_mirror_reflections(ane_ddgi_ctx* c, const ane_scene* s, 
     }
     c->mr_px = n;
   }
-  if (!c->d_mr_work) {
+  if (!c->d_mr_work || !c->d_mr_stats || !c->d_mr_ov) {  // all three, or redo them
+    dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov);
     ANE_TRY(dalloc(&c->d_mr_work, 2, nullptr));
     ANE_TRY(dalloc(&c->d_mr_stats, 4, nullptr));
     ANE_TRY(dalloc(&c->d_mr_ov, 1, nullptr));
@@ -1493,7 +1494,8 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
     ANE_TRY(dalloc(&s->d_sh_hits, n, nullptr));
     s->shadow_px = n;
   }
-  if (!s->d_sh_work) {
+  if (!s->d_sh_work || !s->d_sh_stats || !s->d_sh_ov) {  // all three, or redo them
+    dfree(s->d_sh_work); dfree(s->d_sh_stats); dfree(s->d_sh_ov);
     ANE_TRY(dalloc(&s->d_sh_work, 1, nullptr));
     ANE_TRY(dalloc(&s->d_sh_stats, 2, nullptr));
     ANE_TRY(dalloc(&s->d_sh_ov, 1, nullptr));
@@ -1571,7 +1573,9 @@ extern "C" int ane_scene_light_shadows(ane_scene* s, const ane_light* lights, in
   }
   hipStream_t st = S(stream);
   const int64_t m = (int64_t)num_lights * kLightRays;
-  if (!s->d_lt_wt) {
+  if (s->lt_kmax == 0) {  // set last: a setup that failed part-way is redone from scratch
+    dfree(s->d_lt_order); dfree(s->d_lt_dirs); dfree(s->d_lt_wt); dfree(s->d_lt_idx);
+    dfree(s->d_lt_work); dfree(s->d_lt_stats); dfree(s->d_lt_ov);
     std::vector<float> dirs((size_t)kLightRays * 3), wt((size_t)kLightRays * 64);
     fibonacci_dirs(kLightRays, dirs.data());  // worldDirFromFibonacciIndex, light_shadow.rgen:27-38
     light_weights(dirs.data(), wt.data());
@@ -1603,10 +1607,10 @@ extern "C" int ane_scene_light_shadows(ane_scene* s, const ane_light* lights, in
     ANE_CHECK_HIP(hipMemcpy(s->d_lt_dirs, dirs.data(), sizeof(float) * dirs.size(), hipMemcpyHostToDevice));
     ANE_CHECK_HIP(hipMemcpy(s->d_lt_wt, lw.data(), sizeof(float) * lw.size(), hipMemcpyHostToDevice));
     ANE_CHECK_HIP(hipMemcpy(s->d_lt_idx, li.data(), sizeof(int32_t) * li.size(), hipMemcpyHostToDevice));
-    s->lt_kmax = kmax;
     ANE_TRY(dalloc(&s->d_lt_work, 1, nullptr));
     ANE_TRY(dalloc(&s->d_lt_stats, 2, nullptr));
     ANE_TRY(dalloc(&s->d_lt_ov, 1, nullptr));
+    s->lt_kmax = kmax;
   }
   if (num_lights > s->lt_cap) {
     ANE_CHECK_HIP(hipStreamSynchronize(st));

@@ -316,7 +316,9 @@ int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const flo
  * something is hit, else 1; depth == 1 (sky) gives 1 without a ray.
  * inv_view_proj16 is column-major; depth and out_visibility are DEVICE pointers
  * (width * height floats), ordered on `stream`.  counters2 (host, nullable)
- * receives node visits and triangle tests (the stream is then synchronized). */
+ * receives node visits and triangle tests (the stream is then synchronized).  The
+ * rays, hits and sun grid live on the scene: calls on one scene must be ordered on
+ * one stream or must not overlap. */
 int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, int32_t height,
                           const float* inv_view_proj16, const float* light_dir3, float* out_visibility,
                           uint64_t* counters2, void* stream);
@@ -338,7 +340,8 @@ int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, i
  * The lights are read from HOST memory during the call; the images are ordered on
  * `stream`.  counters2 (host, nullable) receives node visits and triangle tests
  * (the stream is then synchronized).  Enabled lights need a finite position and
- * range. */
+ * range.  Like ane_scene_shadow_mask, the call uses workspace kept on the scene:
+ * calls on one scene must be ordered on one stream or must not overlap. */
 int ane_scene_light_shadows(ane_scene* scene, const ane_light* lights, int32_t num_lights,
                             float* dist_img, float* moments_img, uint64_t* counters2, void* stream);
 
