@@ -1566,7 +1566,7 @@ extern "C" int ane_scene_light_shadows(ane_scene* s, const ane_light* lights, in
   if (num_lights == 0) return ANE_OK;
   if (!lights || !moments_img) return fail(ANE_ERR_INVALID_ARG, "NULL lights or moments image");
   for (int32_t l = 0; l < num_lights; ++l) {
-    if (!(lights[l].color[3] >= 0.5f)) continue;  // disabled (NaN counts as disabled, as in the shader)
+    if (lights[l].color[3] < 0.5f) continue;  // disabled; the shaders' test, so NaN counts as enabled
     for (int k = 0; k < 4; ++k)
       if (!std::isfinite(lights[l].world_pos[k]))
         return fail(ANE_ERR_INVALID_ARG, "non-finite position or range of an enabled light");

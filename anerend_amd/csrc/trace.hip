@@ -1097,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_light_rays(const float* lights, int n, 
   if (i >= (int64_t)n * kLightRays) return;
   const int l = (int)(i / kLightRays), r = (int)order[i % kLightRays];
   const float* L = lights + 8 * (int64_t)l;
-  const bool on = L[7] >= 0.5f;
+  const bool on = !(L[7] < 0.5f);  // rgen:54: disabled when color.w < 0.5
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     orig[3 * i + k] = L[k];
@@ -1137,7 +1137,7 @@ __global__ __launch_bounds__(64) void k_light_moments(const float* lights, int n
   for (int k = t; k < kLightRays; k += 64) ds[k] = dist[(int64_t)l * kLightRays + k];
   __syncthreads();
   float ar = 0.0f, ag = 0.0f, aw = 0.0f;
-  if (lights[8 * (int64_t)l + 7] >= 0.5f) {
+  if (!(lights[8 * (int64_t)l + 7] < 0.5f)) {  // sum.comp:35
 #pragma unroll 8
     for (int k = 0; k < kmax; ++k) {
       const float w = lw[k * 64 + t];

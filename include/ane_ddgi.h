@@ -328,7 +328,8 @@ int ane_scene_shadow_mask(ane_scene* scene, const float* depth, int32_t width, i
  * LightShadowSumPass.cpp:65-87 with light_shadow_sum.comp:16-71).  Each of the
  * num_lights (<= 1024) lights casts 1024 rays from its position along the
  * unrotated Fibonacci directions (rgen:27-38) over [0.1, range]; a ray's distance is
- * its closest hit, or the range on a miss, or 0 for a disabled light (color[3] < 0.5).
+ * its closest hit, or the range on a miss, or 0 for a disabled light (color[3] < 0.5;
+ * as in the shaders, a NaN color[3] is enabled).
  *   dist_img     (DEVICE, nullable) R32F 32 x (32 num_lights) image:
  *                ray y * 32 + x of light l at pixel (32 l + x, y);
  *   moments_img  (DEVICE) RG32F 8 x (10 num_lights) image: for texel (tx, ty) of

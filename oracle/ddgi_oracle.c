@@ -833,7 +833,7 @@ void ora_light_shadows(const ora_scene* s, const float* lights, int32_t n, float
     for (int t = 0; t < 64; ++t) {
       const int tx = t & 7, ty = t >> 3;
       float acc_r = 0.0f, acc_g = 0.0f, acc_w = 0.0f;
-      if (lights[8 * (int64_t)l + 7] >= 0.5f) {
+      if (!(lights[8 * (int64_t)l + 7] < 0.5f)) {  /* sum.comp:35: disabled when color.w < 0.5 */
         for (int x = 0; x < ORA_LIGHT_SIDE; ++x)      /* sum.comp:50-63: x outer, y inner */
           for (int y = 0; y < ORA_LIGHT_SIDE; ++y) {
             const int idx = y * ORA_LIGHT_SIDE + x;

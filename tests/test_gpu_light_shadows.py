@@ -47,8 +47,10 @@ def test_room_lights_match_oracle():
     L[7, :3] = np.nan               # a disabled light's contents are not read
     L[9, 3] = 0.05                  # range below t_min: every ray misses, dist = range
     L[12, :3] = (5.0, 1.0, 0.0)     # on the room's +x wall
+    L[14, 7] = np.nan               # the shaders test color.w < 0.5: NaN is enabled
     dg, mg = _check(ds, osc, L, "room")
     assert np.all(dg[:, 32 * 3:32 * 4] == 0.0) and np.all(dg[:, 32 * 9:32 * 10] == np.float32(0.05))
+    assert np.all(dg[:, 32 * 14:32 * 15] > 0.0) and np.all(mg[:, 141:149, 0] > 0.0)
     # the same scene again with fewer lights (workspace reuse)
     _check(ds, osc, L[:5], "room, 5 lights")
 
