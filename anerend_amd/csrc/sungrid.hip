@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_sun_estimate(const float4* tris, int32_
   }
 }
 
-__global__ __launch_bounds__(256) void k_sun_judge(const float* part, SunGridParams* gp)
+__global__ __launch_bounds__(256) void k_sun_decide(const float* part, SunGridParams* gp)
 {
   __shared__ float rs[256], rn[256];
   const int t = threadIdx.x;
@@ -331,7 +331,7 @@ void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridPara
                      (const uint32_t*)cnt, ncell, cap, d_params, cur);
   hipLaunchKernelGGL(k_sun_estimate, dim3(kEstBlocks), dim3(256), 0, s, sc.tris, sc.ntri,
                      (const SunGridParams*)d_params, (const uint32_t*)cnt, est);
-  hipLaunchKernelGGL(k_sun_judge, dim3(1), dim3(256), 0, s, (const float*)est, d_params);
+  hipLaunchKernelGGL(k_sun_decide, dim3(1), dim3(256), 0, s, (const float*)est, d_params);
   if (sc.ntri > 0)
     hipLaunchKernelGGL(k_sun_scatter, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri,
                        (const SunGridParams*)d_params, cur, list, cap);

@@ -22,7 +22,7 @@ from tests.conftest import bits, rel_err
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-4            # FP32 path vs oracle
-BF16_MAX_REL = 2e-3    # BF16 MFMA convolution vs float64
+BF16_MAX_REL = 2e-3    # BF16 MFMA convolution vs float64: the required bound
 FP8_MAX_REL = 6e-2     # FP8-e4m3 (per-probe + per-texel scales): stated bound
 
 
