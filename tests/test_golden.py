@@ -19,3 +19,12 @@ def test_oracle_matches_room8_fixture():
         assert np.isclose(got["radiance_sum"], want["radiance_sum"], rtol=1e-6)
     ref = np.load(os.path.join(HERE, "room8_atlas.npz"))["atlas"]
     assert np.allclose(atlas[::3, ::3], ref, rtol=1e-6, atol=1e-7)
+
+
+def test_oracle_matches_room8_lights_fixture():
+    from tests.golden.make_golden import run_room8_lights
+    want = json.load(open(os.path.join(HERE, "room8_lights.json")))
+    got = run_room8_lights()
+    for k in ("dist_sha256", "moments_sha256", "counters"):
+        assert got[k] == want[k], k
+    assert np.isclose(got["mean_sum"], want["mean_sum"], rtol=1e-9)

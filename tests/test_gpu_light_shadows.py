@@ -75,3 +75,16 @@ def test_invalid_arguments():
         ds.light_shadows(bad)
     with pytest.raises(native.AneError):
         ds.light_shadows(np.zeros((1025, 8), np.float32))
+
+
+def test_room8_lights_match_golden_fixture():
+    import json
+    import os
+    from tests.golden.make_golden import room8_lights, sha
+    want = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "room8_lights.json")))
+    scene, _, _ = scenegen.workload("room8")
+    dist, mom, ctr = native.DeviceScene(scene).light_shadows(room8_lights(), count=True)
+    assert sha(dist) == want["dist_sha256"]
+    assert sha(mom) == want["moments_sha256"]
+    assert [int(c) for c in ctr] == want["counters"]
