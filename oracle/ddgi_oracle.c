@@ -32,6 +32,26 @@ static inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, 
 static inline f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 static inline void st3(float* p, f3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 
+/* Planted mistakes, for the pin tests only (tests/test_oracle_mutants.py builds
+ * copies of this file with -DORA_MUTANT=<id> and checks that the closed-form
+ * pins fail on each).  The default build, and every library loaded by the
+ * other tests, has ORA_MUTANT == 0 and none of them:
+ *   1  Schlick-GGX k = roughness^2 / 2          (pbr_light.glsl:26 has (r+1)^2/8)
+ *   2  Fresnel exponent 4                        (pbr_light.glsl:90 has 5)
+ *   3  wrap weight without the +0.2              (pbr_light.glsl:214)
+ *   4  corners (1,1,0) and (1,0,0) read each other's tiles (pbr_light.glsl:326-335)
+ *   5  tile start 10*i instead of 10*i + 1       (pbr_light.glsl:174-176)
+ *   6  trilinear weights flipped: mix(alpha, 1 - alpha, offset) (pbr_light.glsl:194)
+ *   7  indirect Fresnel with roughness, not roughness^2 (pbr_light.glsl:380)
+ *   8  tile end start + 8 instead of start + 7   (pbr_light.glsl:178)
+ *   9  GGX numerator without the +0.001          (pbr_light.glsl:17)
+ *  10  probes blended in linear, not sqrt, space (pbr_light.glsl:235, 359)
+ *  11  texel y/z swap left out of octEncode      (octahedron_helpers.glsl:10-13) */
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0
+#endif
+#define MUTANT(id) (ORA_MUTANT == (id))
+
 static const float PI_F = 3.14159265359f;          /* pbr_light.glsl:8, rgen:14 */
 static const double GOLDEN_RATIO_D = 1.6180339887; /* rgen:13 */
 static const double PI_D = 3.14159265359;          /* rgen:14 */
@@ -107,7 +127,7 @@ static inline float sign_not_zero(float k) { return (k >= 0.0f) ? 1.0f : -1.0f; 
 
 static void oct_encode(f3 v_in, float* rx, float* ry)
 {
-  f3 v = mk(v_in.x, v_in.z, v_in.y);
+  f3 v = MUTANT(11) ? v_in : mk(v_in.x, v_in.z, v_in.y);
   float l1 = fabsf(v.x) + fabsf(v.y) + fabsf(v.z);
   float inv = 1.0f / l1;
   float x = v.x * inv, y = v.y * inv;
@@ -902,7 +922,7 @@ static inline float distribution_ggx(f3 N, f3 H, float rough)
   float a2 = a * a;
   float ndoth = fmaxf(dot(N, H), 0.0f);
   float ndoth2 = ndoth * ndoth;
-  float num = a2 + 0.001f;
+  float num = MUTANT(9) ? a2 : a2 + 0.001f;
   float den = ndoth2 * (a2 - 1.0f) + 1.0f;
   den = PI_F * den * den;
   return num / den;
@@ -912,12 +932,15 @@ static inline float distribution_ggx(f3 N, f3 H, float rough)
 static inline float schlick_ggx(float ndotv, float rough)
 {
   float r = rough + 1.0f;
-  float k = (r * r) / 8.0f;
+  float k = MUTANT(1) ? rough * rough / 2.0f : (r * r) / 8.0f;
   return ndotv / (ndotv * (1.0f - k) + k);
 }
 
 /* pbr_light.glsl:88-91: pow(clamp(1 - c, 0, 1), 5) via libm powf (contract) */
-static inline float pow5_term(float c) { return powf(clampf(1.0f - c, 0.0f, 1.0f), 5.0f); }
+static inline float pow5_term(float c)
+{
+  return powf(clampf(1.0f - c, 0.0f, 1.0f), MUTANT(2) ? 4.0f : 5.0f);
+}
 
 /* pbr_light.glsl:98-166 with directional == 1, useSpecular == true */
 static f3 calc_light(const ora_frame* f, f3 N, f3 albedo, float metallic, float rough, f3 pos,
@@ -989,9 +1012,9 @@ static f3 bilinear_repeat(const float* atlas, int W, int H, float u, float v)
 static f3 sample_single_probe(const ora_config* c, const float* atlas, int ix, int iy, int iz, f3 n)
 {
   int W = (int)ora_atlas_width(c), H = (int)ora_atlas_height(c);
-  int sx = 10 * ix + 1;
-  int sy = 10 * c->probes_z * iy + 10 * iz + 1;
-  int ex = sx + 7, ey = sy + 7;
+  int sx = 10 * ix + (MUTANT(5) ? 0 : 1);
+  int sy = 10 * c->probes_z * iy + 10 * iz + (MUTANT(5) ? 0 : 1);
+  int ex = sx + (MUTANT(8) ? 8 : 7), ey = sy + (MUTANT(8) ? 8 : 7);
   float su = (2.0f * (float)sx + 1.0f) / (2.0f * (float)W);
   float sv = (2.0f * (float)sy + 1.0f) / (2.0f * (float)H);
   float eu = (2.0f * (float)ex + 1.0f) / (2.0f * (float)W);
@@ -1042,19 +1065,22 @@ static f3 sample_probe(const ora_config* c, const ora_frame* f, const float* atl
     f3 ppos = add(base, mul(mk((float)ox, (float)oy, (float)oz), step));
     /* weightProbe, pbr_light.glsl:190-238 */
     f3 tri = mk(ox ? alpha.x : 1.0f - alpha.x, oy ? alpha.y : 1.0f - alpha.y, oz ? alpha.z : 1.0f - alpha.z);
+    if (MUTANT(6))
+      tri = mk(ox ? 1.0f - alpha.x : alpha.x, oy ? 1.0f - alpha.y : alpha.y, oz ? 1.0f - alpha.z : alpha.z);
     f3 dir = normalize(sub(ppos, pos));
     float x = fmaxf(0.0001f, (dot(dir, n) + 1.0f) * 0.5f);
-    float w = x * x + 0.2f; /* pow(x, 2) contract: x * x */
+    float w = x * x + (MUTANT(3) ? 0.0f : 0.2f); /* pow(x, 2) contract: x * x */
     w = fmaxf(0.000001f, w);
     if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
     w = w * (tri.x * tri.y * tri.z);
-    f3 s = sample_single_probe(c, atlas, px + ox, py + oy, pz + oz, n);
-    s = mk(sqrtf(s.x), sqrtf(s.y), sqrtf(s.z));
+    int tk = MUTANT(4) ? (k == 2 ? 3 : (k == 3 ? 2 : k)) : k;
+    f3 s = sample_single_probe(c, atlas, px + corners[tk][0], py + corners[tk][1], pz + corners[tk][2], n);
+    if (!MUTANT(10)) s = mk(sqrtf(s.x), sqrtf(s.y), sqrtf(s.z));
     sumi = add(sumi, muls(s, w));
     sumw = sumw + w;
   }
   f3 net = divs(sumi, sumw);
-  net = mul(net, net); /* pow(net, 2) contract: x * x */
+  if (!MUTANT(10)) net = mul(net, net); /* pow(net, 2) contract: x * x */
   return muls(muls(net, 0.5f), 3.1415926f);
 }
 
@@ -1070,7 +1096,7 @@ static f3 calc_indirect(const ora_config* c, const ora_frame* f, const float* at
 {
   f3 V = normalize(sub(view, pos));
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
-  float r = rough * rough; /* pbr_light.glsl:380 passes roughness^2 */
+  float r = MUTANT(7) ? rough : rough * rough; /* pbr_light.glsl:380 passes roughness^2 */
   float p5 = pow5_term(fmaxf(dot(N, V), 0.0f));
   float omr = 1.0f - r;
   f3 kS = mk(F0.x + (fmaxf(omr, F0.x) - F0.x) * p5,

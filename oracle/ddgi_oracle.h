@@ -31,8 +31,20 @@
  *                          traversal == brute force over all triangles
  *   ora_trace              brute force (ora_trace_bruteforce), analytic
  *                          plane/box hits
- *   ora_shade_hit          closed forms at N = L = V, constant-atlas indirect
- *                          (E = c*pi/2), sun-below-horizon, float64 re-evaluation
+ *   ora_shade_hit          Cook-Torrance closed forms at N = L = V, at V and L
+ *                          symmetric about N (H = N, cosines 0.6) and at cosines
+ *                          (N.H, N.V, N.L, H.V) = (0.7 sqrt2, 0.6, 0.8, 1/sqrt2);
+ *                          indirect Fresnel-roughness at N.V = 0.6; shadowed,
+ *                          sun-below-horizon, emissive and miss cases
+ *   ora_sample_probe,      constant atlas (E = c*pi/2); alpha = 0 reads the base
+ *   ora_gi_lookup          probe's own tile; grid edges read the REPEAT-wrapped
+ *                          tile; alpha = 1/2 on x, y, z and at a cell centre gives
+ *                          the sqrt-space blend of distinct tiles with the wrap
+ *                          weights of cosines 0, +-1/sqrt2, +-1; a linear ramp per
+ *                          tile fixes the texel addressing (start 10 i + 1, end
+ *                          start + 7, octEncode's y/z swap); Fresnel-roughness
+ *                          with roughness^2 at N.V < 1.  Eleven planted mistakes
+ *                          (ORA_MUTANT, ddgi_oracle.c) each fail one of these.
  *   ora_convolve_blend     float64 GEMM, uniform radiance => E = L, fixed point,
  *                          geometric series over k updates
  *   ora_translate          independent per-texel restatement of the two copies

@@ -13,7 +13,9 @@ import numpy as np
 from anerend_amd.types import HIT_DTYPE, Config, Frame, ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libddgi_oracle.so")
+# DDGI_ORACLE_LIB names another build of ddgi_oracle.c to load instead (the
+# planted-mistake copies of tests/test_oracle_mutants.py); unset, the in-tree one.
+LIB_PATH = os.environ.get("DDGI_ORACLE_LIB") or os.path.join(HERE, "libddgi_oracle.so")
 
 
 def build():

@@ -11,8 +11,8 @@ from anerend_amd.types import FLAG_DDGI, FLAG_DDGI_MULTI, HIT_DTYPE, INSTANCE_DT
 from oracle import oracle as O
 
 
-def quad_scene(albedo, rough, metal, emissive=(0, 0, 0, 0), occluder=False):
-    """A 40 x 40 m quad facing +y at y = 0 (+ optional occluder quad at y = 2)."""
+def quad_scene(albedo, rough, metal, emissive=(0, 0, 0, 0), occluder=False, y=0.0):
+    """A 40 x 40 m quad facing +y at height y (+ optional occluder quad 2 m above)."""
     mb = scenegen._MeshBuilder()
     p = np.array([[-20, 0, -20], [20, 0, -20], [20, 0, 20], [-20, 0, 20]], np.float32)
     n = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
@@ -24,9 +24,9 @@ def quad_scene(albedo, rough, metal, emissive=(0, 0, 0, 0), occluder=False):
     mats["emissive"][0] = emissive
     mats["base_col_fac"][1] = (0.5, 0.5, 0.5, 1.0)
     mats["met_rough"][1] = (0, 1, 0, 0)
-    inst = [scenegen.make_instance(m, 0, np.eye(3), (1, 1, 1), (0, 0, 0))]
+    inst = [scenegen.make_instance(m, 0, np.eye(3), (1, 1, 1), (0, y, 0))]
     if occluder:
-        inst.append(scenegen.make_instance(m, 1, np.eye(3), (0.05, 1, 0.05), (0, 2, 0)))
+        inst.append(scenegen.make_instance(m, 1, np.eye(3), (0.05, 1, 0.05), (0, y + 2, 0)))
     v, i, meshes = mb.arrays()
     return scenegen.Scene(v, i, meshes, mats, np.array(inst, INSTANCE_DTYPE))
 
