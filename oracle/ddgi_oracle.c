@@ -1274,6 +1274,32 @@ void ora_mirror_reflections(const ora_scene* s, const ora_config* c, const ora_f
 }
 
 /* ------------------------------------------------------------------------ */
+/* RGBA16F storage (Q6, ORA_QUIRK_FP16)                                      */
+/* ------------------------------------------------------------------------ */
+/* Round to the nearest binary16 value, ties to even, and widen back: the
+ * binary16 quantum at binade e is 2^(e - 10) for normal halves and 2^-24 below
+ * 2^-14; |x| / quantum is exact in binary64 and rint() rounds it to even.
+ * Magnitudes from 65520 (the midpoint above 65504) round to infinity. */
+static float f16_round(float x)
+{
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  const uint32_t a = u & 0x7fffffffu;
+  if (a >= 0x7f800000u) return x;                 /* inf, nan */
+  const float sgn = (u >> 31) ? -1.0f : 1.0f;
+  if (a >= 0x477ff000u) return sgn * INFINITY;    /* >= 65520 */
+  const int e = (int)(a >> 23) - 127;
+  const int qe = e < -14 ? -24 : e - 10;
+  const double q = ldexp(1.0, qe);
+  return sgn * (float)(rint((double)fabsf(x) / q) * q);
+}
+
+void ora_f16_round(const float* in, float* out, int64_t n)
+{
+  for (int64_t i = 0; i < n; ++i) out[i] = f16_round(in[i]);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Convolution + hysteresis: probe_conv.comp:21-89                           */
 /* ------------------------------------------------------------------------ */
 void ora_convolve_blend(const ora_config* c, const ora_frame* f, const float* dirs,
@@ -1323,6 +1349,8 @@ void ora_convolve_blend(const ora_config* c, const ora_frame* f, const float* di
         if (acc[3] > 1e-6f) { acc[0] = acc[0] / acc[3]; acc[1] = acc[1] / acc[3]; acc[2] = acc[2] / acc[3]; }
         for (int a = 0; a < 4; ++a) res[a] = acc[a] * (1.0f - h) + o4[a] * h;
       }
+      if (c->quirks & ORA_QUIRK_FP16)  /* imageStore into the RGBA16F atlas, :89 */
+        for (int a = 0; a < 4; ++a) res[a] = f16_round(res[a]);
       memcpy(o4, res, 16);
     }
   }
@@ -1369,6 +1397,7 @@ void ora_update(const ora_scene* s, const ora_config* c, const ora_frame* f, flo
   int n = c->rays_per_probe;
   float* base = (float*)malloc(sizeof(float) * 3 * (size_t)n);
   float* dirs = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+  const int fp16 = (c->quirks & ORA_QUIRK_FP16) != 0;
   ora_fibonacci_dirs(n, base);
   ora_rotate_dirs(n, f->rotation, base, dirs);
   int64_t np = (int64_t)c->probes_x * c->probes_z * f->layer_count;
@@ -1392,12 +1421,15 @@ void ora_update(const ora_scene* s, const ora_config* c, const ora_frame* f, flo
       ora_shade_hit(s, c, f, atlas, org, &h, L, &sh, sc);
       c2 += sc[0]; c3 += sc[1];
       float* R = rad + q * 3 * n;
+      if (fp16) { L[0] = f16_round(L[0]); L[1] = f16_round(L[1]); L[2] = f16_round(L[2]); } /* rgen:109 */
       R[r] = L[0]; R[n + r] = L[1]; R[2 * n + r] = L[2];
       if (hits) hits[q * n + r] = h;
       if (shadowed) shadowed[q * n + r] = (uint8_t)(sh < 0 ? 255 : sh);
     }
   }
-  ora_convolve_blend(c, f, dirs, rad, atlas);
+  if (fp16) /* the convolution reads the directions back from the RGBA16F image, rgen:108 */
+    for (int k = 0; k < 3 * n; ++k) base[k] = f16_round(dirs[k]);
+  ora_convolve_blend(c, f, fp16 ? base : dirs, rad, atlas);
   if (counters) { counters[0] += c0; counters[1] += c1; counters[2] += c2; counters[3] += c3; }
   if (!radiance) free(rad);
   free(base);

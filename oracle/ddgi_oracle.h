@@ -65,6 +65,17 @@
 extern "C" {
 #endif
 
+/* Oracle-only quirk bit of ora_config.quirks (the library takes quirks = 0):
+ * emulate the reference's RGBA16F images (Q6).  Ray radiance (ProbeRaysIrrTex),
+ * ray directions as the convolution reads them (ProbeRaysDirTex) and every atlas
+ * texel the convolution writes (ProbeRaysConvTex) are rounded to IEEE binary16,
+ * round to nearest even (IrradianceProbeRayTracingPass.cpp:42-68,
+ * IrradianceProbeConvolvePass.cpp:91; rgen:108-109, probe_conv.comp:49-50,89).
+ * Tracing keeps the binary32 direction, as the rgen does.  Hardware filtering at
+ * reduced precision is not modelled.  With this bit the oracle measures how far
+ * the reference's own storage moves the result from the FP32 contract. */
+#define ORA_QUIRK_FP16 (1u << 0)
+
 /* scene_ubo.glsl:5,6,12 */
 #define ORA_FLAG_DDGI (1u << 4)
 #define ORA_FLAG_DDGI_MULTI (1u << 5)
@@ -201,6 +212,9 @@ void ora_shade_hit(const ora_scene* s, const ora_config* cfg, const ora_frame* f
                    int32_t* shadowed, uint64_t* counters);
 void ora_sample_probe(const ora_config* cfg, const ora_frame* fr, const float* atlas,
                       const float* pos, const float* normal, float* out3);
+
+/* binary32 -> binary16 (round to nearest, ties to even; >= 65520 -> inf) -> binary32 */
+void ora_f16_round(const float* in, float* out, int64_t n);
 
 /* ---- convolution + hysteresis, translation ---- */
 /* radiance: [P_layers][3][n] for the probes of layers [layer_begin, +layer_count) */

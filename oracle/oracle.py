@@ -66,6 +66,7 @@ def lib():
         L.ora_translate.argtypes = [vp, vp, vp, vp]
         L.ora_update.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.ora_atlas_clear.argtypes = [vp, vp]
+        L.ora_f16_round.argtypes = [vp, vp, i64]
         L.ora_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -291,6 +292,17 @@ def translate(cfg, camera_pos, last_idx, atlas):
     last = np.ascontiguousarray(last_idx, np.int32)
     moved = lib().ora_translate(_ref(cfg), ptr(cam), ptr(last), ptr(atlas))
     return bool(moved), last
+
+
+QUIRK_FP16 = 1 << 0   # ORA_QUIRK_FP16: emulate the reference's RGBA16F images
+
+
+def f16_round(x):
+    """binary32 -> binary16 (nearest even) -> binary32, the oracle's own conversion."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().ora_f16_round(ptr(x), ptr(out), x.size)
+    return out
 
 
 def new_atlas(cfg):
