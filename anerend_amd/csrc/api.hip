@@ -577,6 +577,9 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   if (num_materials > (1 << 10)) return fail(ANE_ERR_INVALID_ARG, "at most 1024 materials (10-bit material index)");
   for (int32_t m = 0; m < num_meshes; ++m) {
     const ane_mesh& me = meshes[m];
+    if (me.reserved != 0)
+      return fail(ANE_ERR_INVALID_ARG, "mesh %d: reserved field must be 0 (a GPUMeshInfo array goes to "
+                  "ane_scene_create_mesh_info)", m);
     if (me.index_count % 3 != 0)
       return fail(ANE_ERR_INVALID_ARG, "mesh %d index_count %u is not a multiple of 3", m, me.index_count);
     if ((int64_t)me.index_offset + me.index_count > num_indices)
@@ -663,6 +666,31 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   if (st) return bail(st);
   *out = s;
   return ANE_OK;
+}
+
+extern "C" int ane_scene_create_mesh_info(const float* vertices, int64_t num_vertices,
+                                          const uint32_t* indices, int64_t num_indices,
+                                          const ane_gpu_mesh_info* meshes,
+                                          const uint32_t* mesh_index_counts, int32_t num_meshes,
+                                          const ane_material* materials, int32_t num_materials,
+                                          const ane_instance* instances, int32_t num_instances,
+                                          void* stream, ane_scene** out)
+{
+  if (num_meshes < 0) return fail(ANE_ERR_INVALID_ARG, "negative count");
+  if (num_meshes && (!meshes || !mesh_index_counts))
+    return fail(ANE_ERR_INVALID_ARG, "NULL mesh array or index counts with a non-zero count");
+  if (num_meshes > (1 << 14)) return fail(ANE_ERR_INVALID_ARG, "at most 16384 meshes (14-bit mesh index)");
+  std::vector<ane_mesh> m((size_t)num_meshes);
+  for (int32_t k = 0; k < num_meshes; ++k) {
+    memcpy(m[k].min_pos, meshes[k].min_pos, sizeof(m[k].min_pos));
+    memcpy(m[k].max_pos, meshes[k].max_pos, sizeof(m[k].max_pos));
+    m[k].vertex_offset = meshes[k].vertex_offset;
+    m[k].index_offset = meshes[k].index_offset;
+    m[k].index_count = mesh_index_counts[k];
+    m[k].reserved = 0;
+  }
+  return ane_scene_create(vertices, num_vertices, indices, num_indices, m.data(), num_meshes, materials,
+                          num_materials, instances, num_instances, stream, out);
 }
 
 static int check_same_meshes(const ane_scene* s, const ane_instance* inst, int32_t n)
@@ -788,7 +816,8 @@ struct ane_ddgi_ctx {
   float2* d_tiles = nullptr;  // tile texture bounds: nx + 1 columns, then (ny + 1) * (nz + 1) rows
   uint32_t* d_work = nullptr;
   unsigned long long* d_stats = nullptr;
-  uint32_t* d_overflow = nullptr;
+  uint32_t* d_overflow = nullptr;  // sticky traversal fault of the probe traces (ane_ddgi_check)
+  uint32_t step_limit = 0;          // ane_ddgi_set_traversal_limit
   int32_t* d_spill = nullptr;
   int64_t spill_words = 0;
   StageTimer timers[kStages];
@@ -1022,6 +1051,7 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
   a.shadow = c->d_shadow;
   a.stats = c->d_stats;
   a.overflow = c->d_overflow;
+  a.step_limit = c->step_limit;
   a.spill = c->d_spill;
   const int64_t lanes = trace_lanes_per_launch();
   a.spill_depth = (int32_t)(lanes ? c->spill_words / lanes : 0);
@@ -1455,8 +1485,24 @@ extern "C" int ane_ddgi_stats_get(const ane_ddgi_ctx* cc, ane_ddgi_stats* out)
   }
   uint32_t ov = 0;
   ANE_CHECK_HIP(hipMemcpy(&ov, c->d_overflow, 4, hipMemcpyDeviceToHost));
-  if (ov) return fail(ANE_ERR_DEVICE, "traversal stack exceeded its spill capacity");
+  s.traversal_fault = ov ? 1u : 0u;
   *out = s;
+  if (ov) return fail(ANE_ERR_DEVICE, "traversal fault: some probe rays ended without a result "
+                      "(stack over its spill capacity, step limit or malformed hierarchy)");
+  return ANE_OK;
+}
+
+// reads and clears the context's traversal fault flags (probe traces and mirror rays)
+static int take_faults(ane_ddgi_ctx* c, hipStream_t st, uint32_t* probe, uint32_t* mirror)
+{
+  uint32_t v[2] = {0, 0};
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  ANE_CHECK_HIP(hipMemcpy(&v[0], c->d_overflow, 4, hipMemcpyDeviceToHost));
+  if (c->d_mr_ov) ANE_CHECK_HIP(hipMemcpy(&v[1], c->d_mr_ov, 4, hipMemcpyDeviceToHost));
+  ANE_CHECK_HIP(hipMemset(c->d_overflow, 0, 4));
+  if (c->d_mr_ov) ANE_CHECK_HIP(hipMemset(c->d_mr_ov, 0, 4));
+  *probe = v[0];
+  *mirror = v[1];
   return ANE_OK;
 }
 
@@ -1467,6 +1513,26 @@ extern "C" int ane_ddgi_stats_reset(ane_ddgi_ctx* c)
     ANE_TRY(t.harvest_all());
     t.reset();
   }
+  uint32_t p = 0, m = 0;
+  return take_faults(c, nullptr, &p, &m);
+}
+
+extern "C" int ane_ddgi_check(ane_ddgi_ctx* c, void* stream)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  uint32_t p = 0, m = 0;
+  ANE_TRY(take_faults(c, S(stream), &p, &m));
+  if (p || m)
+    return fail(ANE_ERR_DEVICE, "traversal fault in the %s: some rays ended without a result (stack over "
+                "its spill capacity, step limit or malformed hierarchy)",
+                p && m ? "probe traces and mirror rays" : (p ? "probe traces" : "mirror rays"));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_set_traversal_limit(ane_ddgi_ctx* c, uint32_t max_node_visits)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  c->step_limit = max_node_visits;
   return ANE_OK;
 }
 
@@ -1553,6 +1619,21 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
     counters2[1] = c2[1];
     if (ov) return fail(ANE_ERR_DEVICE, "traversal stack overflow");
   }
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_check(ane_scene* s, void* stream)
+{
+  if (!s) return fail(ANE_ERR_INVALID_ARG, "NULL scene");
+  uint32_t v[2] = {0, 0};
+  ANE_CHECK_HIP(hipStreamSynchronize(S(stream)));
+  if (s->d_sh_ov) ANE_CHECK_HIP(hipMemcpy(&v[0], s->d_sh_ov, 4, hipMemcpyDeviceToHost));
+  if (s->d_lt_ov) ANE_CHECK_HIP(hipMemcpy(&v[1], s->d_lt_ov, 4, hipMemcpyDeviceToHost));
+  if (s->d_sh_ov) ANE_CHECK_HIP(hipMemset(s->d_sh_ov, 0, 4));
+  if (s->d_lt_ov) ANE_CHECK_HIP(hipMemset(s->d_lt_ov, 0, 4));
+  if (v[0] || v[1])
+    return fail(ANE_ERR_DEVICE, "traversal fault in the last %s: some rays ended without a result",
+                v[0] && v[1] ? "shadow mask and light shadows" : (v[0] ? "shadow mask" : "light shadows"));
   return ANE_OK;
 }
 

@@ -113,7 +113,9 @@ struct TraceArgs {
   uint8_t* shadow;        // out (probe rays): 0 lit, 1 shadowed, 255 miss
   uint32_t* work;         // dynamic ray counter (zeroed before launch)
   unsigned long long* stats; // counting kernels: nodes, triangles [, shadow nodes, shadow triangles]
-  uint32_t* overflow;     // set to 1 if a stack would exceed the spill capacity
+  uint32_t* overflow;     // set to 1 when a ray ends without a result: a stack would exceed the
+                          // spill capacity, the step limit ran out or the hierarchy is malformed
+  uint32_t step_limit;    // node visits allowed per ray; 0: 2 * ntri + 8 (never reached by a valid tree)
   int32_t* spill;         // [spill_depth][total_lanes]
   int32_t spill_depth;
   int32_t total_lanes;

@@ -314,7 +314,8 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   const int n = a.g.rays;
   const uint32_t nrays = (uint32_t)a.nrays;
   const int32_t root = (sc.ntri >= 2) ? 0 : (sc.ntri == 1 ? ~0 : kDone);
-  const uint32_t budget = 2u * (uint32_t)sc.ntri + 8u;  // a tree traversal visits < 2 * ntri nodes
+  // a tree traversal visits < 2 * ntri nodes; a caller's step limit ends rays earlier (as faults)
+  const uint32_t budget = a.step_limit ? a.step_limit : 2u * (uint32_t)sc.ntri + 8u;
 
   int32_t ray = -1;  // active ray (processing index), -1 idle
   bool exhausted = false;

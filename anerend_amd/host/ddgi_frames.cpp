@@ -134,13 +134,28 @@ int main(int argc, char** argv)
 
   SceneFile sf;
   if (!load_scene(scene_path, sf)) return 1;
+  // The renderer holds GPUMeshInfo records (GpuBuffers.h:38-44, filled at
+  // VulkanRenderer.cpp:3486-3493, _blasRef = the BLAS device address) and the
+  // index counts on the host (InternalMesh::_numIndices): the scene is created
+  // from those, as the reference would hand them over.
+  std::vector<ane_gpu_mesh_info> mesh_info(sf.meshes.size());
+  std::vector<uint32_t> index_counts(sf.meshes.size());
+  for (size_t m = 0; m < sf.meshes.size(); ++m) {
+    std::memcpy(mesh_info[m].min_pos, sf.meshes[m].min_pos, sizeof(mesh_info[m].min_pos));
+    std::memcpy(mesh_info[m].max_pos, sf.meshes[m].max_pos, sizeof(mesh_info[m].max_pos));
+    mesh_info[m].vertex_offset = sf.meshes[m].vertex_offset;
+    mesh_info[m].index_offset = sf.meshes[m].index_offset;
+    mesh_info[m].blas_ref = 0x7f3a00000000ull + 0x10000ull * m;
+    index_counts[m] = sf.meshes[m].index_count;
+  }
   ane_scene* scene = nullptr;
-  int st = ane_scene_create(sf.vertices.data(), (int64_t)(sf.vertices.size() / 24), sf.indices.data(),
-                            (int64_t)sf.indices.size(), sf.meshes.data(), (int32_t)sf.meshes.size(),
-                            sf.materials.data(), (int32_t)sf.materials.size(), sf.instances.data(),
-                            (int32_t)sf.instances.size(), nullptr, &scene);
+  int st = ane_scene_create_mesh_info(sf.vertices.data(), (int64_t)(sf.vertices.size() / 24),
+                                      sf.indices.data(), (int64_t)sf.indices.size(), mesh_info.data(),
+                                      index_counts.data(), (int32_t)sf.meshes.size(), sf.materials.data(),
+                                      (int32_t)sf.materials.size(), sf.instances.data(),
+                                      (int32_t)sf.instances.size(), nullptr, &scene);
   if (st != ANE_OK) {
-    std::fprintf(stderr, "ane_scene_create: %s (%s)\n", ane_status_string(st), ane_last_error());
+    std::fprintf(stderr, "ane_scene_create_mesh_info: %s (%s)\n", ane_status_string(st), ane_last_error());
     return 1;
   }
   ane_ddgi_ctx* grid = nullptr;

@@ -39,7 +39,8 @@ class Stats(ctypes.Structure):
                 ("primary_nodes", ctypes.c_uint64), ("primary_tris", ctypes.c_uint64),
                 ("shadow_nodes", ctypes.c_uint64), ("shadow_tris", ctypes.c_uint64),
                 ("rays", ctypes.c_uint64), ("device_bytes", ctypes.c_int64),
-                ("sun_grid_state", ctypes.c_int32), ("sun_grid_candidates", ctypes.c_float)]
+                ("sun_grid_state", ctypes.c_int32), ("sun_grid_candidates", ctypes.c_float),
+                ("traversal_fault", ctypes.c_uint32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_[3:]}
@@ -58,9 +59,12 @@ _SIGS = [
     ("ane_device", ctypes.c_int, [_vp]),
     ("ane_stream_synchronize", ctypes.c_int, [_vp]),
     ("ane_scene_create", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp]),
+    ("ane_scene_create_mesh_info", ctypes.c_int,
+     [_vp, _i64, _vp, _i64, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp]),
     ("ane_scene_refit", ctypes.c_int, [_vp, _vp, _i32, _vp]),
     ("ane_scene_rebuild", ctypes.c_int, [_vp, _vp, _i32, _vp]),
     ("ane_scene_info_get", ctypes.c_int, [_vp, _vp]),
+    ("ane_scene_check", ctypes.c_int, [_vp, _vp]),
     ("ane_scene_destroy", ctypes.c_int, [_vp]),
     ("ane_ddgi_create", ctypes.c_int, [_vp, _vp]),
     ("ane_ddgi_destroy", ctypes.c_int, [_vp]),
@@ -82,6 +86,8 @@ _SIGS = [
     ("ane_ddgi_set_counting", ctypes.c_int, [_vp, _i32]),
     ("ane_ddgi_stats_get", ctypes.c_int, [_vp, _vp]),
     ("ane_ddgi_stats_reset", ctypes.c_int, [_vp]),
+    ("ane_ddgi_check", ctypes.c_int, [_vp, _vp]),
+    ("ane_ddgi_set_traversal_limit", ctypes.c_int, [_vp, ctypes.c_uint32]),
     ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
     ("ane_scene_shadow_mask", ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
@@ -161,14 +167,24 @@ def ray_directions(n, rotation16):
 class DeviceScene:
     """Geometry + LBVH resident in HBM (replaces the BLAS/TLAS builds)."""
 
-    def __init__(self, scene, stream=None):
+    def __init__(self, scene, stream=None, mesh_info=None):
+        """``mesh_info`` = (GPUMeshInfo array of GPU_MESH_INFO_DTYPE, uint32 index counts)
+        builds through ane_scene_create_mesh_info instead of scene.meshes."""
         L = load()
         self._h = ctypes.c_void_p()
         arrs = [np.ascontiguousarray(a) for a in
                 (scene.vertices, scene.indices, scene.meshes, scene.materials, scene.instances)]
         v, i, m, mt, inst = arrs
-        check(L.ane_scene_create(ptr(v), len(v), ptr(i), len(i), ptr(m), len(m), ptr(mt), len(mt),
-                                 ptr(inst), len(inst), stream, ctypes.byref(self._h)))
+        if mesh_info is None:
+            check(L.ane_scene_create(ptr(v), len(v), ptr(i), len(i), ptr(m), len(m), ptr(mt), len(mt),
+                                     ptr(inst), len(inst), stream, ctypes.byref(self._h)))
+        else:
+            gm = np.ascontiguousarray(mesh_info[0])
+            cnt = np.ascontiguousarray(mesh_info[1], np.uint32)
+            assert len(cnt) == len(gm)
+            check(L.ane_scene_create_mesh_info(ptr(v), len(v), ptr(i), len(i), ptr(gm), ptr(cnt), len(gm),
+                                               ptr(mt), len(mt), ptr(inst), len(inst), stream,
+                                               ctypes.byref(self._h)))
         self.instances = inst.copy()
 
     @property
@@ -186,6 +202,11 @@ class DeviceScene:
         s = SceneInfo()
         check(load().ane_scene_info_get(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def check(self, stream=None):
+        """Raises AneError if the last shadow-mask or light-shadow pass had a ray end
+        without a result (clears the flags)."""
+        check(load().ane_scene_check(self._h, stream))
 
     NODES_AUTO, NODES_FULL, NODES_QUANTIZED = 0, 1, 2
 
@@ -399,12 +420,25 @@ class ProbeGrid:
         check(load().ane_ddgi_set_counting(self._h, int(mode)))
 
     def stats(self):
+        """Stats dict; raises AneError (its ``stats`` holds the dict) while a traversal
+        fault is pending."""
         s = Stats()
-        check(load().ane_ddgi_stats_get(self._h, ctypes.byref(s)))
+        rc = load().ane_ddgi_stats_get(self._h, ctypes.byref(s))
+        if rc != ANE_OK:
+            err = AneError(rc, load().ane_last_error().decode())
+            err.stats = s.as_dict() if rc == 4 else None
+            raise err
         return s.as_dict()
 
     def reset_stats(self):
         check(load().ane_ddgi_stats_reset(self._h))
+
+    def check(self, stream=None):
+        """Raises AneError if a ray since the last check ended without a result (clears it)."""
+        check(load().ane_ddgi_check(self._h, stream))
+
+    def set_traversal_limit(self, max_node_visits):
+        check(load().ane_ddgi_set_traversal_limit(self._h, int(max_node_visits)))
 
 
 def synchronize(stream=None):

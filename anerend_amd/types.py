@@ -4,7 +4,9 @@ Each numpy dtype / ctypes structure has the byte layout of the reference's
 GPU-side struct it stands for:
 
 * ``VERTEX_WORDS`` packed vertex, 6 x vec4 = 96 B        (bindless.glsl:60-68, Vertex.h:7-19)
-* ``MESH_DTYPE``  GPUMeshInfo, 48 B, index count in place of blasRef (GpuBuffers.h:38-44)
+* ``MESH_DTYPE``  this library's mesh record: GPUMeshInfo's fields with the index
+                  count where GPUMeshInfo keeps blasRef (GpuBuffers.h:38-44)
+* ``GPU_MESH_INFO_DTYPE`` GPUMeshInfo itself, 48 B, blasRef ignored
 * ``MATERIAL_DTYPE`` GPUMaterialInfo, 64 B                 (GpuBuffers.h:29-36)
 * ``INSTANCE_DTYPE`` VkAccelerationStructureInstanceKHR, 64 B (tlas_update.comp:10-21)
 * ``Config``  probe grid constants                        (probe_helpers.glsl:1-7, probe_conv.comp:85)
@@ -25,6 +27,15 @@ MESH_DTYPE = np.dtype([
     ("reserved", "<u4"),
 ])
 assert MESH_DTYPE.itemsize == 48
+
+GPU_MESH_INFO_DTYPE = np.dtype([   # GPUMeshInfo verbatim (GpuBuffers.h:38-44), ane_gpu_mesh_info
+    ("min_pos", "<f4", (4,)),
+    ("max_pos", "<f4", (4,)),
+    ("vertex_offset", "<u4"),
+    ("index_offset", "<u4"),
+    ("blas_ref", "<u8"),
+])
+assert GPU_MESH_INFO_DTYPE.itemsize == 48
 
 MATERIAL_DTYPE = np.dtype([
     ("base_col_fac", "<f4", (4,)),

@@ -97,14 +97,30 @@ typedef struct ane_ddgi_frame {
   int32_t bake_tile[4];/* bakeTileInfo: tile x, tile z, tile size (bake mode) */
 } ane_ddgi_frame;
 
-typedef struct ane_mesh { /* GPUMeshInfo (GpuBuffers.h:38-44), index count in place of blasRef */
+/* This library's own mesh record: the fields of GPUMeshInfo (GpuBuffers.h:38-44)
+ * that the path reads, with the mesh's index count (InternalMesh::_numIndices,
+ * internal/InternalMesh.h:12, used as primitiveCount at VulkanRenderer.cpp:1313)
+ * where GPUMeshInfo keeps its 64-bit _blasRef.  It is NOT GPUMeshInfo: an array the
+ * reference filled goes to ane_scene_create_mesh_info with the counts beside it. */
+typedef struct ane_mesh {
   float min_pos[4];
   float max_pos[4];
   uint32_t vertex_offset;
   uint32_t index_offset;
   uint32_t index_count;
-  uint32_t reserved;
+  uint32_t reserved;   /* must be 0 */
 } ane_mesh;
+
+/* GPUMeshInfo exactly as the reference fills it (GpuBuffers.h:38-44, written by
+ * VulkanRenderer.cpp:3486-3493), 48 bytes.  blas_ref (the Vulkan BLAS device
+ * address) means nothing here and is ignored. */
+typedef struct ane_gpu_mesh_info {
+  float min_pos[4];
+  float max_pos[4];
+  uint32_t vertex_offset;
+  uint32_t index_offset;
+  uint64_t blas_ref;
+} ane_gpu_mesh_info;
 
 typedef struct ane_material { /* GPUMaterialInfo (GpuBuffers.h:29-36); textures unsupported (-1) */
   float base_col_fac[4];
@@ -166,6 +182,11 @@ typedef struct ane_ddgi_stats {
    * ray than traversal pays for); the estimate of its last build */
   int32_t sun_grid_state;
   float sun_grid_candidates;
+  /* 1 when a probe ray of a trace since the last ane_ddgi_check / _stats_reset
+   * ended without a result (stack over its spill capacity, the step limit of
+   * ane_ddgi_set_traversal_limit, or a malformed hierarchy); such a ray reads as
+   * a miss.  ane_ddgi_stats_get then returns ANE_ERR_DEVICE with *out filled. */
+  uint32_t traversal_fault;
 } ane_ddgi_stats;
 
 /* One G-buffer texel for the deferred GI lookup (deferred_tiled.comp:264-298 inputs):
@@ -201,6 +222,17 @@ int ane_scene_create(const float* vertices, int64_t num_vertices, const uint32_t
                      const ane_material* materials, int32_t num_materials,
                      const ane_instance* instances, int32_t num_instances, void* stream,
                      ane_scene** out);
+/* ane_scene_create for the reference's own buffers: `meshes` is its GPUMeshInfo
+ * array, taken as it is (blas_ref ignored), and mesh_index_counts[m] is mesh m's
+ * index count (InternalMesh::_numIndices, the BLAS primitiveCount * 3 of
+ * VulkanRenderer.cpp:1313), which GPUMeshInfo does not carry.  Same checks, same
+ * scene as ane_scene_create with ane_mesh{min, max, offsets, count, 0}. */
+int ane_scene_create_mesh_info(const float* vertices, int64_t num_vertices,
+                               const uint32_t* indices, int64_t num_indices,
+                               const ane_gpu_mesh_info* meshes, const uint32_t* mesh_index_counts,
+                               int32_t num_meshes, const ane_material* materials,
+                               int32_t num_materials, const ane_instance* instances,
+                               int32_t num_instances, void* stream, ane_scene** out);
 /* New instance transforms, same topology (UpdateBlasPass.cpp:131-138 MODE_UPDATE).
  * Instances must keep their mesh; `instances` is a host array. */
 int ane_scene_refit(ane_scene* scene, const ane_instance* instances, int32_t num_instances,
@@ -209,6 +241,11 @@ int ane_scene_refit(ane_scene* scene, const ane_instance* instances, int32_t num
 int ane_scene_rebuild(ane_scene* scene, const ane_instance* instances, int32_t num_instances,
                       void* stream);
 int ane_scene_info_get(const ane_scene* scene, ane_scene_info* out);
+/* Waits for `stream`, then reads and clears the traversal fault flags of the
+ * scene's last shadow-mask and light-shadow passes (those passes report a fault
+ * by themselves only when their counters are requested): ANE_ERR_DEVICE if a
+ * ray ended without a result, else ANE_OK. */
+int ane_scene_check(ane_scene* scene, void* stream);
 int ane_scene_destroy(ane_scene* scene);
 
 /* ---- probe grid context ---- */
@@ -296,8 +333,21 @@ int ane_ddgi_debug_rays(ane_ddgi_ctx* ctx, ane_hit* hits, uint8_t* shadowed, voi
  * shadow rays take the sun grid when it is current, and their counters hold
  * cell lookups and candidate triangle tests instead. */
 int ane_ddgi_set_counting(ane_ddgi_ctx* ctx, int32_t mode);
+/* Fills *out; returns ANE_ERR_DEVICE (with *out filled) while a traversal
+ * fault is pending. */
 int ane_ddgi_stats_get(const ane_ddgi_ctx* ctx, ane_ddgi_stats* out);
+/* Resets the stage timers and clears the traversal fault flags. */
 int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
+/* Waits for `stream`, then reads and clears the context's traversal fault flags
+ * (probe traces and the last mirror pass): ANE_ERR_DEVICE if any ray since the
+ * last check ended without a result, else ANE_OK.  The reference has no such
+ * error path (its traversal is the driver's); the probe passes themselves stay
+ * asynchronous, so this is the call that surfaces it. */
+int ane_ddgi_check(ane_ddgi_ctx* ctx, void* stream);
+/* Caps the node visits of each probe ray (0, the default: 2 * triangles + 8,
+ * which a valid hierarchy never reaches).  A ray at the cap ends as a miss and
+ * raises the traversal fault: a latency guard for serving loops. */
+int ane_ddgi_set_traversal_limit(ane_ddgi_ctx* ctx, uint32_t max_node_visits);
 /* LBVH export in sorted order: keys, triangle ids, internal children (>= 0
  * internal, < 0 leaf ~position), parents, internal boxes [n-1][6] (lo, hi). */
 int ane_scene_debug_bvh(const ane_scene* scene, uint64_t* keys, uint32_t* ids, int32_t* left,

@@ -194,6 +194,64 @@ def test_production_scene_layer_parity(terrain1m, layer):
     assert rel_err(grid.atlas, atlas) <= RTOL
 
 
+def test_production_timed_path_parity(terrain1m):
+    """The update the benchmark times: full updates of all 16 layers with counting
+    off (the non-counting traversal, shadow rays through the sun grid once it is
+    built on the second update with an unchanged scene and sun).  Three updates
+    from one atlas on the GPU and on the oracle: hits (id, t, u, v) and shadow flags
+    bit-exact on every ray, radiance and atlas within 1e-4, sun grid in use.
+    Then the BF16 MFMA convolution on this traced radiance: one update with
+    hysteresis 0 from the same atlas in an FP32 and a BF16 grid; the BF16
+    irradiance is within 2e-3 of a float64 GEMM of the FP32-traced radiance and
+    the FP32 irradiance within 1e-4 (reference: irradiance_probe_update.rchit:44-66,
+    probe_conv.comp:21-61)."""
+    scene, cfg, frame, ds, osc = terrain1m
+    assert cfg.precision == PRECISION_FP32
+    grid = native.ProbeGrid(cfg)
+    grid.set_counting(0)
+    atlas = O.new_atlas(cfg)
+    cam = tuple(frame.camera_pos[:3])
+    for it in range(3):
+        fr = scenegen.make_frame(2, camera=cam, config=cfg, iteration=it)
+        print(f"[timed path] update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), it
+        for f in ("t", "u", "v"):
+            assert np.array_equal(bits(hg[f]), bits(ho[f])), (it, f)
+        assert np.array_equal(shg, sho), it
+        assert rel_err(grid.radiance(fr), rado) <= RTOL, it
+        assert rel_err(grid.atlas, atlas) <= RTOL, it
+    st = grid.stats()
+    assert st["sun_grid_state"] == 1 and st["traversal_fault"] == 0
+    assert 0.05 < (sho == 1).mean() < 0.95 and (ho["tri"] >= 0).mean() > 0.3
+    # BF16 convolution of this radiance vs float64
+    fr = scenegen.make_frame(2, camera=cam, config=cfg, iteration=3)
+    nx, ny, nz, n = cfg.probes_x, cfg.probes_y, cfg.probes_z, cfg.rays_per_probe
+    g32 = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0))
+    g16 = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0,
+                                       precision=PRECISION_BF16))
+    for g in (g32, g16):
+        g.atlas = atlas
+        g.update(ds, fr)
+    L = g32.radiance(fr).astype(np.float64)
+    d = O.rotate_dirs(fr.rotation, O.fibonacci_dirs(n)).astype(np.float64)
+    W = np.maximum(0.0, O.texel_dirs().astype(np.float64) @ d.T)
+    W[W <= 1e-6] = 0
+    P = nx * ny * nz
+    ref = ((L.reshape(P * 3, n) @ W.T) / W.sum(1)).reshape(P, 3, 64).transpose(0, 2, 1)
+    E32 = _interior(g32.atlas, cfg)[..., :3].astype(np.float64)
+    E16 = _interior(g16.atlas, cfg)[..., :3].astype(np.float64)
+    floor = 1e-3
+    rel32 = np.abs(E32 - ref) / np.maximum(np.abs(ref), floor)
+    rel16 = np.abs(E16 - ref) / np.maximum(np.abs(ref), floor)
+    print(f"[timed path] terrain1m irradiance vs float64: fp32 max {rel32.max():.3e}, "
+          f"bf16 max {rel16.max():.3e} rms {np.sqrt((rel16 ** 2).mean()):.3e}", flush=True)
+    assert rel32.max() <= RTOL
+    assert rel16.max() <= BF16_MAX_REL
+
+
 def test_room8_matches_golden_fixture():
     here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
     meta = json.load(open(os.path.join(here, "room8.json")))

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 from anerend_amd import native, scenegen
-from anerend_amd.types import (INSTANCE_DTYPE, MATERIAL_DTYPE, MESH_DTYPE, Config, Frame,
-                               PRECISION_BF16, ptr)
+from anerend_amd.types import (GPU_MESH_INFO_DTYPE, INSTANCE_DTYPE, MATERIAL_DTYPE, MESH_DTYPE, Config,
+                               Frame, PRECISION_BF16, ptr)
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -64,6 +64,8 @@ int main(void) {
          sizeof(ane_mesh), sizeof(ane_material), sizeof(ane_instance), sizeof(ane_hit));
   printf("%zu %zu %zu %zu\n", offsetof(ane_ddgi_frame, rotation), offsetof(ane_ddgi_frame, bake_tile),
          offsetof(ane_instance, custom_index_and_mask), offsetof(ane_ddgi_config, precision));
+  printf("%zu %zu %zu\n", sizeof(ane_gpu_mesh_info), offsetof(ane_gpu_mesh_info, vertex_offset),
+         offsetof(ane_gpu_mesh_info, blas_ref));
   return 0;
 }''')
     exe = tmp_path / "probe"
@@ -72,8 +74,11 @@ int main(void) {
     sizes = [int(x) for x in out]
     assert sizes[:6] == [ctypes.sizeof(Config), ctypes.sizeof(Frame), MESH_DTYPE.itemsize,
                          MATERIAL_DTYPE.itemsize, INSTANCE_DTYPE.itemsize, 16]
-    assert sizes[6:] == [Frame.rotation.offset, Frame.bake_tile.offset,
-                         INSTANCE_DTYPE.fields["custom_index_and_mask"][1], Config.precision.offset]
+    assert sizes[6:10] == [Frame.rotation.offset, Frame.bake_tile.offset,
+                           INSTANCE_DTYPE.fields["custom_index_and_mask"][1], Config.precision.offset]
+    # GPUMeshInfo (GpuBuffers.h:38-44): two vec4, two uint32, uint64 _blasRef = 48 B
+    assert sizes[10:] == [48, 32, 40] and GPU_MESH_INFO_DTYPE.itemsize == 48
+    assert GPU_MESH_INFO_DTYPE.fields["blas_ref"][1] == 40
 
 
 def test_serve_layouts_and_validation(tmp_path):
@@ -156,6 +161,48 @@ def test_scene_validation_without_device():
     bad_x = scenegen.Scene(s.vertices, s.indices, s.meshes, s.materials, s.instances.copy())
     bad_x.instances["transform"][0, 0] = np.nan
     assert create(bad_x) == 1
+
+
+def gpu_mesh_info(scene, seed=7):
+    """The scene's meshes as the reference fills GPUMeshInfo (VulkanRenderer.cpp:3486-3493):
+    bounds, offsets and a BLAS device address (random, non-zero in both 32-bit halves),
+    plus the index counts the reference keeps on the host (InternalMesh::_numIndices)."""
+    gm = np.zeros(len(scene.meshes), GPU_MESH_INFO_DTYPE)
+    for f in ("min_pos", "max_pos", "vertex_offset", "index_offset"):
+        gm[f] = scene.meshes[f]
+    rng = np.random.default_rng(seed)
+    gm["blas_ref"] = rng.integers(1 << 32, 1 << 63, len(gm), dtype=np.uint64) | np.uint64(0x5)
+    return gm, scene.meshes["index_count"].astype(np.uint32).copy()
+
+
+def test_gpu_mesh_info_validation_without_device():
+    """ane_scene_create_mesh_info takes GPUMeshInfo as it is; a GPUMeshInfo array
+    passed to ane_scene_create is refused (its _blasRef lands in index_count and
+    the reserved word, which must be 0), and every check runs before device work."""
+    L = native.load()
+    s = scenegen.room_scene(1, num_boxes=2)
+    gm, cnt = gpu_mesh_info(s)
+    v, i, mt, inst = (np.ascontiguousarray(a) for a in (s.vertices, s.indices, s.materials, s.instances))
+    h = ctypes.c_void_p()
+
+    def create_info(gm_, cnt_):
+        return L.ane_scene_create_mesh_info(ptr(v), len(v), ptr(i), len(i), ptr(gm_), ptr(cnt_), len(gm_),
+                                            ptr(mt), len(mt), ptr(inst), len(inst), None, ctypes.byref(h))
+
+    as_mesh = np.ascontiguousarray(gm).view(MESH_DTYPE)   # the cast the old binding made
+    assert as_mesh["reserved"].min() > 0
+    rc = L.ane_scene_create(ptr(v), len(v), ptr(i), len(i), ptr(as_mesh), len(as_mesh), ptr(mt), len(mt),
+                            ptr(inst), len(inst), None, ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert L.ane_scene_create_mesh_info(ptr(v), len(v), ptr(i), len(i), ptr(gm), None, len(gm),
+                                        ptr(mt), len(mt), ptr(inst), len(inst), None, ctypes.byref(h)) == 1
+    bad = cnt.copy()
+    bad[0] += 1
+    assert create_info(gm, bad) == 1 and b"not a multiple of 3" in L.ane_last_error()
+    bad = cnt.copy()
+    bad[-1] += 3 * 10_000
+    assert create_info(gm, bad) == 1 and b"exceed the index array" in L.ane_last_error()
+    assert not h.value
 
 
 def test_status_strings():
