@@ -1028,12 +1028,86 @@ static f3 sample_single_probe(const ora_config* c, const float* atlas, int ix, i
   return bilinear_repeat(atlas, W, H, u, v);
 }
 
-/* pbr_light.glsl:240-363 (no baked tiles: worldPosToTileInfo lookups return -1) */
-static f3 sample_probe(const ora_config* c, const ora_frame* f, const float* atlas, f3 pos, f3 n)
+/* ------------------------------------------------------------------------ */
+/* Baked tile atlases: Scene::setDDGIAtlas (Scene.cpp:409-438) registers a    */
+/* baked RGBA16F atlas per world tile; prefillGPUTileInfoBuffer               */
+/* (VulkanRenderer.cpp:3698-3738) writes the camera-relative window of        */
+/* GPUTileInfo (GpuBuffers.h:74-76) the shaders read through                  */
+/* worldPosToTileInfo (bindless.glsl:291-308).  Contract for Q14: the window  */
+/* radius is ORA_TILE_RADIUS = 2 (MAX_PAGE_TILE_RADIUS, VulkanRenderer.h:267, */
+/* the buffer the reference fills) on both sides, with floor() for the        */
+/* camera tile as in the shader; beyond it the lookup returns -1.  Test-only  */
+/* global state: the tiles hold their atlas widened to binary32.              */
+/* ------------------------------------------------------------------------ */
+#define ORA_TILE_RADIUS 2
+static struct {
+  int32_t n, size;
+  int32_t* xz;
+  float* atlas; /* [n][H][W][4] */
+  int64_t texels;
+} g_tiles = {0, 32, NULL, NULL, 0};
+
+void ora_tiles_set(const ora_config* c, const int32_t* tile_xz, const float* atlases, int32_t n,
+                   int32_t tile_size)
+{
+  free(g_tiles.xz);
+  free(g_tiles.atlas);
+  g_tiles.xz = NULL;
+  g_tiles.atlas = NULL;
+  g_tiles.n = 0;
+  g_tiles.size = tile_size;
+  if (n <= 0) return;
+  int64_t texels = ora_atlas_width(c) * ora_atlas_height(c);
+  g_tiles.xz = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)n);
+  g_tiles.atlas = (float*)malloc(sizeof(float) * 4 * (size_t)(texels * n));
+  memcpy(g_tiles.xz, tile_xz, sizeof(int32_t) * 2 * (size_t)n);
+  memcpy(g_tiles.atlas, atlases, sizeof(float) * 4 * (size_t)(texels * n));
+  g_tiles.texels = texels;
+  g_tiles.n = n;
+}
+
+/* worldPosToTileInfo + tileInfoBuffer.tiles[i].ddgiAtlasTex: the baked atlas of
+ * the tile holding pos, or NULL; *tx, *tz receive the tile index */
+static const float* tile_atlas(const ora_frame* f, f3 pos, int* tx, int* tz)
+{
+  const float ts = (float)g_tiles.size;
+  *tx = (int)floorf(pos.x / ts);
+  *tz = (int)floorf(pos.z / ts);
+  if (g_tiles.n == 0) return NULL;
+  int cx = (int)floorf(f->camera_pos[0] / ts), cz = (int)floorf(f->camera_pos[2] / ts);
+  int rx = *tx - cx, rz = *tz - cz;
+  if (rx > ORA_TILE_RADIUS || rz > ORA_TILE_RADIUS || rx < -ORA_TILE_RADIUS || rz < -ORA_TILE_RADIUS)
+    return NULL;
+  for (int32_t k = 0; k < g_tiles.n; ++k)
+    if (g_tiles.xz[2 * k] == *tx && g_tiles.xz[2 * k + 1] == *tz)
+      return g_tiles.atlas + 4 * g_tiles.texels * k;
+  return NULL;
+}
+
+/* pbr_light.glsl:240-363.  With a baked tile at pos the probes are placed on the
+ * tile's grid (:258-275); the texture sampled is `atlas` unless tile_tex is set,
+ * in which case it is the tile's own atlas: the deferred lookup passes the baked
+ * texture (deferred_tiled.comp:276-283), the probe and mirror closest-hit shaders
+ * pass the live atlas (rchit:99-106). */
+static f3 sample_probe_t(const ora_config* c, const ora_frame* f, const float* atlas, f3 pos, f3 n,
+                         int tile_tex)
 {
   int px, py, pz;
   f3 base, step;
-  if (f->flags & ORA_FLAG_BAKE) {
+  int tx, tz;
+  const float* baked = tile_atlas(f, pos, &tx, &tz);
+  if (baked) {
+    /* pbr_light.glsl:258-275 */
+    int tsize = g_tiles.size;
+    f3 toff = mk((float)tx * (float)tsize, 0.0f, (float)tz * (float)tsize);
+    step = mk((float)tsize / (float)c->probes_x, 1.0f, (float)tsize / (float)c->probes_z);
+    f3 pit = sub(pos, toff);
+    px = (int)floorf(pit.x / step.x); px = px < 0 ? 0 : (px > c->probes_x - 1 ? c->probes_x - 1 : px);
+    py = (int)floorf(pit.y / step.y); py = py < 0 ? 0 : (py > c->probes_y - 1 ? c->probes_y - 1 : py);
+    pz = (int)floorf(pit.z / step.z); pz = pz < 0 ? 0 : (pz > c->probes_z - 1 ? c->probes_z - 1 : pz);
+    base = add(mul(mk((float)px, (float)py, (float)pz), step), toff);
+    if (tile_tex) atlas = baked;
+  } else if (f->flags & ORA_FLAG_BAKE) {
     /* pbr_light.glsl:276-293 */
     int tsize = f->bake_tile[2];
     f3 toff = mk((float)f->bake_tile[0] * (float)tsize, 0.0f, (float)f->bake_tile[1] * (float)tsize);
@@ -1087,12 +1161,12 @@ static f3 sample_probe(const ora_config* c, const ora_frame* f, const float* atl
 void ora_sample_probe(const ora_config* c, const ora_frame* f, const float* atlas, const float* pos,
                       const float* normal, float* out)
 {
-  st3(out, sample_probe(c, f, atlas, ld3(pos), ld3(normal)));
+  st3(out, sample_probe_t(c, f, atlas, ld3(pos), ld3(normal), 0));
 }
 
 /* pbr_light.glsl:365-388 */
-static f3 calc_indirect(const ora_config* c, const ora_frame* f, const float* atlas, f3 N,
-                        f3 albedo, float metallic, float rough, f3 pos, f3 view)
+static f3 calc_indirect_t(const ora_config* c, const ora_frame* f, const float* atlas, f3 N,
+                          f3 albedo, float metallic, float rough, f3 pos, f3 view, int tile_tex)
 {
   f3 V = normalize(sub(view, pos));
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
@@ -1103,12 +1177,18 @@ static f3 calc_indirect(const ora_config* c, const ora_frame* f, const float* at
              F0.y + (fmaxf(omr, F0.y) - F0.y) * p5,
              F0.z + (fmaxf(omr, F0.z) - F0.z) * p5);
   f3 kD = muls(sub(mk(1.0f, 1.0f, 1.0f), kS), 1.0f - metallic);
-  f3 irr = sample_probe(c, f, atlas, pos, N);
+  f3 irr = sample_probe_t(c, f, atlas, pos, N, tile_tex);
   f3 diffuse = mul(irr, albedo);
   return mul(kD, diffuse);
 }
 
-/* deferred_tiled.comp:264-298 (live atlas; baked tiles are out of scope):
+static f3 calc_indirect(const ora_config* c, const ora_frame* f, const float* atlas, f3 N,
+                        f3 albedo, float metallic, float rough, f3 pos, f3 view)
+{
+  return calc_indirect_t(c, f, atlas, N, albedo, metallic, rough, pos, view, 0);
+}
+
+/* deferred_tiled.comp:264-298 (live atlas, or the baked atlas of the texel's tile):
  * gbuf = n x 12 floats (pos, shadow, normal, metallic, albedo, roughness),
  * out = n x 4 floats (ambient rgb, 0). */
 void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, const float* gbuf,
@@ -1120,14 +1200,16 @@ void ora_gi_lookup(const ora_config* c, const ora_frame* f, const float* atlas, 
     f3 pos = mk(t[0], t[1], t[2]), N = mk(t[4], t[5], t[6]), albedo = mk(t[8], t[9], t[10]);
     float sh = clampf(t[3], 0.3f, 1.0f);
     f3 amb;
-    if (f->flags & ORA_FLAG_DDGI) {
+    int tx, tz;
+    const int baked = tile_atlas(f, pos, &tx, &tz) != NULL; /* deferred_tiled.comp:265-268 */
+    if ((f->flags & ORA_FLAG_DDGI) || baked) {
       f3 cam = mk(f->camera_pos[0], f->camera_pos[1], f->camera_pos[2]);
       f3 d = sub(cam, pos);
       float dist = sqrtf(dot(d, d)); /* distance(ubo.cameraPos.xyz, fragWorldPos) */
-      if (dist > (float)c->probes_x * c->probe_step[0] / 2.0f)
+      if (!baked && dist > (float)c->probes_x * c->probe_step[0] / 2.0f)
         amb = muls(mul(muls(mk(0.3f, 0.3f, 0.3f), f->sky_intensity), albedo), sh);
       else
-        amb = calc_indirect(c, f, atlas, N, albedo, t[7], t[11], pos, cam);
+        amb = calc_indirect_t(c, f, atlas, N, albedo, t[7], t[11], pos, cam, 1);
     } else {
       amb = muls(mul(mk(0.3f, 0.3f, 0.3f), albedo), sh);
     }

@@ -213,6 +213,14 @@ void ora_shade_hit(const ora_scene* s, const ora_config* cfg, const ora_frame* f
 void ora_sample_probe(const ora_config* cfg, const ora_frame* fr, const float* atlas,
                       const float* pos, const float* normal, float* out3);
 
+/* Baked atlases of world tiles (Scene::setDDGIAtlas, Scene.cpp:409-438): tile_xz
+ * [n][2] tile indices (x, z), atlases [n][H][W][4] binary32 (the RGBA16F texels
+ * widened), tile_size in metres (TILE_SIZE, bindless.glsl:5).  Replaces the set
+ * (n = 0 clears).  Global state of the oracle, read by sampleProbe and the
+ * deferred lookup; window radius 2 around the camera tile (see ddgi_oracle.c). */
+void ora_tiles_set(const ora_config* cfg, const int32_t* tile_xz, const float* atlases, int32_t n,
+                   int32_t tile_size);
+
 /* binary32 -> binary16 (round to nearest, ties to even; >= 65520 -> inf) -> binary32 */
 void ora_f16_round(const float* in, float* out, int64_t n);
 

@@ -67,6 +67,7 @@ def lib():
         L.ora_update.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.ora_atlas_clear.argtypes = [vp, vp]
         L.ora_f16_round.argtypes = [vp, vp, i64]
+        L.ora_tiles_set.argtypes = [vp, vp, vp, i32, i32]
         L.ora_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -303,6 +304,14 @@ def f16_round(x):
     out = np.empty_like(x)
     lib().ora_f16_round(ptr(x), ptr(out), x.size)
     return out
+
+
+def tiles_set(cfg, tile_xz, atlases, tile_size=32):
+    """Baked tile atlases: tile_xz [n][2], atlases [n][H][W][4] (RGBA16F or float32;
+    binary16 input is widened exactly).  An empty list clears the set."""
+    xz = np.ascontiguousarray(tile_xz, np.int32).reshape(-1, 2)
+    a = np.ascontiguousarray(np.asarray(atlases).astype(np.float32)).reshape(len(xz), *cfg.atlas_shape)
+    lib().ora_tiles_set(_ref(cfg), ptr(xz), ptr(a), len(xz), int(tile_size))
 
 
 def new_atlas(cfg):
