@@ -814,6 +814,11 @@ struct ane_ddgi_ctx {
   float* d_rad_scale = nullptr;
   float* d_stage = nullptr;  // FP32 staging for radiance get/set
   float2* d_tiles = nullptr;  // tile texture bounds: nx + 1 columns, then (ny + 1) * (nz + 1) rows
+  // baked tile atlases (ane_ddgi_tiles_set): RGBA16F texels per tile, tile indices
+  uint2* d_baked = nullptr;
+  std::vector<int32_t> baked_xz;
+  int32_t baked_size = 32;
+  int64_t baked_bytes = 0;
   uint32_t* d_work = nullptr;
   unsigned long long* d_stats = nullptr;
   uint32_t* d_overflow = nullptr;  // sticky traversal fault of the probe traces (ane_ddgi_check)
@@ -850,6 +855,7 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_order); dfree(c->d_hits); dfree(c->d_shadow); dfree(c->d_rad); dfree(c->d_rad_scale);
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
   dfree(c->d_tiles);
+  dfree(c->d_baked);
   c->sgc.release();
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
@@ -1016,6 +1022,24 @@ static int frame_to_k(const ane_ddgi_ctx* c, const ane_ddgi_frame* f, FrameK* k)
   k->sun_l[1] = L.y;
   k->sun_l[2] = L.z;
   k->sky_intensity = f->sky_intensity;
+  // the camera's window of baked tiles (prefillGPUTileInfoBuffer, VulkanRenderer.cpp:3698-3738)
+  const int R = ANE_TILE_RADIUS, w = 2 * ANE_TILE_RADIUS + 1;
+  k->tile_size = c->baked_size;
+  k->tile_count = 0;
+  for (int i = 0; i < w * w; ++i) k->tile_win[i] = -1;
+  const float cx = std::floor(f->camera_pos[0] / (float)c->baked_size);
+  const float cz = std::floor(f->camera_pos[2] / (float)c->baked_size);
+  if (!c->baked_xz.empty() && (std::fabs(cx) > 1e9f || std::fabs(cz) > 1e9f))
+    return fail(ANE_ERR_INVALID_ARG, "camera too far out for the tile index");
+  k->tile_cam[0] = c->baked_xz.empty() ? 0 : (int32_t)cx;
+  k->tile_cam[1] = c->baked_xz.empty() ? 0 : (int32_t)cz;
+  for (size_t t = 0; t < c->baked_xz.size() / 2; ++t) {
+    const int64_t rx = (int64_t)c->baked_xz[2 * t] - k->tile_cam[0];
+    const int64_t rz = (int64_t)c->baked_xz[2 * t + 1] - k->tile_cam[1];
+    if (rx < -R || rx > R || rz < -R || rz > R) continue;
+    k->tile_win[(rz + R) * w + rx + R] = (int32_t)t;
+    ++k->tile_count;
+  }
   return ANE_OK;
 }
 
@@ -1232,6 +1256,37 @@ extern "C" int ane_ddgi_atlas_import_rgba16f(ane_ddgi_ctx* c, const uint16_t* ho
   launch_half_to_atlas(stage, reinterpret_cast<float4*>(c->d_atlas[c->cur]), c->texels, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+extern "C" int ane_ddgi_tiles_set(ane_ddgi_ctx* c, const int32_t* tile_xz, const uint16_t* rgba16f,
+                                  int32_t num_tiles, int32_t tile_size, void* stream)
+{
+  if (!c) return fail(ANE_ERR_INVALID_ARG, "NULL context");
+  if (num_tiles < 0 || num_tiles > 4096) return fail(ANE_ERR_INVALID_ARG, "num_tiles must be in [0, 4096]");
+  if (tile_size <= 0) return fail(ANE_ERR_INVALID_ARG, "tile_size must be positive");
+  if (num_tiles && (!tile_xz || !rgba16f)) return fail(ANE_ERR_INVALID_ARG, "NULL tiles with a non-zero count");
+  for (int32_t a = 0; a < num_tiles; ++a)
+    for (int32_t b = 0; b < a; ++b)
+      if (tile_xz[2 * a] == tile_xz[2 * b] && tile_xz[2 * a + 1] == tile_xz[2 * b + 1])
+        return fail(ANE_ERR_INVALID_ARG, "tile (%d, %d) given twice", tile_xz[2 * a], tile_xz[2 * a + 1]);
+  hipStream_t st = S(stream);
+  // the previous set may still be read by queued work on this stream
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  dfree(c->d_baked);
+  c->bytes -= c->baked_bytes;
+  c->baked_bytes = 0;
+  c->baked_xz.clear();
+  c->g.tile_atlas = nullptr;
+  c->baked_size = tile_size;
+  if (num_tiles == 0) return ANE_OK;
+  ANE_TRY(dalloc(&c->d_baked, c->texels * num_tiles, &c->baked_bytes));
+  c->bytes += c->baked_bytes;
+  ANE_CHECK_HIP(hipMemcpyAsync(c->d_baked, rgba16f, sizeof(uint2) * c->texels * num_tiles,
+                               hipMemcpyHostToDevice, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  c->baked_xz.assign(tile_xz, tile_xz + 2 * num_tiles);
+  c->g.tile_atlas = c->d_baked;
   return ANE_OK;
 }
 

@@ -66,7 +66,16 @@ struct FrameK {
   float rot[16];
   float sun_l[3];      // normalize(-light_dir): direction to the sun (pbr_light.glsl:103)
   float sky_intensity; // ubo.skyIntensity as given (deferred_tiled.comp:272)
+  // baked tile atlases (worldPosToTileInfo, bindless.glsl:291-308): the window of
+  // (2 * ANE_TILE_RADIUS + 1)^2 tiles around the camera's tile, as the renderer
+  // fills GPUTileInfo each frame (VulkanRenderer.cpp:3698-3738)
+  int32_t tile_cam[2];   // floor(camera x, z / tile_size)
+  int32_t tile_size;     // TILE_SIZE, metres
+  int32_t tile_count;    // window entries that hold a baked atlas (0: no lookup)
+  int32_t tile_win[25];  // baked atlas of tile cam + (i % 5 - 2, i / 5 - 2), -1 none
 };
+
+#define ANE_TILE_RADIUS 2  // MAX_PAGE_TILE_RADIUS, VulkanRenderer.h:267
 
 struct GridK {
   int32_t nx, ny, nz;
@@ -80,6 +89,8 @@ struct GridK {
   // y * (nz + 1) + z for y 0..ny, z 0..nz; device memory
   const float2* tile_u;
   const float2* tile_v;
+  // baked tile atlases: [tiles][atlas_h][atlas_w] RGBA16F texels (4 x binary16)
+  const uint2* tile_atlas;
 };
 
 // Device view of a scene (all pointers device memory).

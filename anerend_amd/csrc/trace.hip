@@ -625,11 +625,13 @@ static constexpr float kPi = 3.14159265359f;  // pbr_light.glsl:8
 // Hardware reciprocal, reciprocal square root and square root (1 ulp) for the
 // shading's continuous terms.  Radiance carries a relative tolerance; no branch
 // of the shading depends on these values, and the hit positions (shadow-ray
-// origins) keep correctly rounded arithmetic.
+// origins) keep correctly rounded arithmetic.  The view and half vectors of
+// calcLight are NOT among them: GGX's 1 + NdotH^2 (a^2 - 1) cancels near the
+// specular peak (a^2 = 0.0016 at roughness 0.2), which turns one ulp of NdotH
+// into 1.5e-4 of radiance, so V and H keep the correctly rounded normalize.
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float hw_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ f3 hw_normalize(f3 v) { return muls(v, hw_rsqrt(dot(v, v))); }
 
 // toLinear (bindless.glsl:240-247) with the power as hardware exp2(2.4 * log2 x)
 // (about 1e-6 relative on the branch's range x >= 0.09): a continuous term of
@@ -672,7 +674,7 @@ __device__ f3 calc_light(const FrameK& f, f3 N, f3 albedo, float metallic, float
   f3 light_color = muls(mk(1.0f, 1.0f, 1.0f), f.sun_intensity);
   f3 F0 = add(muls(mk(0.04f, 0.04f, 0.04f), 1.0f - metallic), muls(albedo, metallic));
   f3 L = mk(f.sun_l[0], f.sun_l[1], f.sun_l[2]);  // normalize(-lightDir), once per frame
-  f3 H = hw_normalize(add(V, L));
+  f3 H = normalize(add(V, L));  // correctly rounded: NdotH feeds GGX's cancellation
   f3 radiance = muls(light_color, 1.0f);
   float ndf = distribution_ggx(N, H, rough);
   float ndotv_ = fmaxf(dot(N, V), 0.0f);
@@ -710,9 +712,19 @@ __device__ __forceinline__ int wrap_repeat(int a, int m)
   return r;
 }
 
+// an atlas texel: FP32 RGBA (the live atlas) or RGBA16F widened exactly (a baked tile)
+__device__ __forceinline__ float4 texel(const float4* a, int64_t i) { return a[i]; }
+__device__ __forceinline__ float h2f(uint32_t b) { return (float)__builtin_bit_cast(_Float16, (uint16_t)b); }
+__device__ __forceinline__ float4 texel(const uint2* a, int64_t i)
+{
+  const uint2 t = a[i];
+  return make_float4(h2f(t.x & 0xFFFFu), h2f(t.x >> 16), h2f(t.y & 0xFFFFu), h2f(t.y >> 16));
+}
+
 // texture() with LINEAR filtering and REPEAT addressing (PipelineUtil.h:98-112),
 // binary32 weights.
-__device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float v)
+template <class T>
+__device__ f3 bilinear_repeat(const T* atlas, int W, int H, float u, float v)
 {
   float x = u * (float)W - 0.5f;
   float y = v * (float)H - 0.5f;
@@ -720,10 +732,10 @@ __device__ f3 bilinear_repeat(const float4* atlas, int W, int H, float u, float 
   float ax = x - fx, ay = y - fy;
   int x0 = wrap_repeat((int)fx, W), x1 = wrap_repeat((int)fx + 1, W);
   int y0 = wrap_repeat((int)fy, H), y1 = wrap_repeat((int)fy + 1, H);
-  float4 t00 = atlas[(int64_t)y0 * W + x0];
-  float4 t10 = atlas[(int64_t)y0 * W + x1];
-  float4 t01 = atlas[(int64_t)y1 * W + x0];
-  float4 t11 = atlas[(int64_t)y1 * W + x1];
+  float4 t00 = texel(atlas, (int64_t)y0 * W + x0);
+  float4 t10 = texel(atlas, (int64_t)y0 * W + x1);
+  float4 t01 = texel(atlas, (int64_t)y1 * W + x0);
+  float4 t11 = texel(atlas, (int64_t)y1 * W + x1);
   float r0 = t00.x * (1.0f - ax) + t10.x * ax, r1 = t01.x * (1.0f - ax) + t11.x * ax;
   float g0 = t00.y * (1.0f - ax) + t10.y * ax, g1 = t01.y * (1.0f - ax) + t11.y * ax;
   float b0 = t00.z * (1.0f - ax) + t10.z * ax, b1 = t01.z * (1.0f - ax) + t11.z * ax;
@@ -739,14 +751,38 @@ __device__ __forceinline__ float tile_coord(float2 t, float o) { return t.x + t.
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-// sampleProbe, pbr_light.glsl:240-363 (no baked tiles)
+// worldPosToTileInfo (bindless.glsl:291-308) + tileInfoBuffer[i].ddgiAtlasTex: the
+// baked atlas of the tile holding pos within the camera's window, or -1.  The
+// window has radius ANE_TILE_RADIUS on both sides (the reference's shader checks
+// only the upper side of a radius-10 window against a radius-2 buffer, Q14).
+__device__ __forceinline__ int tile_lookup(const FrameK& f, f3 pos, int& tx, int& tz)
+{
+  const float ts = (float)f.tile_size;
+  tx = (int)floorf(pos.x / ts);
+  tz = (int)floorf(pos.z / ts);
+  const int rx = tx - f.tile_cam[0], rz = tz - f.tile_cam[1];
+  if (rx < -ANE_TILE_RADIUS || rx > ANE_TILE_RADIUS || rz < -ANE_TILE_RADIUS || rz > ANE_TILE_RADIUS)
+    return -1;
+  return f.tile_win[(rz + ANE_TILE_RADIUS) * (2 * ANE_TILE_RADIUS + 1) + rx + ANE_TILE_RADIUS];
+}
+
+// sampleProbe, pbr_light.glsl:240-363.  A baked tile at pos places the probes on
+// the tile's grid (:251-275); TILE_TEX then also samples the tile's own atlas
+// (the deferred lookup, deferred_tiled.comp:276-283), otherwise `atlas` (the
+// closest-hit shaders pass the live atlas, rchit:99-106).
+template <bool TILE_TEX>
 __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas, f3 pos, f3 n)
 {
   int px, py, pz;
   f3 base, step;
-  if (f.flags & ANE_FLAG_BAKE) {
-    int ts = f.bake_tile[2];
-    f3 toff = mk((float)f.bake_tile[0] * (float)ts, 0.0f, (float)f.bake_tile[1] * (float)ts);
+  int tx = 0, tz = 0;
+  const int ti = f.tile_count > 0 ? tile_lookup(f, pos, tx, tz) : -1;
+  if (ti >= 0 || (f.flags & ANE_FLAG_BAKE)) {
+    // a baked tile's grid (:258-275), else the tile being baked (:276-293)
+    const bool tiled = ti >= 0;
+    const int ts = tiled ? f.tile_size : f.bake_tile[2];
+    const int ix = tiled ? tx : f.bake_tile[0], iz = tiled ? tz : f.bake_tile[1];
+    f3 toff = mk((float)ix * (float)ts, 0.0f, (float)iz * (float)ts);
     step = mk((float)ts / (float)g.nx, 1.0f, (float)ts / (float)g.nz);
     f3 pit = sub(pos, toff);
     px = clampi((int)floorf(pit.x / step.x), 0, g.nx - 1);
@@ -797,7 +833,9 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
     if (w < 0.2f) w = w * (w * w * (1.0f / (0.2f * 0.2f)));
     w = w * (tri.x * tri.y * tri.z);
     const float tvs = oy ? (oz ? tv[3] : tv[2]) : (oz ? tv[1] : tv[0]);
-    f3 s = bilinear_repeat(atlas, W, H, ox ? tu[1] : tu[0], tvs);
+    f3 s = (TILE_TEX && ti >= 0)
+               ? bilinear_repeat(g.tile_atlas + (int64_t)ti * W * H, W, H, ox ? tu[1] : tu[0], tvs)
+               : bilinear_repeat(atlas, W, H, ox ? tu[1] : tu[0], tvs);
     s = mk(hw_sqrt(s.x), hw_sqrt(s.y), hw_sqrt(s.z));
     sumi = add(sumi, muls(s, w));
     sumw = sumw + w;
@@ -808,6 +846,7 @@ __device__ f3 sample_probe(const GridK& g, const FrameK& f, const float4* atlas,
 }
 
 // calcIndirectDiffuseLight, pbr_light.glsl:365-388
+template <bool TILE_TEX = false>
 __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas, f3 N, f3 albedo,
                             float metallic, float rough, f3 pos, f3 V)
 {
@@ -818,7 +857,7 @@ __device__ f3 calc_indirect(const GridK& g, const FrameK& f, const float4* atlas
   f3 kS = mk(F0.x + (fmaxf(omr, F0.x) - F0.x) * p5, F0.y + (fmaxf(omr, F0.y) - F0.y) * p5,
              F0.z + (fmaxf(omr, F0.z) - F0.z) * p5);
   f3 kD = muls(sub(mk(1.0f, 1.0f, 1.0f), kS), 1.0f - metallic);
-  f3 irr = sample_probe(g, f, atlas, pos, N);
+  f3 irr = sample_probe<TILE_TEX>(g, f, atlas, pos, N);
   f3 diffuse = mul(irr, albedo);
   return mul(kD, diffuse);
 }
@@ -874,7 +913,7 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   const float rough = mat.met_rough[1];
   f3 emissive = muls(mk(mat.emissive[0], mat.emissive[1], mat.emissive[2]), mat.emissive[3]);
   f3 acc = mk(0.0f, 0.0f, 0.0f);
-  const f3 V = hw_normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
+  const f3 V = normalize(sub(origin, pos));  // view vector of both terms (pbr_light.glsl:101,369)
   if (lit) acc = calc_light(f, nrm, albedo, metallic, rough, V);
   if (f.flags & (MIRROR ? ANE_FLAG_DDGI : ANE_FLAG_DDGI_MULTI))
     acc = add(acc, calc_indirect(g, f, atlas, nrm, albedo, metallic, rough, pos, V));
@@ -991,7 +1030,8 @@ void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float
 
 // ---------------------------------------------------------------------------
 // deferred GI lookup (deferred_tiled.comp:264-298): ambient light of G-buffer
-// texels from the atlas, through the shading's calcIndirectDiffuseLight
+// texels from the atlas, or the baked atlas of the texel's tile, through the
+// shading's calcIndirectDiffuseLight
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_gi_lookup(GridK g, FrameK f, const float4* atlas,
                                                    const float4* gbuf, int64_t n, float4* out)
@@ -1002,15 +1042,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_W
   const f3 pos = mk(a.x, a.y, a.z), N = mk(b.x, b.y, b.z), albedo = mk(c.x, c.y, c.z);
   const float sh = clampf(a.w, 0.3f, 1.0f);
   f3 amb;
-  if (f.flags & ANE_FLAG_DDGI) {
+  // a baked atlas for the texel's tile (deferred_tiled.comp:265-268) switches the
+  // lookup on without ANE_FLAG_DDGI and lifts the distance cut-off
+  int tx, tz;
+  const bool baked = f.tile_count > 0 && tile_lookup(f, pos, tx, tz) >= 0;
+  if ((f.flags & ANE_FLAG_DDGI) || baked) {
     const f3 cam = mk(f.cam[0], f.cam[1], f.cam[2]);
     const f3 d = sub(cam, pos);
     const float dist = sqrtf(dot(d, d));
-    if (dist > (float)g.nx * g.step[0] / 2.0f) {
+    if (!baked && dist > (float)g.nx * g.step[0] / 2.0f) {
       amb = muls(mul(muls(mk(0.3f, 0.3f, 0.3f), f.sky_intensity), albedo), sh);
     } else {
       const f3 V = normalize(sub(cam, pos));
-      amb = calc_indirect(g, f, atlas, N, albedo, b.w, c.w, pos, V);
+      amb = calc_indirect<true>(g, f, atlas, N, albedo, b.w, c.w, pos, V);
     }
   } else {
     amb = muls(mul(mk(0.3f, 0.3f, 0.3f), albedo), sh);

@@ -87,6 +87,7 @@ _SIGS = [
     ("ane_ddgi_stats_get", ctypes.c_int, [_vp, _vp]),
     ("ane_ddgi_stats_reset", ctypes.c_int, [_vp]),
     ("ane_ddgi_check", ctypes.c_int, [_vp, _vp]),
+    ("ane_ddgi_tiles_set", ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp]),
     ("ane_ddgi_set_traversal_limit", ctypes.c_int, [_vp, ctypes.c_uint32]),
     ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
@@ -436,6 +437,15 @@ class ProbeGrid:
     def check(self, stream=None):
         """Raises AneError if a ray since the last check ended without a result (clears it)."""
         check(load().ane_ddgi_check(self._h, stream))
+
+    def set_tiles(self, tile_xz, atlases_rgba16f, tile_size=32):
+        """Baked tile atlases: tile_xz [n][2], atlases [n][H][W][4] float16 ([] clears)."""
+        xz = np.ascontiguousarray(tile_xz, np.int32).reshape(-1, 2)
+        a = np.ascontiguousarray(atlases_rgba16f, np.float16)
+        if len(xz):
+            assert a.shape == (len(xz),) + tuple(self.config.atlas_shape)
+        check(load().ane_ddgi_tiles_set(self._h, ptr(xz), ptr(a) if len(xz) else None, len(xz),
+                                        int(tile_size), None))
 
     def set_traversal_limit(self, max_node_visits):
         check(load().ane_ddgi_set_traversal_limit(self._h, int(max_node_visits)))

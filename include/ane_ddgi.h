@@ -338,6 +338,19 @@ int ane_ddgi_set_counting(ane_ddgi_ctx* ctx, int32_t mode);
 int ane_ddgi_stats_get(const ane_ddgi_ctx* ctx, ane_ddgi_stats* out);
 /* Resets the stage timers and clears the traversal fault flags. */
 int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
+/* Baked DDGI atlases of world tiles (Scene::setDDGIAtlas, Scene.cpp:409-438; the
+ * per-frame GPUTileInfo window, GpuBuffers.h:74-76, VulkanRenderer.cpp:3698-3738).
+ * tile_xz [n][2] = tile indices (x, z) of tiles tile_size metres wide (TILE_SIZE,
+ * bindless.glsl:5: 32); rgba16f [n][H][W][4] = each tile's baked atlas, the
+ * context's atlas layout in IEEE binary16 (what ane_ddgi_atlas_export_rgba16f and
+ * downloadDDGIAtlas, VulkanRenderer.cpp:2705-2770, produce).  Replaces the set
+ * (n = 0 clears); tiles must be distinct.  From the next call on, a shading point
+ * in a tile within 2 tiles of the camera's (both axes; floor(pos / tile_size))
+ * takes its probes from that tile's grid (pbr_light.glsl:251-275): the deferred
+ * lookup samples the tile's atlas, the probe and mirror rays the live atlas (as
+ * the reference's closest-hit shaders do).  Waits for `stream` first. */
+int ane_ddgi_tiles_set(ane_ddgi_ctx* ctx, const int32_t* tile_xz, const uint16_t* rgba16f,
+                       int32_t num_tiles, int32_t tile_size, void* stream);
 /* Waits for `stream`, then reads and clears the context's traversal fault flags
  * (probe traces and the last mirror pass): ANE_ERR_DEVICE if any ray since the
  * last check ended without a result, else ANE_OK.  The reference has no such
