@@ -233,7 +233,7 @@ struct StageTimer {
   }
 };
 
-enum { kStSetup = 0, kStTrace, kStShadow, kStShade, kStConv, kStTrans, kStages };
+enum { kStSetup = 0, kStTrace, kStSunGrid, kStShade, kStConv, kStTrans, kStages };
 
 }  // namespace ane
 
@@ -266,6 +266,7 @@ struct SunGridCache {
   float* est = nullptr;       // [2 * 2048] candidate-estimate partials
   int g = 0;
   uint32_t cap = 0;
+  int64_t bytes = 0;          // device bytes of the buffers above
   struct Key {
     const void* scene = nullptr;
     uint64_t version = 0;
@@ -283,6 +284,7 @@ struct SunGridCache {
     dfree(params); dfree(acc); dfree(start); dfree(cur); dfree(list); dfree(scan); dfree(est);
     g = 0;
     cap = 0;
+    bytes = 0;
     built = Key{};
   }
   // device buffers for a scene of ntri triangles: about ANE_SUNGRID_DENSITY cells
@@ -295,13 +297,13 @@ struct SunGridCache {
     if (gg <= g && capl <= (int64_t)cap) return ANE_OK;
     release();
     const int64_t ncell = (int64_t)gg * gg;
-    ANE_TRY(dalloc(&params, 1, nullptr));
-    ANE_TRY(dalloc(&acc, 6 * 256, nullptr));
-    ANE_TRY(dalloc(&start, ncell + 1, nullptr));
-    ANE_TRY(dalloc(&cur, ncell, nullptr));
-    ANE_TRY(dalloc(&list, capl, nullptr));
-    ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, nullptr));
-    ANE_TRY(dalloc(&est, 2 * 2048, nullptr));
+    ANE_TRY(dalloc(&params, 1, &bytes));
+    ANE_TRY(dalloc(&acc, 6 * 256, &bytes));
+    ANE_TRY(dalloc(&start, ncell + 1, &bytes));
+    ANE_TRY(dalloc(&cur, ncell, &bytes));
+    ANE_TRY(dalloc(&list, capl, &bytes));
+    ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, &bytes));
+    ANE_TRY(dalloc(&est, 2 * 2048, &bytes));
     g = gg;
     cap = (uint32_t)capl;
     return ANE_OK;
@@ -312,8 +314,9 @@ struct SunGridCache {
   // components take the reciprocal of +-1e-30) culls boxes whose triangles the
   // watertight test would touch on an edge, so a grid's answer could differ
   // from the traversal's there.
+  // timer (nullable) brackets a build's launches on the stream
   int update(const SceneK& view, int32_t ntri, const void* scene, uint64_t version, const float* dir3,
-             hipStream_t st, bool* use)
+             hipStream_t st, bool* use, StageTimer* timer = nullptr)
   {
     *use = false;
     for (int k = 0; k < 3; ++k)
@@ -334,7 +337,9 @@ struct SunGridCache {
     ANE_TRY(alloc(ntri));
     SunGridParams p;
     sun_grid_setup(p, dir3, g, g);
+    if (timer) ANE_TRY(timer->begin(st));
     launch_sun_grid_build(view, p, params, acc, start, cur, list, cap, scan, est, st);
+    if (timer) ANE_TRY(timer->end(st));
     ANE_CHECK_HIP(hipGetLastError());
     built = k;
     *use = true;
@@ -883,10 +888,10 @@ static int validate_config(const ane_ddgi_config* c)
     return fail(ANE_ERR_INVALID_ARG, "hysteresis must be in [0, 1]");
   if (!(c->t_min >= 0.0f) || !(c->t_max > c->t_min))
     return fail(ANE_ERR_INVALID_ARG, "need 0 <= t_min < t_max");
-  if (c->precision < ANE_PRECISION_FP32 || c->precision > ANE_PRECISION_FP8)
+  if (c->precision < ANE_PRECISION_FP32 || c->precision > ANE_PRECISION_FP16)
     return fail(ANE_ERR_INVALID_ARG, "unknown precision %d", c->precision);
   if (c->precision != ANE_PRECISION_FP32 && c->rays_per_probe % 32 != 0)
-    return fail(ANE_ERR_UNSUPPORTED, "BF16/FP8 convolution needs rays_per_probe %% 32 == 0");
+    return fail(ANE_ERR_UNSUPPORTED, "BF16/FP16/FP8 convolution needs rays_per_probe %% 32 == 0");
   if (c->quirks != 0) return fail(ANE_ERR_UNSUPPORTED, "quirks must be 0");
   return ANE_OK;
 }
@@ -910,7 +915,7 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
   c->texels = (int64_t)g.atlas_w * g.atlas_h;
   int st = ANE_OK;
   const int64_t PN = c->P * c->N;
-  const int esz = cfg->precision == ANE_PRECISION_FP32 ? 4 : (cfg->precision == ANE_PRECISION_BF16 ? 2 : 1);
+  const int esz = cfg->precision == ANE_PRECISION_FP32 ? 4 : (cfg->precision == ANE_PRECISION_FP8 ? 1 : 2);
 #define ANE_A(p, cnt) do { st = dalloc(&c->p, (cnt), &c->bytes); if (st) { ctx_free(c); delete c; return st; } } while (0)
   ANE_A(d_atlas[0], c->texels * 4);
   ANE_A(d_atlas[1], c->texels * 4);
@@ -1058,6 +1063,30 @@ static int ensure_spill(ane_ddgi_ctx* c, int32_t depth)
 
 static bool sun_grid_wanted(const ane_ddgi_ctx* c) { return c->counting != 1 && trace_grid_supported(); }
 
+// Device bytes the context holds: its fixed buffers, baked tiles, the traversal
+// spill, the sun grid, and the mirror workspace if a mirror pass ran.
+static int64_t ctx_device_bytes(const ane_ddgi_ctx* c)
+{
+  return c->bytes + c->spill_words * 4 + c->sgc.bytes + c->mr_px * 2 * (12 + 12 + (int64_t)sizeof(ane_hit)) +
+         c->mr_spill_words * 4;
+}
+
+namespace ane {
+// Allocates ahead of time what the first update of scene `s` would allocate on
+// the context (traversal spill sized for the scene's depth, sun-grid buffers
+// sized for its triangles) and returns the context's device bytes with them, so
+// that an HBM budget charged at admission stays exact.
+int ddgi_prepare(ane_ddgi_ctx* c, const ane_scene* s, int64_t* bytes)
+{
+  if (!c || !s || !bytes) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  ANE_TRY(ensure_spill(c, s->depth));
+  if (trace_grid_supported()) ANE_TRY(c->sgc.alloc(s->ntri));
+  *bytes = ctx_device_bytes(c);
+  return ANE_OK;
+}
+int64_t ddgi_device_bytes(const ane_ddgi_ctx* c) { return ctx_device_bytes(c); }
+}  // namespace ane
+
 static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& fk)
 {
   TraceArgs a{};
@@ -1089,7 +1118,8 @@ static TraceArgs trace_args(ane_ddgi_ctx* c, const ane_scene* s, const FrameK& f
 static int do_frame_setup(ane_ddgi_ctx* c, const FrameK& fk, bool bake, hipStream_t st)
 {
   launch_frame_setup(c->g, fk, c->d_base_dirs, c->d_order, c->d_dirs, c->d_dirpre, c->d_probe_org, c->d_w32,
-                     c->d_wsum, c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, bake, st);
+                     c->d_wsum, c->d_wbf16, c->d_wfp8, c->d_wfp8_scale, bake,
+                     c->cfg.precision == ANE_PRECISION_FP16, st);
   ANE_CHECK_HIP(hipGetLastError());
   return ANE_OK;
 }
@@ -1112,7 +1142,7 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   a.work = c->d_work;
   if (sun_grid_wanted(c)) {
     bool use = false;
-    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use));
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use, &c->timers[kStSunGrid]));
     if (use) a.sg = c->sgc.view();
   }
   // primary and shadow rays in one persistent launch (the trace stage)
@@ -1256,6 +1286,71 @@ extern "C" int ane_ddgi_atlas_import_rgba16f(ane_ddgi_ctx* c, const uint16_t* ho
   launch_half_to_atlas(stage, reinterpret_cast<float4*>(c->d_atlas[c->cur]), c->texels, st);
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipStreamSynchronize(st));
+  return ANE_OK;
+}
+
+// Directions of the 8 x 8 tile texel centres in float64, t = ty * 8 + tx
+// (probe_conv.comp:37-46 with octDecode, octahedron_helpers.glsl:25-34: y and z swapped).
+static void texel_dirs_f64(double* out)
+{
+  for (int ty = 0; ty < 8; ++ty)
+    for (int tx = 0; tx < 8; ++tx) {
+      double ox = (2.0 * tx + 1.0) / 16.0 * 2.0 - 1.0, oy = (2.0 * ty + 1.0) / 16.0 * 2.0 - 1.0;
+      double z = 1.0 - std::fabs(ox) - std::fabs(oy);
+      if (z < 0.0) {
+        const double nx = (1.0 - std::fabs(oy)) * (ox >= 0.0 ? 1.0 : -1.0);
+        const double ny = (1.0 - std::fabs(ox)) * (oy >= 0.0 ? 1.0 : -1.0);
+        ox = nx;
+        oy = ny;
+      }
+      const double l = std::sqrt(ox * ox + z * z + oy * oy);
+      double* o = out + 3 * (ty * 8 + tx);
+      o[0] = ox / l;
+      o[1] = z / l;
+      o[2] = oy / l;
+    }
+}
+
+extern "C" int ane_ddgi_conv_error_fp64(ane_ddgi_ctx* conv, const ane_ddgi_ctx* ref, double rel_floor,
+                                        double* max_rel, double* rms_rel, void* stream)
+{
+  if (!conv || !ref || !max_rel || !rms_rel) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (ref->cfg.precision != ANE_PRECISION_FP32)
+    return fail(ANE_ERR_INVALID_ARG, "the reference context must hold FP32 radiance");
+  if (ref->g.nx != conv->g.nx || ref->g.ny != conv->g.ny || ref->g.nz != conv->g.nz || ref->N != conv->N)
+    return fail(ANE_ERR_INVALID_ARG, "contexts differ in grid or ray count");
+  if (ref->rad_count <= 0 || ref->rad_begin != conv->rad_begin || ref->rad_count != conv->rad_count)
+    return fail(ANE_ERR_NOT_READY, "the contexts hold radiance of different layers");
+  if (ref->rad_bake || conv->rad_bake) return fail(ANE_ERR_UNSUPPORTED, "bake-mode blends are not checked");
+  if (!(rel_floor > 0.0)) return fail(ANE_ERR_INVALID_ARG, "rel_floor must be positive");
+  hipStream_t st = S(stream);
+  double td[64 * 3];
+  texel_dirs_f64(td);
+  double* d_td = nullptr;
+  double* d_out = nullptr;  // [0] max (bits), [1] sum of squares
+  ANE_TRY(dalloc(&d_td, 64 * 3, nullptr));
+  int rc = dalloc(&d_out, 2, nullptr);
+  if (rc) { dfree(d_td); return rc; }
+  const double zero[2] = {0.0, 0.0};
+  hipError_t e;
+  const int nprobes = ref->rad_count * ref->g.nx * ref->g.nz;
+  const double h = conv->cfg.hysteresis;
+  if (!(e = hipMemcpyAsync(d_td, td, sizeof(td), hipMemcpyHostToDevice, st)) &&
+      !(e = hipMemcpyAsync(d_out, zero, sizeof(zero), hipMemcpyHostToDevice, st))) {
+    launch_conv_err64(ref->g, ref->d_dirs, d_td, static_cast<const float*>(ref->d_rad), conv->d_atlas[conv->cur],
+                      h != 0.0 ? ref->d_atlas[ref->cur] : nullptr, h, rel_floor, nprobes,
+                      ref->rad_begin * ref->g.nx * ref->g.nz,
+                      reinterpret_cast<unsigned long long*>(d_out), d_out + 1, st);
+    double out[2] = {0.0, 0.0};
+    if (!(e = hipGetLastError()) && !(e = hipMemcpyAsync(out, d_out, sizeof(out), hipMemcpyDeviceToHost, st)) &&
+        !(e = hipStreamSynchronize(st))) {
+      *max_rel = out[0];  // the bits of a non-negative double
+      *rms_rel = std::sqrt(out[1] / (3.0 * 64.0 * nprobes));
+    }
+  }
+  dfree(d_td);
+  dfree(d_out);
+  if (e) return fail_hip(e, "conv_error_fp64", __FILE__, __LINE__);
   return ANE_OK;
 }
 
@@ -1516,7 +1611,7 @@ extern "C" int ane_ddgi_stats_get(const ane_ddgi_ctx* cc, ane_ddgi_stats* out)
   if (!cc || !out) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
   ane_ddgi_ctx* c = const_cast<ane_ddgi_ctx*>(cc);
   ane_ddgi_stats s = c->stats;
-  s.device_bytes = c->bytes + c->spill_words * 4;
+  s.device_bytes = ctx_device_bytes(c);
   for (int k = 0; k < kStages; ++k) {
     StageTimer& t = c->timers[k];
     ANE_TRY(t.harvest_all());

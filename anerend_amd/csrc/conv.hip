@@ -9,10 +9,12 @@
 // (M = 64 texels, K = N rays, columns = probes x 3 channels), followed by the
 // FP32 hysteresis epilogue (probe_conv.comp:63-89) written straight into the
 // atlas texels of each probe:
-//   k_conv_fp32  plain FP32 register-tiled kernel (the drop-in default)
-//   k_conv_mfma  BF16 (v_mfma_f32_16x16x32_bf16) or FP8-e4m3
-//                (v_mfma_f32_16x16x32_fp8_fp8) operands, FP32 accumulate;
-//                FP8 carries a per-texel weight scale and a per-probe radiance scale
+//   k_conv_mfma  FP32 (v_mfma_f32_16x16x4_f32, the plain drop-in path), BF16
+//                (v_mfma_f32_16x16x32_bf16) or FP8-e4m3 (v_mfma_f32_16x16x32_fp8_fp8)
+//                operands, FP32 accumulate; FP8 carries a per-texel weight scale
+//                and a per-probe radiance scale
+//   k_conv_fp32  FP32 register-tiled kernel for ray counts that are not a
+//                multiple of 32
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -116,105 +118,165 @@ __global__ __launch_bounds__(256) void k_conv_fp32(GridK g, const float* rad, co
 }
 
 // ---------------------------------------------------------------------------
-// MFMA: one wave = 64 texels (4 row tiles) x 16 probes x 3 channels.
-//   A (weights)   16 x 32 per tile, lane l: row l%16, k = 8*(l/16) .. +8
-//   B (radiance)  32 x 16 per tile, lane l: column (probe) l%16, same k range
-//   C             16 x 16, lane l: column l%16, rows 4*(l/16) + i, i = 0..3
-// The weights of a K chunk are staged in LDS once per block.
+// MFMA convolution (FP32, BF16, FP8 operands, FP32 accumulate).
+// Block = 16 probes (one MFMA column tile) x 64 texels x 3 channels, 4 waves.
+// The 4 waves split the rays: wave w takes the 32-ray steps s = w, w + 4, ...,
+// loading the weight (A) and radiance (B) fragments of up to CH steps at once
+// so that their latencies overlap, and accumulates 4 row tiles x 3 channels.
+// The partial sums meet in LDS; then all 256 threads run the hysteresis
+// epilogue with consecutive threads on consecutive texels of a tile row (each
+// probe's row is 128 contiguous bytes of the atlas).
+//   16x16x32 BF16 / FP8: lane l holds A[row l%16][k = 8(l/16) .. +7] and
+//     B[same k][col l%16]; C rows 4(l/16) + i, col l%16
+//   16x16x4 FP32: the same 8 k per lane, one per instruction: instruction e
+//     contracts k = 8q + e over the lane groups q = 0..3 (a permutation of the
+//     32 rays of the step, the same for A and B)
 // ---------------------------------------------------------------------------
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
-
-static constexpr int kWaves = 4;
-static constexpr int kKChunk = 256;  // rays of A staged per LDS fill
+typedef __attribute__((ext_vector_type(4))) int i32x4;
 
 template <int PREC>
-__global__ __launch_bounds__(64 * kWaves) void k_conv_mfma(GridK g, const void* rad,
-                                                          const float* rad_scale, const void* wq,
-                                                          const float* wscale, const float* wsum,
-                                                          float4* atlas, int nprobes,
-                                                          int probe_base, int32_t bake)
+struct ConvFrag;
+template <>
+struct ConvFrag<ANE_PRECISION_FP32> {
+  static constexpr int EB = 4, CH = 2;
+  float v[8];
+  __device__ __forceinline__ void load(const uint8_t* p)
+  {
+    const float4 x = *reinterpret_cast<const float4*>(p), y = *reinterpret_cast<const float4*>(p + 16);
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w; v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+  }
+  __device__ __forceinline__ void zero()
+  {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = 0.0f;
+  }
+  static __device__ __forceinline__ f32x4 mma(const ConvFrag& a, const ConvFrag& b, f32x4 c)
+  {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[e], b.v[e], c, 0, 0, 0);
+    return c;
+  }
+};
+template <>
+struct ConvFrag<ANE_PRECISION_BF16> {
+  static constexpr int EB = 2, CH = 4;
+  bf16x8 v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const bf16x8*>(p); }
+  __device__ __forceinline__ void zero() { v = bf16x8{}; }
+  static __device__ __forceinline__ f32x4 mma(const ConvFrag& a, const ConvFrag& b, f32x4 c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+  }
+};
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+template <>
+struct ConvFrag<ANE_PRECISION_FP16> {
+  static constexpr int EB = 2, CH = 4;
+  f16x8 v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const f16x8*>(p); }
+  __device__ __forceinline__ void zero() { v = f16x8{}; }
+  static __device__ __forceinline__ f32x4 mma(const ConvFrag& a, const ConvFrag& b, f32x4 c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a.v, b.v, c, 0, 0, 0);
+  }
+};
+template <>
+struct ConvFrag<ANE_PRECISION_FP8> {
+  static constexpr int EB = 1, CH = 4;
+  long v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const long*>(p); }
+  __device__ __forceinline__ void zero() { v = 0l; }
+  static __device__ __forceinline__ f32x4 mma(const ConvFrag& a, const ConvFrag& b, f32x4 c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a.v, b.v, c, 0, 0, 0);
+  }
+};
+
+static constexpr int kConvProbes = 16;
+static constexpr int kConvWaves = 4;
+static constexpr int kPartStride = 17;  // texel stride of the LDS partials (probe-minor, padded)
+
+template <int PREC>
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const void* rad,
+                                                              const float* rad_scale, const void* wq,
+                                                              const float* wscale, const float* wsum,
+                                                              float4* atlas, int nprobes,
+                                                              int probe_base, int32_t bake)
 {
-  constexpr int EB = (PREC == ANE_PRECISION_BF16) ? 2 : 1;  // bytes per element
-  __shared__ __attribute__((aligned(16))) uint8_t As[64 * kKChunk * EB];
-  const int n = g.rays;
+  using F = ConvFrag<PREC>;
+  constexpr int EB = F::EB, CH = F::CH;
+  __shared__ float part[kConvWaves][3][64 * kPartStride];
+  const int n = g.rays, S = n / 32;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  const int pw = (blockIdx.x * kWaves + wave) * 16;  // first probe of the wave
-  const int pcol = pw + col;
-  const bool pvalid = pcol < nprobes;
-  const uint8_t* radb = (const uint8_t*)rad;
-  const int64_t rowB = (int64_t)3 * n * EB;  // bytes per probe
+  const int p0 = blockIdx.x * kConvProbes;
+  const bool pvalid = p0 + col < nprobes;
+  const uint8_t* radp = (const uint8_t*)rad + (int64_t)(pvalid ? p0 + col : 0) * 3 * n * EB;
+  const uint8_t* wp = (const uint8_t*)wq;
   f32x4 acc[4][3];
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[rt][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-  for (int k0 = 0; k0 < n; k0 += kKChunk) {
-    const int kc = min(kKChunk, n - k0);
-    // stage A[64][kc] (row stride kKChunk) with 16-byte copies
-    const int vec_per_row = kc * EB / 16;
-    for (int e = tid; e < 64 * vec_per_row; e += 64 * kWaves) {
-      int t = e / vec_per_row, v = e % vec_per_row;
-      const uint4* src = reinterpret_cast<const uint4*>((const uint8_t*)wq + ((int64_t)t * n + k0) * EB) + v;
-      *reinterpret_cast<uint4*>(&As[(t * kKChunk) * EB + v * 16]) = *src;
-    }
-    __syncthreads();
-    for (int kk = 0; kk < kc; kk += 32) {
-      const int kl = kk + 8 * kq;  // this lane's k offset within the chunk
-      if (PREC == ANE_PRECISION_BF16) {
-        bf16x8 a[4], b[3];
+  for (int s0 = wave; s0 < S; s0 += kConvWaves * CH) {
+    F a[CH][4], b[CH][3];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-          a[rt] = *reinterpret_cast<const bf16x8*>(&As[((rt * 16 + col) * kKChunk + kl) * 2]);
+    for (int j = 0; j < CH; ++j) {
+      const int s = s0 + kConvWaves * j;
+      const int64_t k = 32 * s + 8 * kq;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          if (pvalid)
-            b[c] = *reinterpret_cast<const bf16x8*>(radb + pcol * rowB + ((int64_t)c * n + k0 + kl) * 2);
-          else
-            b[c] = bf16x8{};
-        }
+      for (int rt = 0; rt < 4; ++rt) {
+        if (s < S) a[j][rt].load(wp + ((int64_t)(rt * 16 + col) * n + k) * EB);
+        else a[j][rt].zero();
+      }
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            acc[rt][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], b[c], acc[rt][c], 0, 0, 0);
-      } else {
-        long a[4], b[3];
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-          a[rt] = *reinterpret_cast<const long*>(&As[(rt * 16 + col) * kKChunk + kl]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          b[c] = pvalid ? *reinterpret_cast<const long*>(radb + pcol * rowB + (int64_t)c * n + k0 + kl)
-                        : 0l;
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            acc[rt][c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[rt], b[c], acc[rt][c], 0, 0, 0);
+      for (int c = 0; c < 3; ++c) {
+        if (s < S && pvalid) b[j][c].load(radp + ((int64_t)c * n + k) * EB);
+        else b[j][c].zero();
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[rt][c] = F::mma(a[j][rt], b[j][c], acc[rt][c]);
   }
-  if (!pvalid) return;
-  const int q = probe_base + pcol;
-  const float h = g.hysteresis;
-  const float ps = (PREC == ANE_PRECISION_FP8) ? rad_scale[pcol] : 1.0f;
+  // partial sums: part[wave][c][t * stride + probe]
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int t = rt * 16 + 4 * kq + i;
-      // per-texel weight scale (FP8 dequantization) x weight-sum correction
-      const float s = (PREC == ANE_PRECISION_FP8) ? wscale[t] * ps : wscale[64 + t];
-      const int64_t ti = texel_index(g, q, t);
-      const float4 old = atlas[ti];
-      atlas[ti] = blend_texel(old, acc[rt][0][i] * s, acc[rt][1][i] * s, acc[rt][2][i] * s, wsum[t],
-                              h, bake != 0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) part[wave][c][t * kPartStride + col] = acc[rt][c][i];
     }
+  __syncthreads();
+  const float h = g.hysteresis;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int item = tid + 256 * j;  // (ty, probe, tx), tx fastest
+    const int tx = item & 7, p = (item >> 3) & 15, ty = item >> 7;
+    if (p0 + p >= nprobes) continue;
+    const int t = ty * 8 + tx;
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float v = 0.0f;
+#pragma unroll
+      for (int w = 0; w < kConvWaves; ++w) v += part[w][c][t * kPartStride + p];
+      r[c] = v;
+    }
+    // per-texel weight scale (FP8 dequantization) or weight-sum correction (BF16)
+    const float sc = (PREC == ANE_PRECISION_FP8) ? wscale[t] * rad_scale[p0 + p]
+                     : (PREC == ANE_PRECISION_BF16 || PREC == ANE_PRECISION_FP16) ? wscale[64 + t] : 1.0f;
+    const int64_t ti = texel_index(g, probe_base + p0 + p, t);
+    const float4 old = atlas[ti];
+    atlas[ti] = blend_texel(old, r[0] * sc, r[1] * sc, r[2] * sc, wsum[t], h, bake != 0);
+  }
 }
 
 void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,
@@ -225,21 +287,30 @@ void launch_convolve(const GridK& g, const FrameK& f, int precision, const void*
   (void)f;
   if (num_probes <= 0) return;
   float4* a4 = reinterpret_cast<float4*>(atlas);
+  const unsigned blocks = (unsigned)((num_probes + kConvProbes - 1) / kConvProbes);
   if (precision == ANE_PRECISION_FP32) {
-    const unsigned blocks = (unsigned)((num_probes + kPB - 1) / kPB);
-    hipLaunchKernelGGL(k_conv_fp32, dim3(blocks), dim3(256), 0, s, g, (const float*)rad, w32, wsum,
-                       a4, num_probes, probe_base, (int32_t)bake);
-    return;
-  }
-  const unsigned blocks = (unsigned)((num_probes + 16 * kWaves - 1) / (16 * kWaves));
-  if (precision == ANE_PRECISION_BF16)
-    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_BF16>, dim3(blocks), dim3(64 * kWaves), 0, s, g,
+    if (g.rays % 32 != 0) {  // the register-tiled kernel takes any ray count
+      const unsigned b32 = (unsigned)((num_probes + kPB - 1) / kPB);
+      hipLaunchKernelGGL(k_conv_fp32, dim3(b32), dim3(256), 0, s, g, (const float*)rad, w32, wsum,
+                         a4, num_probes, probe_base, (int32_t)bake);
+      return;
+    }
+    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_FP32>, dim3(blocks), dim3(64 * kConvWaves), 0, s, g, rad,
+                       rad_scale, (const void*)w32, wfp8_scale, wsum, a4, num_probes, probe_base,
+                       (int32_t)bake);
+  } else if (precision == ANE_PRECISION_FP16) {
+    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_FP16>, dim3(blocks), dim3(64 * kConvWaves), 0, s, g,
                        rad, rad_scale, (const void*)wbf16, wfp8_scale, wsum, a4,
                        num_probes, probe_base, (int32_t)bake);
-  else
-    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_FP8>, dim3(blocks), dim3(64 * kWaves), 0, s, g,
+  } else if (precision == ANE_PRECISION_BF16) {
+    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_BF16>, dim3(blocks), dim3(64 * kConvWaves), 0, s, g,
+                       rad, rad_scale, (const void*)wbf16, wfp8_scale, wsum, a4,
+                       num_probes, probe_base, (int32_t)bake);
+  } else {
+    hipLaunchKernelGGL(k_conv_mfma<ANE_PRECISION_FP8>, dim3(blocks), dim3(64 * kConvWaves), 0, s, g,
                        rad, rad_scale, (const void*)wfp8, wfp8_scale, wsum, a4, num_probes,
                        probe_base, (int32_t)bake);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -251,9 +322,12 @@ __global__ __launch_bounds__(256) void k_quantize(int prec, const float* src, vo
   __shared__ float red[256];
   const int p = blockIdx.x;
   const float* s = src + (int64_t)p * 3 * n;
-  if (prec == ANE_PRECISION_BF16) {
+  if (prec == ANE_PRECISION_BF16 || prec == ANE_PRECISION_FP16) {
     uint16_t* d = (uint16_t*)dst + (int64_t)p * 3 * n;
-    for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_bf16(s[i]);
+    if (prec == ANE_PRECISION_BF16)
+      for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_bf16(s[i]);
+    else
+      for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_f16_sat(s[i]);
     return;
   }
   float mx = 0.0f;
@@ -278,6 +352,9 @@ __global__ __launch_bounds__(256) void k_dequantize(int prec, const void* src, c
   if (prec == ANE_PRECISION_BF16) {
     const uint16_t* s = (const uint16_t*)src + (int64_t)p * 3 * n;
     for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = bf16_to_f32(s[i]);
+  } else if (prec == ANE_PRECISION_FP16) {
+    const uint16_t* s = (const uint16_t*)src + (int64_t)p * 3 * n;
+    for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f16_to_f32(s[i]);
   } else {
     const uint8_t* s = (const uint8_t*)src + (int64_t)p * 3 * n;
     const float sc = scale[p];
@@ -297,6 +374,85 @@ void launch_dequantize(int precision, const void* src, const float* scale, float
 {
   if (num_probes <= 0) return;
   hipLaunchKernelGGL(k_dequantize, dim3(num_probes), dim3(256), 0, s, precision, src, scale, dst, n);
+}
+
+// ---------------------------------------------------------------------------
+// Validation: the irradiance of a convolution against a float64 evaluation of
+// the reference's sum (probe_conv.comp:49-61) over FP32 radiance, reduced on
+// the GPU.  One block per probe: the probe's radiance and the ray directions
+// are staged in LDS; thread t < 64 evaluates texel t in float64:
+//   W = max(0, d_r . o_t) (kept if > 1e-6), E64 = sum W L / sum W,
+// recovers the convolution's E from the blended atlas, E = (new - h old)/(1 - h),
+// and folds |E - E64| / max(|E64|, floor) over its 3 channels into the block's
+// max and sum of squares, then one atomic each (max as the bit pattern of a
+// non-negative double).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_conv_err64(GridK g, const float* dirs, const double* texel_dirs,
+                                                   const float* rad, const float4* atlas_new,
+                                                   const float4* atlas_old, double h, double floor_,
+                                                   int probe_base, unsigned long long* out_max,
+                                                   double* out_sumsq)
+{
+  extern __shared__ float sm[];  // [3][N] radiance, then [N][3] directions
+  const int n = g.rays;
+  const int ql = blockIdx.x;
+  const float* L = rad + (int64_t)ql * 3 * n;
+  for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) {
+    sm[i] = L[i];
+    sm[3 * n + i] = dirs[i];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const double ox = texel_dirs[3 * t], oy = texel_dirs[3 * t + 1], oz = texel_dirs[3 * t + 2];
+  double acc[3] = {0.0, 0.0, 0.0}, ws = 0.0;
+  for (int r = 0; r < n; ++r) {
+    const float* d = sm + 3 * n + 3 * r;
+    double w = (double)d[0] * ox + (double)d[1] * oy + (double)d[2] * oz;
+    if (w > 1e-6) {
+      acc[0] += w * (double)sm[r];
+      acc[1] += w * (double)sm[n + r];
+      acc[2] += w * (double)sm[2 * n + r];
+      ws += w;
+    }
+  }
+  const int64_t ti = texel_index(g, probe_base + ql, t);
+  const float4 a = atlas_new[ti];
+  const float4 o = atlas_old ? atlas_old[ti] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const double e[3] = {((double)a.x - h * (double)o.x) / (1.0 - h), ((double)a.y - h * (double)o.y) / (1.0 - h),
+                       ((double)a.z - h * (double)o.z) / (1.0 - h)};
+  double mx = 0.0, sq = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double ref = ws > 1e-6 ? acc[c] / ws : acc[c];
+    const double den = fabs(ref) > floor_ ? fabs(ref) : floor_;
+    const double rel = fabs(e[c] - ref) / den;
+    mx = rel > mx ? rel : mx;
+    sq += rel * rel;
+  }
+  // wave reduction (the block is one wave of 64)
+  for (int off = 32; off > 0; off >>= 1) {
+    const double m2 = __shfl_xor(mx, off, 64);
+    const double s2 = __shfl_xor(sq, off, 64);
+    mx = m2 > mx ? m2 : mx;
+    sq += s2;
+  }
+  if (t == 0) {
+    atomicMax(out_max, (unsigned long long)__double_as_longlong(mx));
+    atomicAdd(out_sumsq, sq);
+  }
+}
+
+void launch_conv_err64(const GridK& g, const float* dirs, const double* texel_dirs, const float* rad,
+                       const float* atlas_new, const float* atlas_old, double h, double floor_,
+                       int num_probes, int probe_base, unsigned long long* out_max, double* out_sumsq,
+                       hipStream_t s)
+{
+  if (num_probes <= 0) return;
+  const size_t lds = sizeof(float) * 6 * (size_t)g.rays;  // up to 96 KB at 4096 rays
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_conv_err64,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_conv_err64, dim3(num_probes), dim3(64), lds, s, g, dirs, texel_dirs, rad,
+                     reinterpret_cast<const float4*>(atlas_new), reinterpret_cast<const float4*>(atlas_old),
+                     h, floor_, probe_base, out_max, out_sumsq);
 }
 
 }  // namespace ane

@@ -138,7 +138,7 @@ int trace_stack_lds();
 void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipStream_t s);
 void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
                         const uint32_t* order, float* dirs, float4* dirpre, float4* probe_org, float* w32, float* wsum, uint16_t* wbf16,
-                        uint8_t* wfp8, float* wfp8_scale, bool bake, hipStream_t s);
+                        uint8_t* wfp8, float* wfp8_scale, bool bake, bool f16, hipStream_t s);
 struct RayDiv {
   uint64_t magic;
   int32_t shift;
@@ -151,6 +151,12 @@ void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float
                   float* rad_scale, int num_probes, int probe_base, hipStream_t s);
 
 // conv.hip
+// FP64 validation of a convolution (see conv.hip): out_max (the bits of a
+// non-negative double) and out_sumsq accumulate over the probes
+void launch_conv_err64(const GridK& g, const float* dirs, const double* texel_dirs, const float* rad,
+                       const float* atlas_new, const float* atlas_old, double h, double floor_,
+                       int num_probes, int probe_base, unsigned long long* out_max, double* out_sumsq,
+                       hipStream_t s);
 void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,
                      const float* rad_scale, const float* w32, const float* wsum,
                      const uint16_t* wbf16, const uint8_t* wfp8, const float* wfp8_scale,
@@ -212,6 +218,15 @@ __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
   __builtin_memcpy(&x, &b, 4);
   return x;
 }
+
+// IEEE binary16, round to nearest even (the hardware conversion), saturating at 65504
+__device__ __forceinline__ uint16_t f32_to_f16_sat(float x)
+{
+  x = fminf(fmaxf(x, -65504.0f), 65504.0f);
+  return __builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+__device__ __forceinline__ float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
 // OCP FP8 E4M3 (e4m3fn: bias 7, max 448, no infinities), round to nearest even,
 // saturating at 448.

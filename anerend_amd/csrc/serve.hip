@@ -30,6 +30,8 @@
 namespace ane {
 int fail(int status, const char* fmt, ...);
 int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st);
+int ddgi_prepare(ane_ddgi_ctx* c, const ane_scene* s, int64_t* bytes);
+int64_t ddgi_device_bytes(const ane_ddgi_ctx* c);
 }
 using ane::fail;
 
@@ -269,11 +271,17 @@ extern "C" int ane_serve_add_session(ane_serve* s, const ane_scene* scene,
   }
   st = ane_ddgi_create(config, &ctx);
   if (st != ANE_OK) return st;
-  ane_ddgi_stats ds{};
-  ane_ddgi_stats_get(ctx, &ds);
+  // the spill and sun-grid buffers the first update would allocate are allocated
+  // now and charged with the rest (the budget covers every byte the session holds)
+  int64_t ctx_bytes = 0;
+  st = ane::ddgi_prepare(ctx, scene, &ctx_bytes);
+  if (st != ANE_OK) {
+    ane_ddgi_destroy(ctx);
+    return st;
+  }
   std::lock_guard<std::mutex> g(s->mu);
   const int64_t scene_bytes = s->scenes.count(scene) ? 0 : si.device_bytes;
-  const int64_t need = ds.device_bytes + scene_bytes;
+  const int64_t need = ctx_bytes + scene_bytes;
   if (s->used_bytes + need > s->cfg.hbm_budget_bytes) {
     ane_ddgi_destroy(ctx);
     return fail(ANE_ERR_ADMISSION_REFUSED, "HBM budget: %lld of %lld bytes in use, session needs %lld",
@@ -282,7 +290,7 @@ extern "C" int ane_serve_add_session(ane_serve* s, const ane_scene* scene,
   Session ses;
   ses.scene = scene;
   ses.ctx = ctx;
-  ses.bytes = ds.device_bytes;
+  ses.bytes = ctx_bytes;
   ses.rays = (int64_t)config->probes_x * config->probes_y * config->probes_z * config->rays_per_probe;
   ses.probes_y = config->probes_y;
   const int32_t id = s->next_session++;
@@ -399,6 +407,15 @@ extern "C" int ane_serve_stats_get(ane_serve* s, ane_serve_stats* out)
   out->rays_done = s->rays_done;
   out->used_bytes = s->used_bytes;
   out->budget_bytes = s->cfg.hbm_budget_bytes;
+  // what the sessions hold now: equal to used_bytes unless an update allocated
+  // beyond what admission charged
+  int64_t held = 0;
+  for (const auto& kv : s->sessions) held += ane::ddgi_device_bytes(kv.second.ctx);
+  for (const auto& kv : s->scenes) {
+    ane_scene_info si{};
+    if (ane_scene_info_get(kv.first, &si) == ANE_OK) held += si.device_bytes;
+  }
+  out->held_bytes = held;
   return ANE_OK;
 }
 

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot
                                                      float* dirs, float4* dirpre,
                                                      float4* probe_org, float* w32, float* wsum,
                                                      uint16_t* wbf16, uint8_t* wfp8,
-                                                     float* wfp8_scale, int32_t bake)
+                                                     float* wfp8_scale, int32_t bake, int32_t f16)
 {
   const int t = blockIdx.x;  // texel 0..63
   // probe origins of the whole grid (rgen:60-84), read by ray generation and shading
@@ -179,7 +179,8 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot
     float w = w32[(int64_t)t * n + r];
     if (norm) w = w / sum;
     w32[(int64_t)t * n + r] = w;
-    wbf16[(int64_t)t * n + r] = f32_to_bf16(w);
+    // the 16-bit weights: BF16, or FP16 for the FP16 precision
+    wbf16[(int64_t)t * n + r] = f16 ? f32_to_f16_sat(w) : f32_to_bf16(w);
     mx = fmaxf(mx, w);
   }
   red[threadIdx.x] = mx;
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot
     const uint8_t q = f32_to_e4m3(w / sc);
     wfp8[(int64_t)t * n + r] = q;
     s32 += w;
-    sb += bf16_to_f32(wbf16[(int64_t)t * n + r]);
+    sb += f16 ? f16_to_f32(wbf16[(int64_t)t * n + r]) : bf16_to_f32(wbf16[(int64_t)t * n + r]);
     s8 += e4m3_to_f32(q) * sc;
   }
   __shared__ float red3[3][256];
@@ -216,19 +217,19 @@ __global__ __launch_bounds__(256) void k_frame_setup(GridK g, FrameK f, Mat3 rot
     wsum[t] = sum;
     const float S = red3[0][0], SB = red3[1][0], S8 = red3[2][0];
     wfp8_scale[t] = S8 > 0.0f ? sc * (S / S8) : sc;  // FP8: dequantization x correction
-    wfp8_scale[64 + t] = SB > 0.0f ? S / SB : 1.0f;  // BF16: correction
+    wfp8_scale[64 + t] = SB > 0.0f ? S / SB : 1.0f;  // BF16 / FP16: correction
   }
 }
 
 void launch_frame_setup(const GridK& g, const FrameK& f, const float* base_dirs,
                         const uint32_t* order, float* dirs, float4* dirpre, float4* probe_org, float* w32, float* wsum, uint16_t* wbf16,
-                        uint8_t* wfp8, float* wfp8_scale, bool bake, hipStream_t s)
+                        uint8_t* wfp8, float* wfp8_scale, bool bake, bool f16, hipStream_t s)
 {
   Mat3 rot;
   for (int i = 0; i < 16; ++i) rot.m[i] = f.rot[i];
   const f3 sun = normalize(mk(-f.light_dir[0], -f.light_dir[1], -f.light_dir[2]));
   hipLaunchKernelGGL(k_frame_setup, dim3(64), dim3(256), 0, s, g, f, rot, sun, base_dirs, order,
-                     dirs, dirpre, probe_org, w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake);
+                     dirs, dirpre, probe_org, w32, wsum, wbf16, wfp8, wfp8_scale, (int32_t)bake, (int32_t)f16);
 }
 
 // Division of a ray index x < nrays by n as (x * m) >> k, m = ceil(2^k / n).
@@ -973,6 +974,11 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WA
       o[i] = f32_to_bf16(L.x);
       o[n + i] = f32_to_bf16(L.y);
       o[2 * n + i] = f32_to_bf16(L.z);
+    } else if (PREC == ANE_PRECISION_FP16) {
+      uint16_t* o = (uint16_t*)rad + (int64_t)ql * 3 * n;
+      o[i] = f32_to_f16_sat(L.x);
+      o[n + i] = f32_to_f16_sat(L.y);
+      o[2 * n + i] = f32_to_f16_sat(L.z);
     } else {
       sL[i] = L.x;
       sL[n + i] = L.y;
@@ -1003,6 +1009,9 @@ static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, cons
 {
   if (precision == ANE_PRECISION_FP32)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP32, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
+                       probe_org, hits, shadow, a4, rad, rad_scale, probe_base);
+  else if (precision == ANE_PRECISION_FP16)
+    hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP16, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,
                        probe_org, hits, shadow, a4, rad, rad_scale, probe_base);
   else if (precision == ANE_PRECISION_BF16)
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_BF16, TB>), dim3(num_probes), dim3(TB), 0, s, sc, g, f,

@@ -30,7 +30,7 @@ class SceneInfo(ctypes.Structure):
                 ("build_ms", ctypes.c_float), ("node_format", ctypes.c_int32)]
 
 
-STAGES = ("setup", "trace", "shadow", "shade", "convolve", "translate")
+STAGES = ("setup", "trace", "sungrid", "shade", "convolve", "translate")
 
 
 class Stats(ctypes.Structure):
@@ -88,6 +88,7 @@ _SIGS = [
     ("ane_ddgi_stats_reset", ctypes.c_int, [_vp]),
     ("ane_ddgi_check", ctypes.c_int, [_vp, _vp]),
     ("ane_ddgi_tiles_set", ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    ("ane_ddgi_conv_error_fp64", ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
     ("ane_ddgi_set_traversal_limit", ctypes.c_int, [_vp, ctypes.c_uint32]),
     ("ane_scene_debug_bvh", ctypes.c_int, [_vp] * 7),
     ("ane_scene_debug_world_tris", ctypes.c_int, [_vp, _vp]),
@@ -446,6 +447,14 @@ class ProbeGrid:
             assert a.shape == (len(xz),) + tuple(self.config.atlas_shape)
         check(load().ane_ddgi_tiles_set(self._h, ptr(xz), ptr(a) if len(xz) else None, len(xz),
                                         int(tile_size), None))
+
+    def conv_error_fp64(self, ref, rel_floor=1e-3):
+        """(max, rms) relative error of this grid's last convolution against float64 over
+        the FP32 radiance ``ref`` holds, reduced on the GPU (see ane_ddgi_conv_error_fp64)."""
+        mx, rms = ctypes.c_double(), ctypes.c_double()
+        check(load().ane_ddgi_conv_error_fp64(self._h, ref.handle, float(rel_floor), ctypes.byref(mx),
+                                              ctypes.byref(rms), None))
+        return mx.value, rms.value
 
     def set_traversal_limit(self, max_node_visits):
         check(load().ane_ddgi_set_traversal_limit(self._h, int(max_node_visits)))

@@ -31,7 +31,8 @@ class ServeStats(ctypes.Structure):
                 ("inflight", ctypes.c_int32), ("peak_inflight", ctypes.c_int32),
                 ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
                 ("rays_done", ctypes.c_int64),
-                ("used_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64)]
+                ("used_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64),
+                ("held_bytes", ctypes.c_int64)]
 
 
 assert ctypes.sizeof(ServeConfig) == 24

@@ -69,6 +69,7 @@ FLAG_BAKE = 1 << 11
 PRECISION_FP32 = 0
 PRECISION_BF16 = 1
 PRECISION_FP8 = 2
+PRECISION_FP16 = 3
 
 
 class Config(ctypes.Structure):

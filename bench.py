@@ -35,7 +35,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp8"])
+    p.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "fp8"],
+                   help="convolution precision of the headline update (fp32: the plain path)")
     p.add_argument("--workload", default="terrain1m",
                    choices=["terrain1m", "room8", "dynamic4m", "serve64"])
     p.add_argument("--requests", type=int, default=2000, help="serve64: open-loop requests")
@@ -48,8 +49,13 @@ def parse():
                    help="dynamic4m: full LBVH rebuild every K frames, refit in between (0 = refit only)")
     p.add_argument("--node-format", default="auto", choices=["auto", "full", "quantized"],
                    help="traversal node layout (auto: quantized from 2M triangles up)")
-    p.add_argument("--cpu-seconds", type=float, default=10.0,
-                   help="CPU oracle baseline: repeat whole updates for at least this long")
+    p.add_argument("--cpu-runs", type=int, default=5,
+                   help="CPU oracle baseline: timed whole updates (after one warm-up); the median is reported")
+    p.add_argument("--legs", default="dynamic,serve",
+                   help="extra legs of the default line: dynamic (compact 4M-triangle run), serve "
+                        "(64-scene request loop); empty for none")
+    p.add_argument("--leg-frames", type=int, default=60, help="frames of the compact dynamic leg")
+    p.add_argument("--leg-requests", type=int, default=600, help="open-loop requests of the serving leg")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-conv-study", action="store_true",
                    help="skip the FP32/BF16/FP8 convolution timing and FP64 error study")
@@ -78,27 +84,41 @@ def dynamic_cpu_baseline(scene, vel, cfg, frames=1):
                       f"{dt:.2f} s, extrapolated per frame; CPU LBVH build {build_s:.2f} s not counted"}
 
 
-def cpu_baseline(scene, cfg, frame_at, min_seconds):
-    """The oracle (C, OpenMP over rays and probes) on a bounded sample of the same
-    workload: whole probe updates, repeated until ``min_seconds`` have passed."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, cfg, frame_at, runs):
+    """The oracle (C, OpenMP over rays and probes) on the same workload: one warm-up
+    update, then ``runs`` whole probe updates, each timed per stage (trace + shadow
+    + shade, convolve + blend); the medians are reported."""
     from oracle import oracle as O
     t0 = time.time()
     osc = O.OracleScene(scene)
     build_s = time.time() - t0
     atlas = O.new_atlas(cfg)
     per = cfg.probes_x * cfg.probes_y * cfg.probes_z * cfg.rays_per_probe
-    n = 0
-    t0 = time.time()
-    while True:
-        osc.update(cfg, frame_at(n), atlas, want_rays=False)
-        n += 1
-        dt = time.time() - t0
-        if dt >= min_seconds or n >= 100:
-            break
-    return {"value": n * per / dt, "unit": "probe rays/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{n} full probe updates of the same workload ({n * per} rays: trace, shadow, "
-                      f"shade, convolve, blend) in {dt:.2f} s; CPU LBVH build {build_s:.2f} s not "
-                      f"counted"}
+    osc.update(cfg, frame_at(0), atlas, want_rays=False)
+    wall, stages = [], []
+    for i in range(runs):
+        t0 = time.time()
+        osc.update(cfg, frame_at(1 + i), atlas, want_rays=False)
+        wall.append(time.time() - t0)
+        stages.append(O.last_update_stage_ms())
+    med = float(np.median(wall))
+    st = np.median(np.array(stages), axis=0)
+    return {"value": per / med, "unit": "probe rays/s", "cores": O.num_threads(), "kind": "port",
+            "cpu_model": cpu_model(), "ms_per_update": round(med * 1000.0, 1),
+            "stages_ms": {"trace_shadow_shade": round(float(st[0]), 1), "convolve_blend": round(float(st[1]), 1)},
+            "sample": f"median of {runs} full probe updates of the same workload ({per} rays each: trace, "
+                      f"shadow, shade, convolve, blend) after one warm-up; CPU LBVH build {build_s:.2f} s "
+                      f"(single thread) not counted"}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "trace_terrain1m_traffic.md")
@@ -136,50 +156,36 @@ def copy_calibration(nbytes):
         hip.hipFree(a)
 
 
-def texel_dirs64():
-    """Octahedral directions of the 8x8 tile texel centres in float64, t = ty * 8 + tx
-    (probe_conv.comp:37-46, octahedron_helpers.glsl:25-34: y and z swapped)."""
-    u = (2.0 * np.arange(8) + 1.0) / 16.0 * 2.0 - 1.0
-    ox, oy = (a.ravel() for a in np.meshgrid(u, u))
-    z = 1.0 - np.abs(ox) - np.abs(oy)
-    nx = np.where(z < 0, (1.0 - np.abs(oy)) * np.where(ox >= 0, 1.0, -1.0), ox)
-    ny = np.where(z < 0, (1.0 - np.abs(ox)) * np.where(oy >= 0, 1.0, -1.0), oy)
-    v = np.stack([nx, z, ny], 1)
-    return v / np.linalg.norm(v, axis=1, keepdims=True)
-
-
 def conv_study(cfg0, frame, reps=20):
     """Convolution alone at the production shape (16,384 probes x 512 rays x RGB) on
-    seeded radiance, log-uniform on [1e-3, 50] (seed 3): FP32 / BF16 (MFMA) / FP8
+    seeded radiance, log-uniform on [1e-3, 50] (seed 3): FP32 / BF16 / FP16 / FP8
     (MFMA) kernel times and their error against a float64 GEMM of the same
-    weights.  Hysteresis 0, so the atlas interior is the convolution itself."""
+    weights, evaluated and reduced in float64 on the GPU.  Hysteresis 0, so the
+    atlas interior is the convolution itself."""
     from anerend_amd import native
-    from anerend_amd.types import Config, PRECISION_BF16, PRECISION_FP8, PRECISION_FP32
+    from anerend_amd.types import Config, PRECISION_BF16, PRECISION_FP8, PRECISION_FP16, PRECISION_FP32
     P = cfg0.probes_x * cfg0.probes_y * cfg0.probes_z
     N = cfg0.rays_per_probe
     rng = np.random.default_rng(3)
     rad = np.exp(rng.uniform(np.log(1e-3), np.log(50.0), (P, 3, N))).astype(np.float32)
     fr = frame.copy()
     fr.layer_begin, fr.layer_count = 0, cfg0.probes_y
-    d = native.ray_directions(N, list(fr.rotation)).astype(np.float64)
-    W = np.maximum(0.0, texel_dirs64() @ d.T)
-    W[W <= 1e-6] = 0.0
-    Wn = W / W.sum(1, keepdims=True)
-    ref = (rad.reshape(P * 3, N).astype(np.float64) @ Wn.T).reshape(P, 3, 64)
     nx, ny, nz = cfg0.probes_x, cfg0.probes_y, cfg0.probes_z
+    # the FP32 radiance the errors are measured against (float64 on the GPU)
+    ref = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg0.probe_step), rays=N, hysteresis=0.0))
+    ref.set_radiance(fr, rad)
     flops = 2.0 * 64 * N * 3 * P
-    peaks = {"fp32": 157.3e12, "bf16": 2.5e15, "fp8": 5.0e15}
-    rad_bytes = {"fp32": 4, "bf16": 2, "fp8": 1}
+    peaks = {"fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15, "fp8": 5.0e15}
+    rad_bytes = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}
     out = {}
-    for name, prec in (("fp32", PRECISION_FP32), ("bf16", PRECISION_BF16), ("fp8", PRECISION_FP8)):
+    for name, prec in (("fp32", PRECISION_FP32), ("bf16", PRECISION_BF16), ("fp16", PRECISION_FP16),
+                       ("fp8", PRECISION_FP8)):
         cfg = Config.make((nx, ny, nz), step=tuple(cfg0.probe_step), rays=N, hysteresis=0.0,
                           precision=prec)
         grid = native.ProbeGrid(cfg)
         grid.set_radiance(fr, rad)
         grid.convolve(fr)
-        a = grid.atlas.reshape(ny, nz, 10, nx, 10, 4)[:, :, 1:9, :, 1:9, :3]
-        got = a.transpose(0, 1, 3, 2, 4, 5).reshape(P, 64, 3).transpose(0, 2, 1)
-        rel = np.abs(got - ref) / ref
+        mx, rms = grid.conv_error_fp64(ref, rel_floor=1e-3)  # radiance >= 1e-3: the floor is inert
         native.synchronize()
         grid.reset_stats()
         for _ in range(reps):
@@ -187,8 +193,8 @@ def conv_study(cfg0, frame, reps=20):
         native.synchronize()
         ms = grid.stats()["mean_ms"]["convolve"]
         hbm = P * 3 * N * rad_bytes[name] + 2 * P * 64 * 16
-        out[name] = {"ms": round(ms, 5), "max_rel_err_vs_fp64": float(f"{rel.max():.3e}"),
-                     "rms_rel_err_vs_fp64": float(f"{np.sqrt((rel ** 2).mean()):.3e}"),
+        out[name] = {"ms": round(ms, 5), "max_rel_err_vs_fp64": float(f"{mx:.3e}"),
+                     "rms_rel_err_vs_fp64": float(f"{rms:.3e}"),
                      "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
                      "compute_frac": round(flops / (ms * 1e-3) / peaks[name], 4),
                      "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 1),
@@ -196,6 +202,7 @@ def conv_study(cfg0, frame, reps=20):
         grid.close()
         log(f"[bench] conv {name}: {json.dumps(out[name])}")
     out["bf16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["bf16"]["ms"], 3)
+    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
     out["fp8_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp8"]["ms"], 3)
     out["shape"] = f"{P} probes x {N} rays x RGB, GEMM 64 x {N} x {3 * P}"
     return out
@@ -432,65 +439,44 @@ def light_shadow_study(ds, num_lights=1024, reps=10, seed=21):
             "node_visits": int(ctr[0]), "triangle_tests": int(ctr[1])}
 
 
-def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr, cfg):
-    """Error of one convolution of ``grid_conv`` (its precision) against a float64
-    GEMM of the FP32 radiance of the same frame traced by ``grid_ref`` (FP32) from
-    the same atlas.  E is recovered from the blend: E = (new - h old) / (1 - h)."""
-    from anerend_amd import native
-    P = cfg.probes_x * cfg.probes_y * cfg.probes_z
-    N = cfg.rays_per_probe
-    old = grid_conv.atlas
-    grid_ref.atlas = old
+def fp64_conv_error(grid_conv, grid_ref, scene_dev, fr):
+    """Error of one convolution of ``grid_conv`` (its precision) against float64
+    over the FP32 radiance of the same frame traced by ``grid_ref`` from the same
+    atlas, evaluated and reduced in float64 on the GPU (ane_ddgi_conv_error_fp64)."""
+    grid_ref.atlas = grid_conv.atlas
     grid_ref.trace(scene_dev, fr)
-    rad = grid_ref.radiance(fr).astype(np.float64)
     grid_conv.update(scene_dev, fr)
-    new = grid_conv.atlas
-    h = float(np.float32(cfg.hysteresis))
-    nx, ny, nz = cfg.probes_x, cfg.probes_y, cfg.probes_z
-
-    def interior(a):
-        a = a.astype(np.float64).reshape(ny, nz, 10, nx, 10, 4)[:, :, 1:9, :, 1:9, :3]
-        return a.transpose(0, 1, 3, 2, 4, 5).reshape(P, 64, 3).transpose(0, 2, 1)
-
-    E = (interior(new) - h * interior(old)) / (1.0 - h)
-    d = native.ray_directions(N, list(fr.rotation)).astype(np.float64)
-    W = np.maximum(0.0, texel_dirs64() @ d.T)
-    W[W <= 1e-6] = 0.0
-    Wn = W / W.sum(1, keepdims=True)
-    ref = (rad.reshape(P * 3, N) @ Wn.T).reshape(P, 3, 64)
-    rel = np.abs(E - ref) / np.maximum(np.abs(ref), 1e-3)
-    return {"max_rel": float(f"{rel.max():.3e}"), "rms_rel": float(f"{np.sqrt((rel ** 2).mean()):.3e}"),
-            "floor": 1e-3}
+    mx, rms = grid_conv.conv_error_fp64(grid_ref, rel_floor=1e-3)
+    return {"max_rel": float(f"{mx:.3e}"), "rms_rel": float(f"{rms:.3e}"), "floor": 1e-3,
+            "reduction": "float64 on the GPU"}
 
 
-def bench_dynamic(args):
+def run_dynamic(frames, warmup, rebuild_every, node_format, with_cpu):
     """Dynamic large scene: 64 x 32 x 64 probes x 256 rays over ~4M triangles whose
-    20,000 instances move every frame (refit), a camera crossing an integer cell
-    every 3 frames (translation), one full probe update per frame with FP8
-    convolution.  Prints one JSON line (ms per frame)."""
+    20,000 instances move every frame (refit, a rebuild every ``rebuild_every``),
+    a camera crossing an integer cell every 3 frames (translation), one full probe
+    update per frame with FP8 convolution.  Returns its figures as a dict."""
     from anerend_amd import native, scenegen
     from anerend_amd.types import PRECISION_FP32
-    frames = args.frames
     cfg = scenegen.dynamic_config()
     log("[bench] building the dynamic scene")
     scene, vel = scenegen.dynamic_scene()
     insts = [scenegen.dynamic_instances(scene, vel, f) for f in range(frames + 1)]
     frs = [scenegen.dynamic_frame(5, f, cfg) for f in range(frames + 1)]
     ds = native.DeviceScene(scene)
-    ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[args.node_format])
+    ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[node_format])
     info = ds.info()
     log(f"[bench] {scene.num_tris} triangles, LBVH {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
     grid = native.ProbeGrid(cfg)
-    for f in range(args.warmup):  # warm-up frames outside the timed run, then restart the path
+    for f in range(warmup):  # warm-up frames outside the timed run, then restart the path
         grid.update(ds, frs[f])
     native.synchronize()
     grid.translation_state((0, 0))
     grid.reset_stats()
     moves = 0
-    refit_ms = 0.0
     log(f"[bench] timing {frames} frames")
     t0 = time.perf_counter()
-    K = args.rebuild_every
+    K = rebuild_every
     for f in range(frames):
         if K > 0 and f % K == 0:
             ds.rebuild(insts[f])
@@ -506,46 +492,64 @@ def bench_dynamic(args):
     copy_calibration(1 << 30)   # scale for the profiler's byte counters, outside the timed run
     log(f"[bench] {ms:.3f} ms/frame, {moves} translations, stages "
         f"{json.dumps({k: round(v, 4) for k, v in st['mean_ms'].items()})}")
-    # FP8 convolution error against FP64 on the frame after the run
+    # FP8 convolution error against float64 on the frame after the run (GPU reduction)
     ds.refit(insts[frames])
     ref_grid = native.ProbeGrid(scenegen.dynamic_config(PRECISION_FP32))
-    err = fp64_conv_error(grid, ref_grid, ds, frs[frames], cfg)
+    err = fp64_conv_error(grid, ref_grid, ds, frs[frames])
+    ref_grid.close()
     log(f"[bench] FP8 error vs FP64: {err}")
     cpu = None
-    if not args.no_cpu_baseline:
+    if with_cpu:
         log("[bench] CPU oracle on one frame of the same workload")
         cpu = dynamic_cpu_baseline(scene, vel, scenegen.dynamic_config(PRECISION_FP32))
         log(f"[bench] cpu baseline {json.dumps(cpu)}")
     P = cfg.probes_x * cfg.probes_y * cfg.probes_z
+    out = {"ms_per_frame": round(ms, 4), "frames": frames, "warmup": warmup,
+           "probe_rays_per_s": round(P * cfg.rays_per_probe / (ms / 1000.0), 1),
+           "probes": P, "rays_per_probe": cfg.rays_per_probe, "triangles": scene.num_tris,
+           "conv_precision": "fp8",
+           "bvh_nodes": "quantized 32 B" if ds.info()["node_format"] == 2 else "full 64 B",
+           "translations": moves,
+           "bvh_update": (f"rebuild every {K} frames, refit in between" if K > 1 else
+                          "rebuild every frame" if K == 1 else "refit every frame"),
+           "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items()},
+           "last_bvh_update_ms": round(refit_ms, 4),
+           "fp8_error_vs_fp64": err,
+           "cpu_baseline": cpu}
+    grid.close()
+    ds.close()
+    return out
+
+
+def bench_dynamic(args):
+    """--workload dynamic4m: the dynamic scene over ``--frames`` frames, one JSON line
+    (ms per frame)."""
+    d = run_dynamic(args.frames, args.warmup, args.rebuild_every, args.node_format,
+                    not args.no_cpu_baseline)
     out = {
         "metric": "ms per frame (dynamic 4M-triangle scene: refit + translation + full probe update)",
-        "value": round(ms, 4), "unit": "ms/frame", "n_gpus": 1, "steps": frames,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "value": d["ms_per_frame"], "unit": "ms/frame", "n_gpus": 1, "steps": d["frames"],
+        "warmup": d["warmup"], "ms_per_step": d["ms_per_frame"], "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+fp8",
         "data": "synthetic (seeded scene generator, seed 5)",
         "config": {"workload": "dynamic4m: 64x32x64 probes x 256 rays, 600^2 terrain + 20k moving "
                                "box/icosphere instances, camera +1/3 m per frame",
-                   "rebuild_every": K,
-                   "probes": P, "rays_per_probe": cfg.rays_per_probe,
-                   "triangles": scene.num_tris, "conv_precision": "fp8",
-                   "bvh_nodes": "quantized 32 B" if ds.info()["node_format"] == 2 else "full 64 B",
-                   "probe_rays_per_s": round(P * cfg.rays_per_probe / (ms / 1000.0), 1)},
-        "translations": moves,
-        "bvh_update": (f"rebuild every {K} frames, refit in between" if K > 1 else
-                       "rebuild every frame" if K == 1 else "refit every frame"),
-        "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items() if k != "shadow"},
-        "last_bvh_update_ms": round(refit_ms, 4),
-        "fp8_error_vs_fp64": err,
-        "cpu_baseline": cpu,
+                   "rebuild_every": args.rebuild_every, "probes": d["probes"],
+                   "rays_per_probe": d["rays_per_probe"], "triangles": d["triangles"],
+                   "conv_precision": "fp8", "bvh_nodes": d["bvh_nodes"],
+                   "probe_rays_per_s": d["probe_rays_per_s"]},
+        "translations": d["translations"], "bvh_update": d["bvh_update"], "stages_ms": d["stages_ms"],
+        "last_bvh_update_ms": d["last_bvh_update_ms"], "fp8_error_vs_fp64": d["fp8_error_vs_fp64"],
+        "cpu_baseline": d["cpu_baseline"],
     }
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args):
+def run_serve(requests, burst, inflight, streams, graphs):
     """Serving: 64 resident scenes of uneven sizes (10k-2M triangles, grids 4^3 to
     32x16x32, 128-512 rays), BF16 convolution, requests arriving as a Poisson
     process at 200/s (open loop), admitted continuously under a 200 GB budget.
-    Reports p50/p99 latency and aggregate rays/s, then the throughput of a burst
+    Returns p50/p99 latency and aggregate rays/s, then the throughput of a burst
     of requests through the server against the plain per-scene FP32 path."""
     from anerend_amd import native, scenegen, serving
     from anerend_amd.types import PRECISION_BF16, PRECISION_FP32
@@ -558,11 +562,12 @@ def bench_serve(args):
         dss.append(native.DeviceScene(sc))
     cfgs = [scenegen.serving_config(sp, PRECISION_BF16) for sp in specs]
     budget = 200 << 30
-    srv = serving.Server(hbm_budget_bytes=budget, max_inflight=args.inflight, num_streams=args.streams,
-                         capture_graphs=not args.no_graphs)
+    srv = serving.Server(hbm_budget_bytes=budget, max_inflight=inflight, num_streams=streams,
+                         capture_graphs=graphs)
     sids = [srv.add_session(d, c) for d, c in zip(dss, cfgs)]
-    log(f"[bench] {tris} triangles resident, {srv.stats()['used_bytes'] / 2**30:.2f} GiB admitted")
-    R, rate = args.requests, 200.0
+    admitted = srv.stats()
+    log(f"[bench] {tris} triangles resident, {admitted['used_bytes'] / 2**30:.2f} GiB admitted")
+    R, rate = requests, 200.0
     rng = np.random.default_rng(4)
     arrive = np.cumsum(rng.exponential(1.0 / rate, R))
     which = rng.integers(0, len(specs), R)
@@ -597,7 +602,7 @@ def bench_serve(args):
                  "peak_inflight": st["peak_inflight"], "wall_s": round(t1 - t0, 3)}
     log(f"[bench] open loop {json.dumps(open_loop)}")
     # saturation: a burst of B requests at once through the server (BF16) ...
-    B = min(args.burst, R)
+    B = min(burst, R)
     native.synchronize()
     t0 = time.perf_counter()
     tk = [srv.submit(sids[which[i]], frames[i]) for i in range(B)]
@@ -605,6 +610,7 @@ def bench_serve(args):
         srv.poll(t, -1)
     t1 = time.perf_counter()
     serve_rays_s = float(rays[:B].sum()) / (t1 - t0)
+    held = srv.stats()
     srv.close()
     # ... and the same requests through the plain per-scene FP32 path, one stream
     plain = {}
@@ -619,21 +625,37 @@ def bench_serve(args):
     native.synchronize()
     t1 = time.perf_counter()
     plain_rays_s = float(rays[:B].sum()) / (t1 - t0)
-    burst = {"requests": B, "serve_bf16_rays_per_s": round(serve_rays_s, 1),
-             "plain_fp32_rays_per_s": round(plain_rays_s, 1),
-             "speedup": round(serve_rays_s / plain_rays_s, 3)}
-    log(f"[bench] burst {json.dumps(burst)}")
+    for g in plain.values():
+        g.close()
+    for d in dss:
+        d.close()
+    burst_d = {"requests": B, "serve_bf16_rays_per_s": round(serve_rays_s, 1),
+               "plain_fp32_rays_per_s": round(plain_rays_s, 1),
+               "speedup": round(serve_rays_s / plain_rays_s, 3)}
+    log(f"[bench] burst {json.dumps(burst_d)}")
+    return {"open_loop": open_loop, "burst": burst_d, "triangles": tris, "scenes": len(specs),
+            "hbm_budget_bytes": budget, "admitted_bytes": admitted["used_bytes"],
+            "held_bytes_after": held["held_bytes"], "max_inflight": inflight, "streams": streams,
+            "capture_graphs": graphs}
+
+
+def bench_serve(args):
+    """--workload serve64: the request loop, one JSON line (p99 latency)."""
+    d = run_serve(args.requests, args.burst, args.inflight, args.streams, not args.no_graphs)
+    ol = d["open_loop"]
     out = {
         "metric": "serving p99 latency (ms), Poisson 200 req/s over 64 resident scenes",
-        "value": open_loop["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": R, "warmup": len(sids),
-        "ms_per_step": open_loop["mean_ms"], "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator, seed 4)",
+        "value": ol["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": ol["requests"], "warmup": d["scenes"],
+        "ms_per_step": ol["mean_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic (seeded scene generator, seed 4)",
         "config": {"workload": "serve64: 64 scenes, 10k-2M triangles log-uniform, grids "
                                "{4^3, 8^3, 16x8x16, 32x16x32} x rays {128, 256, 512}, BF16 conv",
-                   "triangles": tris, "hbm_budget_bytes": budget, "max_inflight": args.inflight,
-                   "streams": args.streams, "capture_graphs": not args.no_graphs},
-        "open_loop": open_loop,
-        "burst": burst,
+                   "triangles": d["triangles"], "hbm_budget_bytes": d["hbm_budget_bytes"],
+                   "max_inflight": d["max_inflight"], "streams": d["streams"],
+                   "capture_graphs": d["capture_graphs"]},
+        "open_loop": ol,
+        "burst": d["burst"],
+        "admitted_bytes": d["admitted_bytes"], "held_bytes_after": d["held_bytes_after"],
     }
     print(json.dumps(out), flush=True)
 
@@ -657,8 +679,9 @@ def main():
               "sharding of the probe grid is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
     from anerend_amd import native, scenegen
-    from anerend_amd.types import PRECISION_BF16, PRECISION_FP8, PRECISION_FP32
-    prec = {"fp32": PRECISION_FP32, "bf16": PRECISION_BF16, "fp8": PRECISION_FP8}[args.precision]
+    from anerend_amd.types import PRECISION_BF16, PRECISION_FP8, PRECISION_FP16, PRECISION_FP32, Config
+    PRECS = {"fp32": PRECISION_FP32, "bf16": PRECISION_BF16, "fp16": PRECISION_FP16, "fp8": PRECISION_FP8}
+    prec = PRECS[args.precision]
 
     log(f"[bench] building workload {args.workload}")
     scene, cfg, frame = scenegen.workload(args.workload)
@@ -688,25 +711,64 @@ def main():
     grid.update(ds, frame_at(0))
     gwork = grid.stats()
     grid.set_counting(0)
+    sungrid_ms = gwork["mean_ms"]["sungrid"]   # the one build of this scene and sun
     log(f"[bench] work: primary nodes {work['primary_nodes']} tris {work['primary_tris']}, "
         f"shadow nodes {work['shadow_nodes']} tris {work['shadow_tris']}")
-    log(f"[bench] sun grid: shadow-ray cell lookups {gwork['shadow_nodes']}, candidate tests {gwork['shadow_tris']}")
+    log(f"[bench] sun grid: shadow-ray cell lookups {gwork['shadow_nodes']}, candidate tests "
+        f"{gwork['shadow_tris']}, build {sungrid_ms:.3f} ms")
 
-    for i in range(args.warmup):
-        grid.update(ds, frame_at(1 + i))
-    native.synchronize()
-    grid.reset_stats()
-    log(f"[bench] timing {args.steps} updates")
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        grid.update(ds, frame_at(100 + i))
-    native.synchronize()
-    t1 = time.perf_counter()
-    st = grid.stats()
-    ms = (t1 - t0) * 1000.0 / args.steps
+    def time_updates(g, frames, warmup):
+        """Warm-up updates, then the timed ones; wall ms per update and the stage means."""
+        for fr in frames[:warmup]:
+            g.update(ds, fr)
+        native.synchronize()
+        g.reset_stats()
+        t0 = time.perf_counter()
+        for fr in frames[warmup:]:
+            g.update(ds, fr)
+        native.synchronize()
+        t1 = time.perf_counter()
+        return (t1 - t0) * 1000.0 / (len(frames) - warmup), g.stats()
+
+    log(f"[bench] timing {args.steps} updates ({args.precision} convolution)")
+    ms, st = time_updates(grid, [frame_at(1 + i) for i in range(args.warmup)] +
+                          [frame_at(100 + i) for i in range(args.steps)], args.warmup)
     value = P * N / (ms / 1000.0)
     mean = st["mean_ms"]
     log(f"[bench] {ms:.3f} ms/update, stages {json.dumps({k: round(v, 4) for k, v in mean.items()})}")
+
+    # the other convolution precisions in the same run, each with its error against
+    # float64 on this scene's traced radiance (reduced on the GPU)
+    def grid_of(name, h=cfg.hysteresis):
+        return native.ProbeGrid(Config.make((cfg.probes_x, cfg.probes_y, cfg.probes_z), step=tuple(cfg.probe_step),
+                                            rays=N, hysteresis=h, precision=PRECS[name]))
+    ref32 = grid_of("fp32")
+    variants = {}
+    for name in ("fp32", "bf16", "fp16"):
+        g = grid if name == args.precision else grid_of(name)
+        if name != args.precision:
+            vms, vst = time_updates(g, [frame_at(1 + i) for i in range(args.warmup)] +
+                                    [frame_at(200 + i) for i in range(args.steps)], args.warmup)
+            variants[name] = {"ms_per_update": round(vms, 4), "probe_rays_per_s": round(P * N / (vms / 1000.0), 1),
+                              "stages_ms": {k: round(v, 4) for k, v in vst["mean_ms"].items()
+                                            if k not in ("translate", "sungrid")}}
+        else:
+            variants[name] = {"ms_per_update": round(ms, 4), "probe_rays_per_s": round(value, 1), "headline": True}
+        if name != "fp32":
+            variants[name]["conv_error_vs_fp64_on_traced_radiance"] = fp64_conv_error(g, ref32, ds, frame_at(999))
+        log(f"[bench] {name} update {json.dumps(variants[name])}")
+        if g is not grid:
+            g.close()
+    ref32.close()
+    # a sun that moves every update: the sun grid is never current, shadow rays traverse
+    def moving_sun_frame(i):
+        return scenegen.make_frame(500 + i, camera=tuple(frame.camera_pos[:3]), config=cfg, iteration=i)
+    ms_ms, st_ms = time_updates(grid, [moving_sun_frame(i) for i in range(args.warmup + args.steps)],
+                                args.warmup)
+    moving = {"ms_per_update": round(ms_ms, 4), "stages_ms": {k: round(v, 4) for k, v in st_ms["mean_ms"].items()
+                                                             if k != "translate"},
+              "sun_grid_builds": int(st_ms["calls"]["sungrid"])}
+    log(f"[bench] moving sun {json.dumps(moving)}")
 
     # calibration kernels outside the timed region, for the profiler's FETCH_SIZE /
     # WRITE_SIZE counters: one camera-cell move (the translation remap reads and
@@ -761,9 +823,21 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU oracle baseline for at least {args.cpu_seconds:.0f} s")
-        cpu = cpu_baseline(scene, cfg, frame_at, args.cpu_seconds)
-        log(f"[bench] CPU {cpu['value']:.4g} rays/s on {cpu['cores']} threads")
+        log(f"[bench] CPU oracle baseline: median of {args.cpu_runs} updates")
+        cpu = cpu_baseline(scene, cfg, frame_at, args.cpu_runs)
+        log(f"[bench] CPU {cpu['value']:.4g} rays/s on {cpu['cores']} threads ({cpu['cpu_model']})")
+
+    legs = [x for x in args.legs.split(",") if x]
+    node_format = ds.info()["node_format"]
+    grid.close()
+    ds.close()
+    dyn = serve = None
+    if "dynamic" in legs:
+        log(f"[bench] dynamic 4M-triangle leg, {args.leg_frames} frames")
+        dyn = run_dynamic(args.leg_frames, 3, args.rebuild_every, args.node_format, False)
+    if "serve" in legs:
+        log(f"[bench] serving leg, {args.leg_requests} requests")
+        serve = run_serve(args.leg_requests, args.burst, args.inflight, args.streams, not args.no_graphs)
 
     out = {
         "metric": "probe rays/s (16k probes x 512 rays DDGI update)",
@@ -776,16 +850,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {"fp32": "f32", "bf16": "f32+bf16", "fp16": "f32+fp16", "fp8": "f32+fp8"}[args.precision],
         "data": "synthetic (seeded scene generator, seed 2)",
         "config": {"workload": "terrain1m: 32x16x32 probes x 512 rays, 512^2 value-noise terrain "
                                "+ 10k box/icosphere instances",
                    "probes": P, "rays_per_probe": N, "triangles": scene.num_tris,
-                   "bvh_nodes": "quantized 32 B" if ds.info()["node_format"] == 2 else "full 64 B",
+                   "bvh_nodes": "quantized 32 B" if node_format == 2 else "full 64 B",
                    "conv_precision": args.precision, "bvh_depth": info["bvh_depth"],
                    "lbvh_build_ms": round(info["build_ms"], 3),
-                   "atlas_bytes": atlas_bytes},
-        "stages_ms": {k: round(v, 4) for k, v in mean.items() if k not in ("translate", "shadow")},
+                   "sun_grid_build_ms": round(sungrid_ms, 3),
+                   "atlas_bytes": atlas_bytes,
+                   "precision_note": "trace, shading and hysteresis blend in FP32; the convolution "
+                                     f"GEMM in {args.precision} (FP32 accumulate)"},
+        "stages_ms": {k: round(v, 4) for k, v in mean.items() if k not in ("translate", "sungrid")},
+        "updates_by_conv_precision": variants,
+        "moving_sun_update": moving,
         "work": {"primary_node_visits": work["primary_nodes"], "primary_triangle_tests": work["primary_tris"],
                  "shadow_rays_traversing": {"node_visits": work["shadow_nodes"],
                                             "triangle_tests": work["shadow_tris"]},
@@ -800,6 +879,8 @@ def main():
         "shadow_mask": shadows,
         "mirror_reflections": mirror,
         "light_shadows": lights,
+        "dynamic4m": dyn,
+        "serve64": serve,
     }
     print(json.dumps(out), flush=True)
 

@@ -65,7 +65,10 @@ typedef enum ane_status {
 typedef enum ane_precision {
   ANE_PRECISION_FP32 = 0, /* plain path: FP32 radiance, FP32 convolution */
   ANE_PRECISION_BF16 = 1, /* BF16 radiance and weights, MFMA, FP32 accumulate */
-  ANE_PRECISION_FP8 = 2   /* FP8-e4m3 radiance (per-probe scale) and weights, MFMA, FP32 accumulate */
+  ANE_PRECISION_FP8 = 2,  /* FP8-e4m3 radiance (per-probe scale) and weights, MFMA, FP32 accumulate */
+  ANE_PRECISION_FP16 = 3  /* FP16 radiance (saturating at 65504) and weights, MFMA, FP32 accumulate:
+                             the 16-bit mode whose 10-bit mantissa keeps correlated rounding (runs
+                             of nearly equal radiance: sky, lit planes) within 2e-3 of FP64 */
 } ane_precision;
 
 /* Probe grid constants: probe_helpers.glsl:1-7 (compile-time in the reference),
@@ -168,7 +171,8 @@ typedef struct ane_ddgi_stats {
   /* device time per stage from hipEvent pairs, ms: the last call and the mean
    * over all calls since the last ane_ddgi_stats_reset.  Stage order:
    * setup, trace (primary rays and, in the same launch, the shadow ray of
-   * every hit), shadow (kept for the layout: always 0 calls), shade, convolve,
+   * every hit), sun grid (the build of the shadow rays' candidate lists, only in
+   * the updates that build one: a new scene version or sun), shade, convolve,
    * translate. */
   float last_ms[6];
   float mean_ms[6];
@@ -351,6 +355,17 @@ int ane_ddgi_stats_reset(ane_ddgi_ctx* ctx);
  * the reference's closest-hit shaders do).  Waits for `stream` first. */
 int ane_ddgi_tiles_set(ane_ddgi_ctx* ctx, const int32_t* tile_xz, const uint16_t* rgba16f,
                        int32_t num_tiles, int32_t tile_size, void* stream);
+/* Validation in FP64 on the GPU (no counterpart in the reference): the error of
+ * conv's last convolution against a float64 evaluation of the reference's sum
+ * (probe_conv.comp:49-61) over the FP32 radiance `ref` holds.  Use: give both
+ * contexts the same grid and atlas, trace the same frame in ref (ane_ddgi_trace,
+ * or set its radiance) and update conv with it (any precision).  Per texel and
+ * channel, E = (conv atlas - h * ref atlas) / (1 - h) with conv's hysteresis h,
+ * E64 = sum W L / sum W in float64 (W = max(0, d . o) over the FP32 directions,
+ * kept above 1e-6), rel = |E - E64| / max(|E64|, rel_floor); out: the max and
+ * the root mean square of rel over the layers ref holds.  Waits for `stream`. */
+int ane_ddgi_conv_error_fp64(ane_ddgi_ctx* conv, const ane_ddgi_ctx* ref, double rel_floor,
+                             double* max_rel, double* rms_rel, void* stream);
 /* Waits for `stream`, then reads and clears the context's traversal fault flags
  * (probe traces and the last mirror pass): ANE_ERR_DEVICE if any ray since the
  * last check ended without a result, else ANE_OK.  The reference has no such

@@ -7,8 +7,9 @@
  *
  *  - a session = a resident scene (ane_scene, its LBVH) + a probe grid
  *    (atlas and work buffers); sessions are admitted against an HBM budget
- *    (context bytes + each distinct scene's bytes once) or refused with
- *    ANE_ERR_ADMISSION_REFUSED;
+ *    (context bytes, including the traversal spill and sun grid the updates
+ *    use, allocated at admission, + each distinct scene's bytes once) or
+ *    refused with ANE_ERR_ADMISSION_REFUSED;
  *  - requests (one update of one session with its frame constants) are queued
  *    from any host thread; a dispatcher thread launches every queued request
  *    whose session has no update on the device while fewer than max_inflight
@@ -53,6 +54,8 @@ typedef struct ane_serve_stats {
   uint64_t completed, failed;
   int64_t rays_done;
   int64_t used_bytes, budget_bytes;
+  int64_t held_bytes; /* device bytes the sessions and their scenes hold now (= used_bytes:
+                         admission charges each session's spill and sun grid up front) */
 } ane_serve_stats;
 
 typedef struct ane_serve ane_serve;

@@ -10,6 +10,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1473,9 +1474,27 @@ int ora_translate(const ora_config* c, const float* cam, int32_t* last, float* a
 /* ------------------------------------------------------------------------ */
 /* One update (IrradianceProbeRT then IrradianceProbeConvolve).              */
 /* ------------------------------------------------------------------------ */
+/* wall-clock ms of the last ora_update's two stages: [0] trace + shadow + shade,
+ * [1] convolve + blend (the CPU baseline's per-stage figures) */
+static double g_stage_ms[2];
+
+static double now_ms(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+void ora_last_update_stage_ms(double* out2)
+{
+  out2[0] = g_stage_ms[0];
+  out2[1] = g_stage_ms[1];
+}
+
 void ora_update(const ora_scene* s, const ora_config* c, const ora_frame* f, float* atlas,
                 ora_hit* hits, uint8_t* shadowed, float* radiance, uint64_t* counters)
 {
+  const double t0 = now_ms();
   int n = c->rays_per_probe;
   float* base = (float*)malloc(sizeof(float) * 3 * (size_t)n);
   float* dirs = (float*)malloc(sizeof(float) * 3 * (size_t)n);
@@ -1509,9 +1528,12 @@ void ora_update(const ora_scene* s, const ora_config* c, const ora_frame* f, flo
       if (shadowed) shadowed[q * n + r] = (uint8_t)(sh < 0 ? 255 : sh);
     }
   }
+  const double t1 = now_ms();
   if (fp16) /* the convolution reads the directions back from the RGBA16F image, rgen:108 */
     for (int k = 0; k < 3 * n; ++k) base[k] = f16_round(dirs[k]);
   ora_convolve_blend(c, f, fp16 ? base : dirs, rad, atlas);
+  g_stage_ms[0] = t1 - t0;
+  g_stage_ms[1] = now_ms() - t1;
   if (counters) { counters[0] += c0; counters[1] += c1; counters[2] += c2; counters[3] += c3; }
   if (!radiance) free(rad);
   free(base);

@@ -221,6 +221,9 @@ void ora_sample_probe(const ora_config* cfg, const ora_frame* fr, const float* a
 void ora_tiles_set(const ora_config* cfg, const int32_t* tile_xz, const float* atlases, int32_t n,
                    int32_t tile_size);
 
+/* wall-clock ms of the last ora_update: [0] trace + shadow + shade, [1] convolve + blend */
+void ora_last_update_stage_ms(double* out2);
+
 /* binary32 -> binary16 (round to nearest, ties to even; >= 65520 -> inf) -> binary32 */
 void ora_f16_round(const float* in, float* out, int64_t n);
 

@@ -68,6 +68,7 @@ def lib():
         L.ora_atlas_clear.argtypes = [vp, vp]
         L.ora_f16_round.argtypes = [vp, vp, i64]
         L.ora_tiles_set.argtypes = [vp, vp, vp, i32, i32]
+        L.ora_last_update_stage_ms.argtypes = [vp]
         L.ora_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -318,6 +319,13 @@ def new_atlas(cfg):
     a = np.zeros(cfg.atlas_shape, np.float32)
     lib().ora_atlas_clear(_ref(cfg), ptr(a))
     return a
+
+
+def last_update_stage_ms():
+    """Wall-clock ms of the last update: (trace + shadow + shade, convolve + blend)."""
+    out = np.zeros(2, np.float64)
+    lib().ora_last_update_stage_ms(ptr(out))
+    return float(out[0]), float(out[1])
 
 
 def num_threads():

@@ -1,15 +1,12 @@
 """bench.py on the CPU: the one-GPU rule for every workload, the committed traffic
-summary it prices the roofline with, and its float64 texel directions (the FP64
-reference of the convolution study) against the oracle's binary32 ones."""
+summary it prices the roofline with, and the CPU model string of its baseline."""
 import os
 import subprocess
 import sys
 
-import numpy as np
 import pytest
 
 import bench
-from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -28,5 +25,5 @@ def test_traffic_summary_is_parsed():
     assert isinstance(t, int) and t > 0
 
 
-def test_texel_directions_float64_match_oracle():
-    np.testing.assert_allclose(bench.texel_dirs64(), O.texel_dirs().astype(np.float64), atol=1e-7)
+def test_cpu_model_is_named():
+    assert isinstance(bench.cpu_model(), str) and bench.cpu_model()

@@ -15,14 +15,18 @@ import pytest
 
 from anerend_amd import native, scenegen
 from anerend_amd.types import (FLAG_BAKE, FLAG_DDGI, FLAG_DDGI_MULTI, PRECISION_BF16, PRECISION_FP8,
-                               PRECISION_FP32, Config)
+                               PRECISION_FP16, PRECISION_FP32, Config)
 from oracle import oracle as O
 from tests.conftest import bits, rel_err
 
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-4            # FP32 path vs oracle
-BF16_MAX_REL = 2e-3    # BF16 MFMA convolution vs float64: the required bound
+BF16_MAX_REL = 2e-3    # 16-bit MFMA convolution vs float64 on log-uniform radiance: the required bound
+FP16_MAX_REL = 2e-3    # FP16 MFMA convolution vs float64, also on traced radiance
+BF16_TRACED_MAX_REL = 2.0 ** -8  # BF16 on traced radiance: runs of nearly equal values (sky, lit
+                                 # planes) round alike, so the error reaches half an ulp of an
+                                 # 8-bit mantissa, at most 2^-8 relative
 FP8_MAX_REL = 6e-2     # FP8-e4m3 (per-probe + per-texel scales): stated bound
 
 
@@ -64,9 +68,21 @@ def _interior(atlas, cfg):
     return np.array(out)
 
 
+def texel_dirs64():
+    """Octahedral directions of the 8x8 tile texel centres in float64, t = ty * 8 + tx
+    (probe_conv.comp:37-46, octahedron_helpers.glsl:25-34: y and z swapped)."""
+    u = (2.0 * np.arange(8) + 1.0) / 16.0 * 2.0 - 1.0
+    ox, oy = (a.ravel() for a in np.meshgrid(u, u))
+    z = 1.0 - np.abs(ox) - np.abs(oy)
+    nx = np.where(z < 0, (1.0 - np.abs(oy)) * np.where(ox >= 0, 1.0, -1.0), ox)
+    ny = np.where(z < 0, (1.0 - np.abs(ox)) * np.where(oy >= 0, 1.0, -1.0), oy)
+    v = np.stack([nx, z, ny], 1)
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
 def _fp64_irradiance(cfg, fr, L):
     d = O.rotate_dirs(fr.rotation, O.fibonacci_dirs(cfg.rays_per_probe)).astype(np.float64)
-    W = np.maximum(0.0, O.texel_dirs().astype(np.float64) @ d.T)
+    W = np.maximum(0.0, texel_dirs64() @ d.T)
     W[W <= 1e-6] = 0
     return np.einsum("tr,pcr->ptc", W, L.astype(np.float64)) / W.sum(1)[None, :, None]
 
@@ -86,7 +102,7 @@ def test_convolution_fp32_matches_oracle():
 
 
 @pytest.mark.parametrize("precision,bound", [(PRECISION_FP32, 1e-5), (PRECISION_BF16, BF16_MAX_REL),
-                                             (PRECISION_FP8, FP8_MAX_REL)])
+                                             (PRECISION_FP16, FP16_MAX_REL / 4), (PRECISION_FP8, FP8_MAX_REL)])
 def test_convolution_precisions_vs_float64(precision, bound):
     cfg, fr, L = _conv_case(precision)
     grid = native.ProbeGrid(cfg)
@@ -200,11 +216,11 @@ def test_production_timed_path_parity(terrain1m):
     built on the second update with an unchanged scene and sun).  Three updates
     from one atlas on the GPU and on the oracle: hits (id, t, u, v) and shadow flags
     bit-exact on every ray, radiance and atlas within 1e-4, sun grid in use.
-    Then the BF16 MFMA convolution on this traced radiance: one update with
-    hysteresis 0 from the same atlas in an FP32 and a BF16 grid; the BF16
-    irradiance is within 2e-3 of a float64 GEMM of the FP32-traced radiance and
-    the FP32 irradiance within 1e-4 (reference: irradiance_probe_update.rchit:44-66,
-    probe_conv.comp:21-61)."""
+    Then the 16-bit MFMA convolutions on this traced radiance: one update with
+    hysteresis 0 from the same atlas in FP32, FP16 and BF16 grids against a
+    float64 GEMM of the FP32-traced radiance: FP32 within 1e-4, FP16 within 2e-3,
+    BF16 within half an ulp of its 8-bit mantissa (reference:
+    irradiance_probe_update.rchit:44-66, probe_conv.comp:21-61)."""
     scene, cfg, frame, ds, osc = terrain1m
     assert cfg.precision == PRECISION_FP32
     grid = native.ProbeGrid(cfg)
@@ -232,24 +248,35 @@ def test_production_timed_path_parity(terrain1m):
     g32 = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0))
     g16 = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0,
                                        precision=PRECISION_BF16))
-    for g in (g32, g16):
+    gh = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0,
+                                      precision=PRECISION_FP16))
+    for g in (g32, g16, gh):
         g.atlas = atlas
         g.update(ds, fr)
     L = g32.radiance(fr).astype(np.float64)
     d = O.rotate_dirs(fr.rotation, O.fibonacci_dirs(n)).astype(np.float64)
-    W = np.maximum(0.0, O.texel_dirs().astype(np.float64) @ d.T)
+    W = np.maximum(0.0, texel_dirs64() @ d.T)
     W[W <= 1e-6] = 0
     P = nx * ny * nz
     ref = ((L.reshape(P * 3, n) @ W.T) / W.sum(1)).reshape(P, 3, 64).transpose(0, 2, 1)
     E32 = _interior(g32.atlas, cfg)[..., :3].astype(np.float64)
     E16 = _interior(g16.atlas, cfg)[..., :3].astype(np.float64)
+    Eh = _interior(gh.atlas, cfg)[..., :3].astype(np.float64)
     floor = 1e-3
     rel32 = np.abs(E32 - ref) / np.maximum(np.abs(ref), floor)
     rel16 = np.abs(E16 - ref) / np.maximum(np.abs(ref), floor)
+    relh = np.abs(Eh - ref) / np.maximum(np.abs(ref), floor)
     print(f"[timed path] terrain1m irradiance vs float64: fp32 max {rel32.max():.3e}, "
+          f"fp16 max {relh.max():.3e} rms {np.sqrt((relh ** 2).mean()):.3e}, "
           f"bf16 max {rel16.max():.3e} rms {np.sqrt((rel16 ** 2).mean()):.3e}", flush=True)
     assert rel32.max() <= RTOL
-    assert rel16.max() <= BF16_MAX_REL
+    assert relh.max() <= FP16_MAX_REL
+    assert rel16.max() <= BF16_TRACED_MAX_REL
+    # the reduction on the GPU agrees with numpy's
+    ref_grid = native.ProbeGrid(Config.make((nx, ny, nz), step=tuple(cfg.probe_step), rays=n, hysteresis=0.0))
+    ref_grid.atlas = atlas
+    ref_grid.trace(ds, fr)
+    assert gh.conv_error_fp64(ref_grid)[0] == pytest.approx(relh.max(), rel=1e-5)
 
 
 def test_room8_matches_golden_fixture():
@@ -405,3 +432,34 @@ def test_shadow_mask_and_mirror_through_sun_grid_match_oracle():
         assert np.array_equal(bits(og[..., 3]), bits(oo[..., 3])), f"mirror distances differ at call {call}"
         assert rel_err(og[..., :3], oo[..., :3]) <= RTOL
     assert 0 < (vo == 0).sum() < vo.size
+
+
+@pytest.mark.parametrize("precision,h", [(PRECISION_FP32, 0.0), (PRECISION_BF16, 0.96), (PRECISION_FP16, 0.5),
+                                         (PRECISION_FP8, 0.0)])
+def test_fp64_error_reduction_on_gpu_matches_numpy(precision, h):
+    """ane_ddgi_conv_error_fp64 (float64 sum and reduction on the GPU) gives the
+    max and RMS relative irradiance error that numpy computes in float64 from the
+    same atlases and radiance; with hysteresis the convolution's E is recovered
+    from the blend as (new - h old) / (1 - h)."""
+    cfg, fr, L = _conv_case(precision, h=h)
+    ref = native.ProbeGrid(Config.make((cfg.probes_x, cfg.probes_y, cfg.probes_z), step=tuple(cfg.probe_step),
+                                       rays=cfg.rays_per_probe, hysteresis=h))
+    grid = native.ProbeGrid(cfg)
+    rng = np.random.default_rng(8)
+    old = O.new_atlas(cfg)
+    old[..., :3] = rng.uniform(0.01, 3.0, old[..., :3].shape)
+    grid.atlas = old
+    ref.atlas = old
+    grid.set_radiance(fr, L)
+    grid.convolve(fr)
+    ref.set_radiance(fr, L)
+    mx, rms = grid.conv_error_fp64(ref, rel_floor=1e-3)
+    new = _interior(grid.atlas, cfg)[..., :3].astype(np.float64)
+    prev = _interior(old, cfg)[..., :3].astype(np.float64)
+    h32 = float(np.float32(h))       # the context holds the hysteresis in binary32
+    E = (new - h32 * prev) / (1.0 - h32)
+    want = _fp64_irradiance(cfg, fr, L)
+    rel = np.abs(E - want) / np.maximum(np.abs(want), 1e-3)
+    print(f"precision {precision} h {h}: GPU fp64 max {mx:.6e} rms {rms:.6e}; numpy max {rel.max():.6e}")
+    assert mx == pytest.approx(rel.max(), rel=1e-6)
+    assert rms == pytest.approx(np.sqrt((rel ** 2).mean()), rel=1e-6)

@@ -135,3 +135,10 @@ def test_translation_matches_tile_restatement():
         assert moved == (dx != 0 or dz != 0)
         assert np.array_equal(atlas, expect)
         assert tuple(last) == (ix, iz)
+
+
+def test_float64_texel_directions_match_oracle():
+    """The float64 texel directions the GPU parity tests use as their reference
+    equal the oracle's binary32 ones to binary32 rounding."""
+    from tests.test_gpu_passes import texel_dirs64
+    np.testing.assert_allclose(texel_dirs64(), O.texel_dirs().astype(np.float64), atol=1e-7)
