@@ -376,7 +376,6 @@ struct ane_scene {
   NodeBuild* d_nodes = nullptr;
   int32_t* d_leaf_parent = nullptr;
   float4* d_tris = nullptr;
-  uint32_t* d_arrive = nullptr;
   float* d_node_box = nullptr;
   uint32_t* d_cbounds = nullptr;
   uint32_t* d_cpart = nullptr;  // per-block centroid-bound partials of a build
@@ -438,7 +437,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_vertices); dfree(s->d_indices); dfree(s->d_meshes); dfree(s->d_mats);
   dfree(s->d_inst); dfree(s->d_inv); dfree(s->d_tri_inst); dfree(s->d_tri_rec); dfree(s->d_inst_off);
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
-  dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris); dfree(s->d_arrive);
+  dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_qnodes); dfree(s->d_scratch);
   dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
@@ -633,7 +632,6 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_nodes, n > 1 ? n - 1 : 1);
   ANE_A(d_leaf_parent, n);
   ANE_A(d_tris, n * 3);
-  ANE_A(d_arrive, n);
   ANE_A(d_node_box, (n > 1 ? n - 1 : 1) * 6);
   ANE_A(d_cbounds, 6);
   ANE_A(d_cpart, flatten_partial_words((int)n));
@@ -937,7 +935,8 @@ extern "C" int ane_ddgi_create(const ane_ddgi_config* cfg, ane_ddgi_ctx** out)
     if (st) { ctx_free(c); delete c; return st; }
     c->d_rad = r;
   }
-  ANE_A(d_rad_scale, c->P);
+  // FP8 radiance scales: one float per (probe, channel, 32 rays)
+  ANE_A(d_rad_scale, c->P * 3 * ((c->N + 31) / 32));
   ANE_A(d_work, 4);
   ANE_A(d_stats, 4);
   ANE_A(d_overflow, 1);
@@ -1202,6 +1201,70 @@ namespace ane {
 // capture; stage timers record nothing meanwhile, so the graph holds kernel
 // nodes only (counters are cleared by a fill kernel).  Counting contexts update
 // directly.
+// The front of an update for the request loop's fused stages: frame setup, sun
+// grid and trace of (c, s, f) on `st`, as ane_ddgi_trace does them; the
+// shading and convolution that ane_ddgi_update would launch next are returned
+// as job records (run later in one launch per stage with other updates' jobs,
+// on the same stream).  *precision: the jobs' precision.
+int ddgi_update_front(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st,
+                      ShadeJob* sj, ConvJob* cj, int* precision)
+{
+  if (!c || !s || !sj || !cj || !precision) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  FrameK fk;
+  ANE_TRY(frame_to_k(c, f, &fk));
+  ANE_TRY(ensure_spill(c, s->depth));
+  const bool bake = (fk.flags & ANE_FLAG_BAKE) != 0;
+  const int nprobes = fk.layer_count * c->g.nx * c->g.nz;
+  ANE_TRY(do_frame_setup(c, fk, bake, st));
+  fill_u32(c->d_work, 0u, 4, st);
+  if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->d_stats), 0u, 8, st);
+  TraceArgs a = trace_args(c, s, fk);
+  a.work = c->d_work;
+  if (sun_grid_wanted(c)) {
+    bool use = false;
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use));
+    if (use) a.sg = c->sgc.view();
+  }
+  launch_trace(a, false, false, c->counting, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  *sj = ShadeJob{};
+  sj->sc = s->view;
+  sj->g = c->g;
+  sj->f = fk;
+  sj->probe_org = c->d_probe_org;
+  sj->hits = c->d_hits;
+  sj->shadow = c->d_shadow;
+  sj->atlas = reinterpret_cast<const float4*>(c->d_atlas[c->cur]);
+  sj->rad = c->d_rad;
+  sj->rad_scale = c->d_rad_scale;
+  sj->probe_base = a.probe_base;
+  *cj = ConvJob{};
+  cj->g = c->g;
+  cj->rad = c->d_rad;
+  cj->rad_scale = c->d_rad_scale;
+  const int prec = c->cfg.precision;
+  cj->wq = prec == ANE_PRECISION_FP32 ? (const void*)c->d_w32
+           : prec == ANE_PRECISION_FP8 ? (const void*)c->d_wfp8 : (const void*)c->d_wbf16;
+  cj->wscale = c->d_wfp8_scale;
+  cj->wsum = c->d_wsum;
+  cj->atlas = reinterpret_cast<float4*>(c->d_atlas[c->cur]);
+  cj->nprobes = nprobes;
+  cj->probe_base = a.probe_base;
+  cj->bake = bake ? 1 : 0;
+  *precision = prec;
+  c->rad_begin = fk.layer_begin;
+  c->rad_count = fk.layer_count;
+  c->rad_bake = bake;
+  c->last_begin = fk.layer_begin;
+  c->last_count = fk.layer_count;
+  c->stats.rays = (uint64_t)nprobes * c->N;
+  return ANE_OK;
+}
+
+// whether the context's convolution can run as a fused job (MFMA kernels need
+// a ray count that is a multiple of 32)
+bool ddgi_fusable(const ane_ddgi_ctx* c) { return c && c->N % 32 == 0; }
+
 int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st)
 {
   if (!c || !s) return fail(ANE_ERR_INVALID_ARG, "NULL context or scene");

@@ -314,16 +314,46 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* keys, int
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// One block per digit d: exclusive scan of hist[d][0..ntiles) in place (the
+// digit's count in the tiles before each tile) and the digit's total.
+__global__ __launch_bounds__(kBlock) void k_radix_digit_scan(uint32_t* hist, int64_t ntiles, uint32_t* totals)
+{
+  uint32_t* h = hist + (int64_t)blockIdx.x * ntiles;
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < ntiles; base += kTile) {
+    const int64_t i0 = base + (int64_t)threadIdx.x * kItems;
+    uint32_t v[kItems], s = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      v[i] = (i0 + i < ntiles) ? h[i0 + i] : 0u;
+      s += v[i];
+    }
+    uint32_t total;
+    uint32_t pre = carry + block_exclusive_scan(s, &total);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (i0 + i < ntiles) h[i0 + i] = pre;
+      pre += v[i];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Stable scatter of one tile: digit d of tile t goes to (digits below d in all
+// tiles) + (digit d in the tiles before t) + (rank within the tile).
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin,
                                                          uint64_t* kout, uint32_t* vout, int64_t n,
                                                          int shift, const uint32_t* offs,
-                                                         int64_t ntiles)
+                                                         const uint32_t* totals, int64_t ntiles)
 {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[kBlock / 64][256];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  run[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  uint32_t tot;
+  const uint32_t below = block_exclusive_scan(totals[threadIdx.x], &tot);
+  run[threadIdx.x] = below + offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
   int64_t base = (int64_t)blockIdx.x * kTile;
   for (int r = 0; r < kItems; ++r) {
 #pragma unroll
@@ -365,8 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, c
 int64_t radix_scratch_words(int64_t n)
 {
   int64_t ntiles = (n + kTile - 1) / kTile;
-  int64_t h = 256 * ntiles;
-  return h + scan_scratch_words(h);
+  return 256 * ntiles + 256;  // tile histograms, digit totals
 }
 
 // Sorts (keys, vals) in place using (k2, v2) as ping-pong buffers (8 passes:
@@ -377,7 +406,7 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2
   if (n <= 1) return;
   int64_t ntiles = (n + kTile - 1) / kTile;
   uint32_t* hist = scratch;
-  uint32_t* sc = scratch + 256 * ntiles;
+  uint32_t* totals = scratch + 256 * ntiles;
   uint64_t* ka = keys;
   uint32_t* va = vals;
   uint64_t* kb = k2;
@@ -386,10 +415,10 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* k2, uint32_t* v2
     int shift = 8 * pass;
     hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s, (const uint64_t*)ka,
                        n, shift, hist, ntiles);
-    exclusive_scan(hist, 256 * ntiles, sc, s);
+    hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(kBlock), 0, s, hist, ntiles, totals);
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
                        (const uint64_t*)ka, (const uint32_t*)va, kb, vb, n, shift,
-                       (const uint32_t*)hist, ntiles);
+                       (const uint32_t*)hist, (const uint32_t*)totals, ntiles);
     uint64_t* tk = ka; ka = kb; kb = tk;
     uint32_t* tv = va; va = vb; vb = tv;
   }
@@ -570,12 +599,12 @@ __global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int 
   val[i] = (uint32_t)i;
 }
 
-// After the exclusive scan of the digit-major tile histograms, the offset of
-// digit d in tile 0 is the start of depth level d in the sorted order.
-__global__ void k_level_offsets(const uint32_t* offs, int64_t ntiles, uint32_t* begin)
+// The start of depth level d in the sorted order: the count of all shallower nodes.
+__global__ __launch_bounds__(kBlock) void k_level_offsets(const uint32_t* totals, uint32_t* begin)
 {
-  const int d = threadIdx.x;
-  if (d < kMaxLevels) begin[d] = offs[(int64_t)d * ntiles];
+  uint32_t tot;
+  const uint32_t pre = block_exclusive_scan(totals[threadIdx.x], &tot);
+  if (threadIdx.x < kMaxLevels) begin[threadIdx.x] = pre;
 }
 
 // One depth level: every child is a leaf or belongs to a deeper, finished level.
@@ -657,15 +686,15 @@ int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, 
   // one stable counting pass on the 8-bit depth digit (no global atomics)
   const int64_t ntiles = (m + kTile - 1) / kTile;
   uint32_t* hist = scratch;
-  uint32_t* sc = scratch + 256 * ntiles;
+  uint32_t* totals = scratch + 256 * ntiles;
   hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s, (const uint64_t*)key, m,
                      0, hist, ntiles);
-  exclusive_scan(hist, 256 * ntiles, sc, s);
+  hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(kBlock), 0, s, hist, ntiles, totals);
   hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
                      (const uint64_t*)key, (const uint32_t*)val, key2,
-                     reinterpret_cast<uint32_t*>(order), m, 0, (const uint32_t*)hist, ntiles);
-  hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(kMaxLevels), 0, s, (const uint32_t*)hist, ntiles,
-                     begin);
+                     reinterpret_cast<uint32_t*>(order), m, 0, (const uint32_t*)hist, (const uint32_t*)totals,
+                     ntiles);
+  hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(kBlock), 0, s, (const uint32_t*)totals, begin);
   uint32_t h[kMaxLevels];
   if (hipMemcpyAsync(h, begin, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;

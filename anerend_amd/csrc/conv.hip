@@ -12,7 +12,7 @@
 //   k_conv_mfma  FP32 (v_mfma_f32_16x16x4_f32, the plain drop-in path), BF16
 //                (v_mfma_f32_16x16x32_bf16) or FP8-e4m3 (v_mfma_f32_16x16x32_fp8_fp8)
 //                operands, FP32 accumulate; FP8 carries a per-texel weight scale
-//                and a per-probe radiance scale
+//                and an FP32 radiance scale per probe, channel and 32 rays
 //   k_conv_fp32  FP32 register-tiled kernel for ray counts that are not a
 //                multiple of 32
 #include "common.hpp"
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(256) void k_conv_fp32(GridK g, const float* rad, co
 // MFMA convolution (FP32, BF16, FP8 operands, FP32 accumulate).
 // Block = 16 probes (one MFMA column tile) x 64 texels x 3 channels, 4 waves.
 // The 4 waves split the rays: wave w takes the 32-ray steps s = w, w + 4, ...,
-// loading the weight (A) and radiance (B) fragments of up to CH steps at once
-// so that their latencies overlap, and accumulates 4 row tiles x 3 channels.
+// loading the weight (A) and radiance (B) fragments of CH steps at a time, and
+// accumulates 4 row tiles x 3 channels.
 // The partial sums meet in LDS; then all 256 threads run the hysteresis
 // epilogue with consecutive threads on consecutive texels of a tile row (each
 // probe's row is 128 contiguous bytes of the atlas).
@@ -132,6 +132,14 @@ __global__ __launch_bounds__(256) void k_conv_fp32(GridK g, const float* rad, co
 //     contracts k = 8q + e over the lane groups q = 0..3 (a permutation of the
 //     32 rays of the step, the same for A and B)
 // ---------------------------------------------------------------------------
+// 32-ray steps a wave loads at once (16-bit / FP8 and FP32 operands).  One step
+// keeps the registers low enough for 4 waves per SIMD; 2 and 4 measured 3-8% slower.
+#ifndef ANE_CONV_CH
+#define ANE_CONV_CH 1
+#endif
+#ifndef ANE_CONV_CH32
+#define ANE_CONV_CH32 1
+#endif
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) int i32x4;
@@ -140,7 +148,7 @@ template <int PREC>
 struct ConvFrag;
 template <>
 struct ConvFrag<ANE_PRECISION_FP32> {
-  static constexpr int EB = 4, CH = 2;
+  static constexpr int EB = 4, CH = ANE_CONV_CH32;
   float v[8];
   __device__ __forceinline__ void load(const uint8_t* p)
   {
@@ -161,7 +169,7 @@ struct ConvFrag<ANE_PRECISION_FP32> {
 };
 template <>
 struct ConvFrag<ANE_PRECISION_BF16> {
-  static constexpr int EB = 2, CH = 4;
+  static constexpr int EB = 2, CH = ANE_CONV_CH;
   bf16x8 v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const bf16x8*>(p); }
   __device__ __forceinline__ void zero() { v = bf16x8{}; }
@@ -173,7 +181,7 @@ struct ConvFrag<ANE_PRECISION_BF16> {
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 template <>
 struct ConvFrag<ANE_PRECISION_FP16> {
-  static constexpr int EB = 2, CH = 4;
+  static constexpr int EB = 2, CH = ANE_CONV_CH;
   f16x8 v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const f16x8*>(p); }
   __device__ __forceinline__ void zero() { v = f16x8{}; }
@@ -184,7 +192,7 @@ struct ConvFrag<ANE_PRECISION_FP16> {
 };
 template <>
 struct ConvFrag<ANE_PRECISION_FP8> {
-  static constexpr int EB = 1, CH = 4;
+  static constexpr int EB = 1, CH = ANE_CONV_CH;
   long v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *reinterpret_cast<const long*>(p); }
   __device__ __forceinline__ void zero() { v = 0l; }
@@ -198,23 +206,28 @@ static constexpr int kConvProbes = 16;
 static constexpr int kConvWaves = 4;
 static constexpr int kPartStride = 17;  // texel stride of the LDS partials (probe-minor, padded)
 
+// LDS of the partial sums: waves 2 and 3 hand theirs to waves 0 and 1, which
+// add them and leave two partials for the epilogue (26 KB, 6 blocks per CU)
+typedef float ConvPart[2][3][64 * kPartStride];
+
+// One block's 16 probes (see above); blk is the block's index within its update.
 template <int PREC>
-__global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const void* rad,
-                                                              const float* rad_scale, const void* wq,
-                                                              const float* wscale, const float* wsum,
-                                                              float4* atlas, int nprobes,
-                                                              int probe_base, int32_t bake)
+__device__ __forceinline__ void conv_block(const GridK& g, const void* rad, const float* rad_scale,
+                                           const void* wq, const float* wscale, const float* wsum,
+                                           float4* atlas, int nprobes, int probe_base, int32_t bake, int blk,
+                                           ConvPart& part)
 {
   using F = ConvFrag<PREC>;
   constexpr int EB = F::EB, CH = F::CH;
-  __shared__ float part[kConvWaves][3][64 * kPartStride];
   const int n = g.rays, S = n / 32;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int col = lane & 15, kq = lane >> 4;
-  const int p0 = blockIdx.x * kConvProbes;
+  const int p0 = blk * kConvProbes;
   const bool pvalid = p0 + col < nprobes;
   const uint8_t* radp = (const uint8_t*)rad + (int64_t)(pvalid ? p0 + col : 0) * 3 * n * EB;
+  // FP8: the scale of each (channel, 32-ray step) of this lane's probe
+  const float* rsp = rad_scale + (int64_t)(pvalid ? p0 + col : 0) * 3 * S;
   const uint8_t* wp = (const uint8_t*)wq;
   f32x4 acc[4][3];
 #pragma unroll
@@ -223,10 +236,15 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const vo
     for (int c = 0; c < 3; ++c) acc[rt][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int s0 = wave; s0 < S; s0 += kConvWaves * CH) {
     F a[CH][4], b[CH][3];
+    float bs[CH][3];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
       const int s = s0 + kConvWaves * j;
       const int64_t k = 32 * s + 8 * kq;
+      if (PREC == ANE_PRECISION_FP8) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bs[j][c] = (s < S && pvalid) ? rsp[c * S + s] : 0.0f;
+      }
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
         if (s < S) a[j][rt].load(wp + ((int64_t)(rt * 16 + col) * n + k) * EB);
@@ -243,17 +261,42 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const vo
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[rt][c] = F::mma(a[j][rt], b[j][c], acc[rt][c]);
+        for (int c = 0; c < 3; ++c) {
+          if (PREC == ANE_PRECISION_FP8) {
+            // each 32-ray step carries its own radiance scale: a fresh product, scaled in
+            const f32x4 t = F::mma(a[j][rt], b[j][c], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            acc[rt][c] += t * bs[j][c];
+          } else {
+            acc[rt][c] = F::mma(a[j][rt], b[j][c], acc[rt][c]);
+          }
+        }
   }
-  // partial sums: part[wave][c][t * stride + probe]
+  // partial sums, part[w][c][t * stride + probe]: waves 2, 3 store theirs, waves
+  // 0, 1 add them to their own and store the pair sums
+  if (wave >= 2) {
 #pragma unroll
-  for (int rt = 0; rt < 4; ++rt)
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int t = rt * 16 + 4 * kq + i;
+      for (int i = 0; i < 4; ++i) {
+        const int t = rt * 16 + 4 * kq + i;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) part[wave][c][t * kPartStride + col] = acc[rt][c][i];
-    }
+        for (int c = 0; c < 3; ++c) part[wave - 2][c][t * kPartStride + col] = acc[rt][c][i];
+      }
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int t = rt * 16 + 4 * kq + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float* q = &part[wave][c][t * kPartStride + col];
+          *q = acc[rt][c][i] + *q;
+        }
+      }
+  }
   __syncthreads();
   const float h = g.hysteresis;
 #pragma unroll
@@ -264,19 +307,59 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const vo
     const int t = ty * 8 + tx;
     float r[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float v = 0.0f;
-#pragma unroll
-      for (int w = 0; w < kConvWaves; ++w) v += part[w][c][t * kPartStride + p];
-      r[c] = v;
-    }
+    for (int c = 0; c < 3; ++c) r[c] = part[0][c][t * kPartStride + p] + part[1][c][t * kPartStride + p];
     // per-texel weight scale (FP8 dequantization) or weight-sum correction (BF16)
-    const float sc = (PREC == ANE_PRECISION_FP8) ? wscale[t] * rad_scale[p0 + p]
+    const float sc = (PREC == ANE_PRECISION_FP8) ? wscale[t]
                      : (PREC == ANE_PRECISION_BF16 || PREC == ANE_PRECISION_FP16) ? wscale[64 + t] : 1.0f;
     const int64_t ti = texel_index(g, probe_base + p0 + p, t);
     const float4 old = atlas[ti];
     atlas[ti] = blend_texel(old, r[0] * sc, r[1] * sc, r[2] * sc, wsum[t], h, bake != 0);
   }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv_mfma(GridK g, const void* rad,
+                                                              const float* rad_scale, const void* wq,
+                                                              const float* wscale, const float* wsum,
+                                                              float4* atlas, int nprobes,
+                                                              int probe_base, int32_t bake)
+{
+  __shared__ ConvPart part;
+  conv_block<PREC>(g, rad, rad_scale, wq, wscale, wsum, atlas, nprobes, probe_base, bake, blockIdx.x, part);
+}
+
+// The convolutions of many updates in one launch (the request loop's fused
+// stage): block b runs block b - block_begin of the last job whose block_begin <= b.
+template <int PREC>
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv_jobs(const ConvJob* jobs, int njobs)
+{
+  __shared__ ConvPart part;
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const ConvJob& j = jobs[lo];
+  conv_block<PREC>(j.g, j.rad, j.rad_scale, j.wq, j.wscale, j.wsum, j.atlas, j.nprobes, j.probe_base, j.bake,
+                   b - j.block_begin, part);
+}
+
+int conv_job_blocks(int num_probes) { return (num_probes + kConvProbes - 1) / kConvProbes; }
+
+void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, hipStream_t s)
+{
+  if (njobs <= 0 || total_blocks <= 0) return;
+  const dim3 grid((unsigned)total_blocks), block(64 * kConvWaves);
+  if (precision == ANE_PRECISION_FP32)
+    hipLaunchKernelGGL(k_conv_jobs<ANE_PRECISION_FP32>, grid, block, 0, s, jobs, njobs);
+  else if (precision == ANE_PRECISION_FP16)
+    hipLaunchKernelGGL(k_conv_jobs<ANE_PRECISION_FP16>, grid, block, 0, s, jobs, njobs);
+  else if (precision == ANE_PRECISION_BF16)
+    hipLaunchKernelGGL(k_conv_jobs<ANE_PRECISION_BF16>, grid, block, 0, s, jobs, njobs);
+  else
+    hipLaunchKernelGGL(k_conv_jobs<ANE_PRECISION_FP8>, grid, block, 0, s, jobs, njobs);
 }
 
 void launch_convolve(const GridK& g, const FrameK& f, int precision, const void* rad,
@@ -319,7 +402,7 @@ void launch_convolve(const GridK& g, const FrameK& f, int precision, const void*
 __global__ __launch_bounds__(256) void k_quantize(int prec, const float* src, void* dst,
                                                   float* scale, int n)
 {
-  __shared__ float red[256];
+  __shared__ float red[3 * 4096 / 32];  // block factors for up to 4096 rays
   const int p = blockIdx.x;
   const float* s = src + (int64_t)p * 3 * n;
   if (prec == ANE_PRECISION_BF16 || prec == ANE_PRECISION_FP16) {
@@ -330,18 +413,19 @@ __global__ __launch_bounds__(256) void k_quantize(int prec, const float* src, vo
       for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_f16_sat(s[i]);
     return;
   }
-  float mx = 0.0f;
-  for (int i = threadIdx.x; i < 3 * n; i += 256) mx = fmaxf(mx, fabsf(s[i]));
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
-    __syncthreads();
+  // one scale per channel and 32 rays (n % 32 == 0), as the shading stores them
+  float* se = red;  // [3 n / 32] quantization factors
+  const int nb = 3 * (n / 32);
+  float* sco = scale + (int64_t)p * nb;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    float m = 0.0f;
+    for (int k = 0; k < 32; ++k) m = fmaxf(m, fabsf(s[32 * b + k]));
+    se[b] = fp8_block_inv(m);
+    sco[b] = fp8_block_scale(m);
   }
-  const float sc = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
+  __syncthreads();
   uint8_t* d = (uint8_t*)dst + (int64_t)p * 3 * n;
-  for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_e4m3(s[i] / sc);
-  if (threadIdx.x == 0) scale[p] = sc;
+  for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f32_to_e4m3(s[i] * se[i >> 5]);
 }
 
 __global__ __launch_bounds__(256) void k_dequantize(int prec, const void* src, const float* scale,
@@ -357,8 +441,8 @@ __global__ __launch_bounds__(256) void k_dequantize(int prec, const void* src, c
     for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = f16_to_f32(s[i]);
   } else {
     const uint8_t* s = (const uint8_t*)src + (int64_t)p * 3 * n;
-    const float sc = scale[p];
-    for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = e4m3_to_f32(s[i]) * sc;
+    const float* se = scale + (int64_t)p * 3 * (n / 32);
+    for (int i = threadIdx.x; i < 3 * n; i += 256) d[i] = e4m3_to_f32(s[i]) * se[i >> 5];
   }
 }
 

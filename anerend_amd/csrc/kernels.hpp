@@ -131,6 +131,39 @@ struct TraceArgs {
   const uint32_t* skip_if_set; // caller rays: the launch does nothing when *skip_if_set != 0
 };
 
+// A probe update's shading and convolution as job records, so that the request
+// loop can run the probes of many updates in one launch per stage.  block_begin:
+// the job's first block in the fused launch (jobs sorted by it).
+struct ShadeJob {
+  SceneK sc;
+  GridK g;
+  FrameK f;
+  const float4* probe_org;
+  const ane_hit* hits;
+  const uint8_t* shadow;
+  const float4* atlas;
+  void* rad;
+  float* rad_scale;
+  int32_t probe_base;
+  int32_t block_begin;  // one block per probe
+};
+struct ConvJob {
+  GridK g;
+  const void* rad;
+  const float* rad_scale;
+  const void* wq;       // weights in the job's precision
+  const float* wscale;
+  const float* wsum;
+  float4* atlas;
+  int32_t nprobes, probe_base, bake;
+  int32_t block_begin;  // conv_job_blocks(nprobes) blocks
+};
+// max_rays: the largest ray count among the jobs (FP8 stages a probe in LDS)
+void launch_shade_jobs(int precision, const ShadeJob* jobs, int njobs, int total_blocks, int max_rays,
+                       hipStream_t s);
+int conv_job_blocks(int num_probes);
+void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, hipStream_t s);
+
 int trace_lanes_per_launch();
 int trace_stack_lds();
 // rays = false: probe rays (closest hit, then the shadow ray of every hit);
@@ -227,6 +260,12 @@ __device__ __forceinline__ uint16_t f32_to_f16_sat(float x)
 }
 
 __device__ __forceinline__ float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+// FP8 radiance block scales: one FP32 scale per probe, channel and 32 consecutive
+// rays, block maximum / 448, so the largest value of a block (and every value of
+// a constant block, such as rays that all see the sky) is stored exactly.
+__device__ __forceinline__ float fp8_block_scale(float mx) { return mx > 0.0f ? mx / 448.0f : 1.0f; }
+__device__ __forceinline__ float fp8_block_inv(float mx) { return mx > 0.0f ? 448.0f / mx : 1.0f; }
 
 // OCP FP8 E4M3 (e4m3fn: bias 7, max 448, no infinities), round to nearest even,
 // saturating at 448.

@@ -26,9 +26,13 @@
 #include <vector>
 
 #include "../../include/ane_serve.h"
+#include "kernels.hpp"
 
 namespace ane {
 int fail(int status, const char* fmt, ...);
+int ddgi_update_front(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st,
+                      ShadeJob* sj, ConvJob* cj, int* precision);
+bool ddgi_fusable(const ane_ddgi_ctx* c);
 int ddgi_update_graph(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame* f, hipStream_t st);
 int ddgi_prepare(ane_ddgi_ctx* c, const ane_scene* s, int64_t* bytes);
 int64_t ddgi_device_bytes(const ane_ddgi_ctx* c);
@@ -62,6 +66,7 @@ struct Request {
   hipEvent_t done = nullptr;
   bool launched = false, finished = false;
   int64_t rays = 0;
+  int slot = -1;  // fused stages: the job-table slot of the request's batch
 };
 
 }  // namespace
@@ -92,10 +97,108 @@ struct ane_serve {
   uint64_t completed = 0, failed = 0;
   int64_t rays_done = 0;
   int32_t peak_inflight = 0;
+  // fused stages: job tables of the batches on the device, one slot per batch
+  // (pinned host staging + device copy), reused once every request of its batch
+  // has finished
+  int slot_jobs = 0;                 // jobs per slot = the largest batch
+  std::vector<int> slot_refs;        // unfinished requests per slot
+  ane::ShadeJob* h_shade = nullptr;  // [slots][slot_jobs]
+  ane::ConvJob* h_conv = nullptr;
+  ane::ShadeJob* d_shade = nullptr;
+  ane::ConvJob* d_conv = nullptr;
+  std::vector<hipEvent_t> front_ev;  // fused stages: end of each request's front on its stream
 
   void run();
   int retire_locked(double t);
+  struct Launch {
+    uint64_t ticket;
+    ane_ddgi_ctx* ctx;
+    const ane_scene* scene;
+    ane_ddgi_frame frame;
+    hipStream_t stream;
+    hipEvent_t done;
+  };
+  void launch_fused(std::vector<Launch>& batch, int slot, std::vector<std::pair<uint64_t, int>>& launched);
 };
+
+// One batch: each update's front (setup, sun grid, trace) on the server's streams in
+// turn, starting with the batch's stream, so that the traces overlap; the batch's
+// stream waits for them, then shades the probes of every update in one launch and
+// convolves them in one launch per precision, from job tables copied to the
+// batch's slot.
+void ane_serve::launch_fused(std::vector<Launch>& batch, int slot,
+                             std::vector<std::pair<uint64_t, int>>& launched)
+{
+  hipStream_t st = batch[0].stream;
+  const int ns = (int)streams.size();
+  int si = 0;
+  while (si < ns && streams[si] != st) ++si;
+  while (front_ev.size() < batch.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) break;
+    front_ev.push_back(e);
+  }
+  const bool fork = ns > 1 && front_ev.size() >= batch.size();
+  ane::ShadeJob* hs = h_shade + (size_t)slot * slot_jobs;
+  ane::ConvJob* hc = h_conv + (size_t)slot * slot_jobs;
+  ane::ShadeJob* ds = d_shade + (size_t)slot * slot_jobs;
+  ane::ConvJob* dc = d_conv + (size_t)slot * slot_jobs;
+  std::vector<int> status(batch.size(), ANE_OK), prec(batch.size(), -1);
+  std::vector<ane::ShadeJob> sj(batch.size());
+  std::vector<ane::ConvJob> cj(batch.size());
+  for (size_t i = 0; i < batch.size(); ++i) {
+    Launch& b = batch[i];
+    hipStream_t fs = fork ? streams[(si + (int)i) % ns] : st;
+    if (!ane::ddgi_fusable(b.ctx)) {
+      status[i] = ane_ddgi_update(b.ctx, b.scene, &b.frame, fs);  // launched on its own
+    } else {
+      status[i] = ane::ddgi_update_front(b.ctx, b.scene, &b.frame, fs, &sj[i], &cj[i], &prec[i]);
+      if (status[i] != ANE_OK) prec[i] = -1;
+    }
+    // the batch's stream (the fused stages, every request's completion event) waits for the front
+    if (fs != st && (hipEventRecord(front_ev[i], fs) != hipSuccess || hipStreamWaitEvent(st, front_ev[i], 0) != hipSuccess)) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(fs);
+    }
+  }
+  // job tables grouped by precision, each group contiguous in the slot
+  struct Group {
+    int first = 0, count = 0, shade_blocks = 0, conv_blocks = 0, max_rays = 0;
+  } grp[4];
+  int k = 0;
+  for (int p = 0; p < 4; ++p) {
+    grp[p].first = k;
+    for (size_t i = 0; i < batch.size(); ++i) {
+      if (prec[i] != p) continue;
+      sj[i].block_begin = grp[p].shade_blocks;
+      cj[i].block_begin = grp[p].conv_blocks;
+      grp[p].shade_blocks += cj[i].nprobes;
+      grp[p].conv_blocks += ane::conv_job_blocks(cj[i].nprobes);
+      grp[p].max_rays = std::max(grp[p].max_rays, (int)sj[i].g.rays);
+      hs[k] = sj[i];
+      hc[k] = cj[i];
+      ++k;
+      ++grp[p].count;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (k > 0) {
+    if ((e = hipMemcpyAsync(ds, hs, sizeof(ane::ShadeJob) * k, hipMemcpyHostToDevice, st)) == hipSuccess)
+      e = hipMemcpyAsync(dc, hc, sizeof(ane::ConvJob) * k, hipMemcpyHostToDevice, st);
+    for (int p = 0; p < 4 && e == hipSuccess; ++p)
+      ane::launch_shade_jobs(p, ds + grp[p].first, grp[p].count, grp[p].shade_blocks, grp[p].max_rays, st);
+    for (int p = 0; p < 4 && e == hipSuccess; ++p)
+      ane::launch_conv_jobs(p, dc + grp[p].first, grp[p].count, grp[p].conv_blocks, st);
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  for (size_t i = 0; i < batch.size(); ++i) {
+    int sti = status[i];
+    if (sti == ANE_OK && prec[i] >= 0 && e != hipSuccess) sti = ANE_ERR_DEVICE;
+    if (sti == ANE_OK && hipEventRecord(batch[i].done, st) != hipSuccess) sti = ANE_ERR_DEVICE;
+    launched.push_back({batch[i].ticket, sti});
+  }
+  (void)slot;
+}
 
 int ane_serve::retire_locked(double t)
 {
@@ -119,6 +222,7 @@ int ane_serve::retire_locked(double t)
     event_pool.push_back(r.done);
     r.done = nullptr;
     stream_load[r.stream] -= 1;
+    if (r.slot >= 0) slot_refs[r.slot] -= 1;
     Session& s = sessions[r.session];
     s.busy = false;
     s.finished += 1;
@@ -133,20 +237,23 @@ void ane_serve::run()
 {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lk(mu);
-  struct Launch {
-    uint64_t ticket;
-    ane_ddgi_ctx* ctx;
-    const ane_scene* scene;
-    ane_ddgi_frame frame;
-    hipStream_t stream;
-    hipEvent_t done;
-  };
   std::vector<Launch> batch;
   for (;;) {
     if (retire_locked(now_ms()) > 0) cv_done.notify_all();
     // admit: oldest first, one update per session on the device at a time
     batch.clear();
-    const int slots = cfg.max_inflight - (int)inflight.size();
+    int slots = cfg.max_inflight - (int)inflight.size();
+    // fused stages: a batch needs a free job-table slot and holds at most slot_jobs requests
+    int slot = -1;
+    if (cfg.fuse_stages) {
+      for (int k = 0; k < (int)slot_refs.size(); ++k)
+        if (slot_refs[k] == 0) { slot = k; break; }
+      if (slot < 0) slots = 0;
+      slots = std::min(slots, slot_jobs);
+    }
+    int batch_stream = 0;  // fused stages: the whole batch on the least-loaded stream
+    for (int k = 1; k < (int)streams.size(); ++k)
+      if (stream_load[k] < stream_load[batch_stream]) batch_stream = k;
     if (slots > 0 && !queue.empty()) {
       std::vector<int32_t> taken;
       for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < slots;) {
@@ -156,9 +263,10 @@ void ane_serve::run()
           ++it;
           continue;
         }
-        int best = 0;
-        for (int k = 1; k < (int)streams.size(); ++k)
-          if (stream_load[k] < stream_load[best]) best = k;
+        int best = batch_stream;
+        if (!cfg.fuse_stages)
+          for (int k = 0; k < (int)streams.size(); ++k)
+            if (stream_load[k] < stream_load[best]) best = k;
         if (event_pool.empty()) {
           hipEvent_t e = nullptr;
           if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) break;
@@ -167,6 +275,8 @@ void ane_serve::run()
         r.done = event_pool.back();
         event_pool.pop_back();
         r.stream = best;
+        r.slot = cfg.fuse_stages ? slot : -1;
+        if (r.slot >= 0) slot_refs[r.slot] += 1;
         stream_load[best] += 1;
         s.busy = true;
         taken.push_back(r.session);
@@ -179,11 +289,15 @@ void ane_serve::run()
       lk.unlock();
       std::vector<std::pair<uint64_t, int>> launched;
       launched.reserve(batch.size());
-      for (Launch& b : batch) {
-        int st = cfg.capture_graphs ? ane::ddgi_update_graph(b.ctx, b.scene, &b.frame, b.stream)
-                                    : ane_ddgi_update(b.ctx, b.scene, &b.frame, b.stream);
-        if (st == ANE_OK && hipEventRecord(b.done, b.stream) != hipSuccess) st = ANE_ERR_DEVICE;
-        launched.push_back({b.ticket, st});
+      if (cfg.fuse_stages) {
+        launch_fused(batch, slot, launched);
+      } else {
+        for (Launch& b : batch) {
+          int st = cfg.capture_graphs ? ane::ddgi_update_graph(b.ctx, b.scene, &b.frame, b.stream)
+                                      : ane_ddgi_update(b.ctx, b.scene, &b.frame, b.stream);
+          if (st == ANE_OK && hipEventRecord(b.done, b.stream) != hipSuccess) st = ANE_ERR_DEVICE;
+          launched.push_back({b.ticket, st});
+        }
       }
       lk.lock();
       const double t = now_ms();
@@ -202,6 +316,7 @@ void ane_serve::run()
           event_pool.push_back(r.done);
           r.done = nullptr;
           stream_load[r.stream] -= 1;
+          if (r.slot >= 0) slot_refs[r.slot] -= 1;
           Session& s = sessions[r.session];
           s.busy = false;
           s.finished += 1;
@@ -231,12 +346,30 @@ extern "C" int ane_serve_open(const ane_serve_config* cfg, ane_serve** out)
     return fail(ANE_ERR_INVALID_ARG, "max_inflight must be in 1..256");
   if (cfg->capture_graphs != 0 && cfg->capture_graphs != 1)
     return fail(ANE_ERR_INVALID_ARG, "capture_graphs must be 0 or 1");
-  if (cfg->reserved != 0) return fail(ANE_ERR_INVALID_ARG, "reserved must be 0");
+  if (cfg->fuse_stages != 0 && cfg->fuse_stages != 1)
+    return fail(ANE_ERR_INVALID_ARG, "fuse_stages must be 0 or 1");
   if (cfg->num_streams < 1 || cfg->num_streams > 16)
     return fail(ANE_ERR_INVALID_ARG, "num_streams must be in 1..16");
   ane_serve* s = new ane_serve();
   s->cfg = *cfg;
   (void)hipGetDevice(&s->device);
+  if (cfg->fuse_stages) {
+    s->slot_jobs = std::min(cfg->max_inflight, 64);
+    const size_t n = (size_t)cfg->max_inflight * s->slot_jobs;
+    if (hipHostMalloc((void**)&s->h_shade, sizeof(ane::ShadeJob) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_conv, sizeof(ane::ConvJob) * n, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&s->d_shade, sizeof(ane::ShadeJob) * n) != hipSuccess ||
+        hipMalloc((void**)&s->d_conv, sizeof(ane::ConvJob) * n) != hipSuccess) {
+      (void)hipGetLastError();
+      if (s->h_shade) (void)hipHostFree(s->h_shade);
+      if (s->h_conv) (void)hipHostFree(s->h_conv);
+      if (s->d_shade) (void)hipFree(s->d_shade);
+      if (s->d_conv) (void)hipFree(s->d_conv);
+      delete s;
+      return fail(ANE_ERR_OUT_OF_MEMORY, "job tables of the fused stages");
+    }
+    s->slot_refs.assign(cfg->max_inflight, 0);
+  }
   for (int k = 0; k < cfg->num_streams; ++k) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
@@ -430,9 +563,14 @@ extern "C" int ane_serve_close(ane_serve* s)
   if (s->worker.joinable()) s->worker.join();  // drains the queue and the device work
   for (auto& kv : s->sessions) ane_ddgi_destroy(kv.second.ctx);
   for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->front_ev) (void)hipEventDestroy(e);
   for (auto& kv : s->reqs)
     if (kv.second.done) (void)hipEventDestroy(kv.second.done);
   for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+  if (s->h_shade) (void)hipHostFree(s->h_shade);
+  if (s->h_conv) (void)hipHostFree(s->h_conv);
+  if (s->d_shade) (void)hipFree(s->d_shade);
+  if (s->d_conv) (void)hipFree(s->d_conv);
   delete s;
   return ANE_OK;
 }

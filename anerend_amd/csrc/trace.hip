@@ -18,7 +18,7 @@
 //   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
 //                    indirect light from the previous atlas, emissive, sky on miss
 //                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
-//                    context precision (FP8: per-probe scale from the block maximum)
+//                    context precision (FP8: a scale per probe, channel and 32 rays)
 //   k_gi_lookup      the probe term of deferred_tiled.comp:264-298 per G-buffer texel
 //   k_shadow_*       RT hard shadows of a depth buffer (shadow.rgen), through the
 //                    scene's sun grid (k_shadow_mask_grid) or caller-ray traversal
@@ -921,18 +921,17 @@ __device__ f3 shade_ray(const SceneK& sc, const GridK& g, const FrameK& f, const
   return add(acc, emissive);
 }
 
+// One probe's rays (block-level): shading of every ray, stored in the context
+// precision.  The block's shared arrays are passed in (sL: FP8 staging [3][N]).
 template <int PREC, int TB>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
-                                              const ane_hit* hits, const uint8_t* shadow, const float4* atlas, void* rad,
-                                              float* rad_scale, int probe_base)
+__device__ __forceinline__ void shade_probe(const SceneK& sc, const GridK& g, const FrameK& f,
+                                            const float4* probe_org, const ane_hit* hits,
+                                            const uint8_t* shadow, const float4* atlas, void* rad,
+                                            float* rad_scale, int probe_base, int ql, float* sL, float* red,
+                                            uint16_t* lst, uint32_t* cnt, uint32_t* cur)
 {
-  extern __shared__ float sL[];  // FP8: [3][N] staging for the per-probe scale
-  __shared__ float red[TB];
   // the probe's rays grouped as lit hits, shadowed hits, misses, so that a wave
   // runs one kind of work (results are stored per ray, the order is immaterial)
-  __shared__ uint16_t lst[4096];
-  __shared__ uint32_t cnt[3], cur[3];
-  const int ql = blockIdx.x;
   const int q = probe_base + ql;
   const int n = g.rays;
   const float4 po = probe_org[q];
@@ -987,17 +986,74 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WA
     }
   }
   if (PREC == ANE_PRECISION_FP8) {
-    red[threadIdx.x] = mx;
+    // one scale per channel and 32 rays (n % 32 == 0): sL[c n + r] is in block
+    // (c n + r) / 32 = c (n / 32) + r / 32
+    float* se = sL + 3 * n;  // [3 n / 32] quantization factors, after the staged radiance
+    float* sco = rad_scale + (int64_t)ql * 3 * (n / 32);
     __syncthreads();
-    for (int k = TB / 2; k > 0; k >>= 1) {
-      if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
-      __syncthreads();
+    for (int b = threadIdx.x; b < 3 * (n / 32); b += TB) {
+      float m = 0.0f;
+      for (int k = 0; k < 32; ++k) m = fmaxf(m, fabsf(sL[32 * b + k]));
+      se[b] = fp8_block_inv(m);
+      sco[b] = fp8_block_scale(m);
     }
-    const float sc8 = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
+    __syncthreads();
     uint8_t* o = (uint8_t*)rad + (int64_t)ql * 3 * n;
-    for (int i = threadIdx.x; i < 3 * n; i += TB) o[i] = f32_to_e4m3(sL[i] / sc8);
-    if (threadIdx.x == 0) rad_scale[ql] = sc8;
+    for (int i = threadIdx.x; i < 3 * n; i += TB) o[i] = f32_to_e4m3(sL[i] * se[i >> 5]);
+    (void)mx;
   }
+}
+
+template <int PREC, int TB>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_shade(SceneK sc, GridK g, FrameK f, const float4* probe_org,
+                                              const ane_hit* hits, const uint8_t* shadow, const float4* atlas, void* rad,
+                                              float* rad_scale, int probe_base)
+{
+  extern __shared__ float sL[];  // FP8: [3][N] staged radiance, then [3][N / 32] block factors
+  __shared__ float red[TB];
+  __shared__ uint16_t lst[4096];
+  __shared__ uint32_t cnt[3], cur[3];
+  shade_probe<PREC, TB>(sc, g, f, probe_org, hits, shadow, atlas, rad, rad_scale, probe_base, blockIdx.x, sL,
+                        red, lst, cnt, cur);
+}
+
+// The probes of many updates in one launch (the request loop's fused stage): block
+// b shades probe b - block_begin of the last job whose block_begin <= b.
+// dynamic LDS of the FP8 shading: staged radiance [3][N] and block factors [3][N / 32]
+static size_t fp8_shade_smem(int rays) { return sizeof(float) * (3 * (size_t)rays + 3 * (size_t)((rays + 31) / 32)); }
+
+template <int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANE_SHADE_WAVES))) void k_shade_jobs(const ShadeJob* jobs, int njobs)
+{
+  extern __shared__ float sL[];
+  __shared__ float red[256];
+  __shared__ uint16_t lst[4096];
+  __shared__ uint32_t cnt[3], cur[3];
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const ShadeJob& j = jobs[lo];
+  shade_probe<PREC, 256>(j.sc, j.g, j.f, j.probe_org, j.hits, j.shadow, j.atlas, j.rad, j.rad_scale, j.probe_base,
+                         b - j.block_begin, sL, red, lst, cnt, cur);
+}
+
+void launch_shade_jobs(int precision, const ShadeJob* jobs, int njobs, int total_blocks, int max_rays,
+                       hipStream_t s)
+{
+  if (njobs <= 0 || total_blocks <= 0) return;
+  const dim3 grid((unsigned)total_blocks), block(256);
+  if (precision == ANE_PRECISION_FP32)
+    hipLaunchKernelGGL(k_shade_jobs<ANE_PRECISION_FP32>, grid, block, 0, s, jobs, njobs);
+  else if (precision == ANE_PRECISION_FP16)
+    hipLaunchKernelGGL(k_shade_jobs<ANE_PRECISION_FP16>, grid, block, 0, s, jobs, njobs);
+  else if (precision == ANE_PRECISION_BF16)
+    hipLaunchKernelGGL(k_shade_jobs<ANE_PRECISION_BF16>, grid, block, 0, s, jobs, njobs);
+  else
+    hipLaunchKernelGGL(k_shade_jobs<ANE_PRECISION_FP8>, grid, block, fp8_shade_smem(max_rays), s, jobs, njobs);
 }
 
 template <int TB>
@@ -1018,7 +1074,7 @@ static void shade_launch(const SceneK& sc, const GridK& g, const FrameK& f, cons
                        probe_org, hits, shadow, a4, rad, rad_scale, probe_base);
   else
     hipLaunchKernelGGL((k_shade<ANE_PRECISION_FP8, TB>), dim3(num_probes), dim3(TB),
-                       (size_t)12 * g.rays, s, sc, g, f, probe_org, hits, shadow, a4, rad,
+                       fp8_shade_smem(g.rays), s, sc, g, f, probe_org, hits, shadow, a4, rad,
                        rad_scale, probe_base);
 }
 

@@ -15,7 +15,7 @@ class ServeConfig(ctypes.Structure):
                 ("max_inflight", ctypes.c_int32),
                 ("num_streams", ctypes.c_int32),
                 ("capture_graphs", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("fuse_stages", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -42,9 +42,12 @@ assert ctypes.sizeof(ServeResult) == 32
 class Server:
     """Owns the request loop; sessions reference DeviceScenes the caller keeps alive."""
 
-    def __init__(self, hbm_budget_bytes, max_inflight=8, num_streams=4, capture_graphs=True):
+    def __init__(self, hbm_budget_bytes, max_inflight=8, num_streams=4, capture_graphs=True, fuse_stages=False):
+        """fuse_stages: the requests launched together share one shading launch and one
+        convolution launch (capture_graphs then does not apply)."""
         L = native.load()
-        c = ServeConfig(int(hbm_budget_bytes), int(max_inflight), int(num_streams), int(bool(capture_graphs)), 0)
+        c = ServeConfig(int(hbm_budget_bytes), int(max_inflight), int(num_streams), int(bool(capture_graphs)),
+                        int(bool(fuse_stages)))
         self._h = ctypes.c_void_p()
         native.check(L.ane_serve_open(ctypes.byref(c), ctypes.byref(self._h)))
         self.configs = {}

@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--inflight", type=int, default=8, help="serve64: requests on the device at once")
     p.add_argument("--no-graphs", action="store_true", help="serve64: direct launches instead of captured graphs")
     p.add_argument("--streams", type=int, default=4, help="serve64: HIP streams of the server")
+    p.add_argument("--no-fuse", action="store_true",
+                   help="serve64: launch every request on its own instead of fusing the per-probe stages")
     p.add_argument("--frames", type=int, default=300, help="frames of the dynamic4m workload")
     p.add_argument("--rebuild-every", type=int, default=4,
                    help="dynamic4m: full LBVH rebuild every K frames, refit in between (0 = refit only)")
@@ -545,7 +547,7 @@ def bench_dynamic(args):
     print(json.dumps(out), flush=True)
 
 
-def run_serve(requests, burst, inflight, streams, graphs):
+def run_serve(requests, burst, inflight, streams, graphs, fuse):
     """Serving: 64 resident scenes of uneven sizes (10k-2M triangles, grids 4^3 to
     32x16x32, 128-512 rays), BF16 convolution, requests arriving as a Poisson
     process at 200/s (open loop), admitted continuously under a 200 GB budget.
@@ -563,7 +565,7 @@ def run_serve(requests, burst, inflight, streams, graphs):
     cfgs = [scenegen.serving_config(sp, PRECISION_BF16) for sp in specs]
     budget = 200 << 30
     srv = serving.Server(hbm_budget_bytes=budget, max_inflight=inflight, num_streams=streams,
-                         capture_graphs=graphs)
+                         capture_graphs=graphs and not fuse, fuse_stages=fuse)
     sids = [srv.add_session(d, c) for d, c in zip(dss, cfgs)]
     admitted = srv.stats()
     log(f"[bench] {tris} triangles resident, {admitted['used_bytes'] / 2**30:.2f} GiB admitted")
@@ -636,12 +638,12 @@ def run_serve(requests, burst, inflight, streams, graphs):
     return {"open_loop": open_loop, "burst": burst_d, "triangles": tris, "scenes": len(specs),
             "hbm_budget_bytes": budget, "admitted_bytes": admitted["used_bytes"],
             "held_bytes_after": held["held_bytes"], "max_inflight": inflight, "streams": streams,
-            "capture_graphs": graphs}
+            "capture_graphs": graphs and not fuse, "fused_stages": fuse}
 
 
 def bench_serve(args):
     """--workload serve64: the request loop, one JSON line (p99 latency)."""
-    d = run_serve(args.requests, args.burst, args.inflight, args.streams, not args.no_graphs)
+    d = run_serve(args.requests, args.burst, args.inflight, args.streams, not args.no_graphs, not args.no_fuse)
     ol = d["open_loop"]
     out = {
         "metric": "serving p99 latency (ms), Poisson 200 req/s over 64 resident scenes",
@@ -652,7 +654,7 @@ def bench_serve(args):
                                "{4^3, 8^3, 16x8x16, 32x16x32} x rays {128, 256, 512}, BF16 conv",
                    "triangles": d["triangles"], "hbm_budget_bytes": d["hbm_budget_bytes"],
                    "max_inflight": d["max_inflight"], "streams": d["streams"],
-                   "capture_graphs": d["capture_graphs"]},
+                   "capture_graphs": d["capture_graphs"], "fused_stages": d["fused_stages"]},
         "open_loop": ol,
         "burst": d["burst"],
         "admitted_bytes": d["admitted_bytes"], "held_bytes_after": d["held_bytes_after"],
@@ -837,7 +839,8 @@ def main():
         dyn = run_dynamic(args.leg_frames, 3, args.rebuild_every, args.node_format, False)
     if "serve" in legs:
         log(f"[bench] serving leg, {args.leg_requests} requests")
-        serve = run_serve(args.leg_requests, args.burst, args.inflight, args.streams, not args.no_graphs)
+        serve = run_serve(args.leg_requests, args.burst, args.inflight, args.streams, not args.no_graphs,
+                          not args.no_fuse)
 
     out = {
         "metric": "probe rays/s (16k probes x 512 rays DDGI update)",

@@ -65,7 +65,8 @@ typedef enum ane_status {
 typedef enum ane_precision {
   ANE_PRECISION_FP32 = 0, /* plain path: FP32 radiance, FP32 convolution */
   ANE_PRECISION_BF16 = 1, /* BF16 radiance and weights, MFMA, FP32 accumulate */
-  ANE_PRECISION_FP8 = 2,  /* FP8-e4m3 radiance (per-probe scale) and weights, MFMA, FP32 accumulate */
+  ANE_PRECISION_FP8 = 2,  /* FP8-e4m3 radiance (FP32 scale per probe, channel and 32 rays) and weights,
+                              MFMA, FP32 accumulate */
   ANE_PRECISION_FP16 = 3  /* FP16 radiance (saturating at 65504) and weights, MFMA, FP32 accumulate:
                              the 16-bit mode whose 10-bit mantissa keeps correlated rounding (runs
                              of nearly equal radiance: sky, lit planes) within 2e-3 of FP64 */

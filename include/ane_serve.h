@@ -16,6 +16,10 @@
  *    are in flight, on the least-loaded of num_streams HIP streams, so work of
  *    different sessions overlaps on the device and a request that arrives while
  *    others run starts as soon as a slot frees;
+ *  - with fuse_stages the requests launched together run as one batch: their
+ *    shading stages are one launch and their convolutions one launch (the
+ *    per-probe stages of all of them, addressed through a job table), after
+ *    each request's own trace (the traces spread over the streams);
  *  - a session's updates run in submission order and its results depend only
  *    on its own frames: a request gives the same atlas alone and in a mix.
  *
@@ -36,9 +40,14 @@ typedef struct ane_serve_config {
   int64_t hbm_budget_bytes; /* admission budget for resident sessions */
   int32_t max_inflight;     /* requests on the device at once, 1..256 */
   int32_t num_streams;      /* HIP streams the requests are spread over, 1..16 */
-  int32_t capture_graphs;   /* 1: each update is captured into the session's HIP graph
-                               (updated in place, one launch per request); 0: direct launches */
-  int32_t reserved;         /* 0 */
+  int32_t capture_graphs;   /* fuse_stages 0 only. 1: each update is captured into the session's
+                               HIP graph (updated in place, one launch per request); 0: direct launches */
+  int32_t fuse_stages;      /* 1: the requests launched together form a batch: each request's frame
+                               setup and trace, on the streams in turn (so the traces overlap), then
+                               on the batch's stream ONE launch shading the probes of every request
+                               in the batch and ONE launch convolving them (per precision present),
+                               through a request-descriptor table; 0: each request is launched on
+                               its own (graph or direct) */
 } ane_serve_config;
 
 typedef struct ane_serve_result {

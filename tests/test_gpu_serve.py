@@ -27,19 +27,23 @@ def _sessions():
     return specs
 
 
-@pytest.mark.parametrize("graphs,nodes", [(True, "full"), (False, "full"), (True, "quantized")])
-def test_mixed_equals_alone(graphs, nodes):
+@pytest.mark.parametrize("graphs,nodes,fused", [(True, "full", False), (False, "full", False),
+                                               (True, "quantized", False), (False, "full", True),
+                                               (False, "quantized", True)])
+def test_mixed_equals_alone(graphs, nodes, fused):
     """With captured graphs every session's graph is replayed (updated in place)
     once per request, with the eager per-scene updates of the other sessions and
     host synchronizations in between; each replay feeds the next update through
     the atlas, so the final atlas checks them all.  The quantized case runs every
-    scene on 32-byte nodes."""
+    scene on 32-byte nodes.  Fused: the requests launched together share one
+    shading and one convolution launch per precision (FP32, BF16 here)."""
     specs = _sessions()
     scenes = [native.DeviceScene(s) for s, _, _ in specs]
     if nodes == "quantized":
         for ds in scenes:
             ds.set_node_format(native.DeviceScene.NODES_QUANTIZED)
-    srv = serving.Server(hbm_budget_bytes=64 << 30, max_inflight=4, num_streams=3, capture_graphs=graphs)
+    srv = serving.Server(hbm_budget_bytes=64 << 30, max_inflight=4, num_streams=3, capture_graphs=graphs,
+                         fuse_stages=fused)
     ids = [srv.add_session(ds, cfg) for ds, (_, cfg, _) in zip(scenes, specs)]
     # interleaved requests: 3 full updates per session, submitted back to back
     frames = {}
@@ -50,7 +54,8 @@ def test_mixed_equals_alone(graphs, nodes):
             fr = scenegen.make_frame(11 + k, camera=cam, config=cfg, iteration=it)
             frames.setdefault(k, []).append(fr)
             tickets.append(srv.submit(sid, fr))
-    print(f"serve (graphs {graphs}, {nodes} nodes): {len(tickets)} requests submitted", flush=True)
+    print(f"serve (graphs {graphs}, {nodes} nodes, fused {fused}): {len(tickets)} requests submitted",
+          flush=True)
     res = [srv.poll(t, timeout_ms=-1) for t in tickets]
     assert all(r["status"] == 0 for r in res)
     st = srv.stats()
@@ -98,7 +103,8 @@ def test_admission_by_budget_and_errors():
     srv.close()
 
 
-def test_serving_c3_scenes_mixed_equals_alone():
+@pytest.mark.parametrize("fused", [True, False])
+def test_serving_c3_scenes_mixed_equals_alone(fused):
     """The serving workload itself: the 64 seeded scenes (10k-2M triangles, grids
     4^3 .. 32x16x32, 128-512 rays, BF16 convolution) resident under a 200 GB
     budget, 192 requests drawn uniformly over the sessions and submitted in
@@ -112,7 +118,8 @@ def test_serving_c3_scenes_mixed_equals_alone():
         dss.append(native.DeviceScene(scenegen.serving_scene(4, k, sp)))
     print(f"[c3] {len(dss)} scenes, {sum(d.info()['num_triangles'] for d in dss)} triangles", flush=True)
     cfgs = [scenegen.serving_config(sp, PRECISION_BF16) for sp in specs]
-    srv = serving.Server(hbm_budget_bytes=200 << 30, max_inflight=8, num_streams=4, capture_graphs=True)
+    srv = serving.Server(hbm_budget_bytes=200 << 30, max_inflight=8, num_streams=4, capture_graphs=not fused,
+                         fuse_stages=fused)
     sids = [srv.add_session(d, c) for d, c in zip(dss, cfgs)]
     admitted = srv.stats()
     assert admitted["used_bytes"] == admitted["held_bytes"]
@@ -133,7 +140,7 @@ def test_serving_c3_scenes_mixed_equals_alone():
     res = [srv.poll(t, timeout_ms=-1) for t in tickets]
     lat = np.array([r["latency_ms"] for r in res])
     st = srv.stats()
-    print(f"[c3] {R} requests p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms, "
+    print(f"[c3] fused {fused}: {R} requests p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms, "
           f"peak in flight {st['peak_inflight']}", flush=True)
     assert all(r["status"] == 0 for r in res) and st["completed"] == R + 0 and st["failed"] == 0
     assert st["used_bytes"] == st["held_bytes"] == admitted["used_bytes"]
