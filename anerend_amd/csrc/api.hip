@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <algorithm>
 #include <utility>
@@ -2038,6 +2039,99 @@ extern "C" int ane_scene_trace_rays(const ane_scene* s, const float* origins, co
 // ===========================================================================
 // library
 // ===========================================================================
+// ===========================================================================
+// Allocation-free ray query (ane_scene_trace_rays_noalloc).  The traversal's
+// scratch (ray counter, work counters, fault flag, stack spill) lives in one
+// file-scope device array per GPU, so the call allocates nothing.  Host code
+// never uses the array's name as an address: its device address comes from
+// hipGetSymbolAddress, once per device with that device current.
+// ===========================================================================
+static constexpr int64_t kScratchBytes = (int64_t)16 << 20;  // per GPU, held until the process ends
+static constexpr int64_t kScratchSpillWord = 64;              // spill area starts at byte 256
+static __device__ __align__(256) uint32_t scratch[kScratchBytes / 4];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_for_current_device(uint32_t** out)
+{
+  int dev = 0;
+  ANE_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ANE_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(ANE_ERR_DEVICE, "hipGetSymbolAddress of the traversal scratch failed: %s",
+                  hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return ANE_OK;
+}
+
+extern "C" int ane_scene_trace_rays_noalloc(const ane_scene* s, const float* origins, const float* dirs,
+                                            int64_t num_rays, float t_min, float t_max, int32_t any_hit,
+                                            ane_hit* hits, uint64_t* counters2, void* stream)
+{
+  if (!s || !origins || !dirs || !hits) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
+  if (num_rays <= 0) return ANE_OK;
+  if (num_rays >= ((int64_t)1 << 31)) return fail(ANE_ERR_INVALID_ARG, "too many rays");
+  if (!(t_min >= 0.0f) || !(t_max > t_min)) return fail(ANE_ERR_INVALID_ARG, "need 0 <= t_min < t_max");
+  uint32_t* sp = nullptr;
+  ANE_TRY(scratch_for_current_device(&sp));
+  // lanes of the persistent launch: all resident blocks, fewer if the hierarchy's
+  // depth needs more spilled stack than the scratch holds, at least one block per CU
+  const int64_t tb = trace_block_size();
+  const int64_t full = trace_lanes_per_launch();
+  const int64_t depth = s->depth - trace_stack_lds() + 1 > 0 ? s->depth - trace_stack_lds() + 1 : 0;
+  const int64_t cap_words = kScratchBytes / 4 - kScratchSpillWord;
+  int64_t lanes = full;
+  if (depth > 0 && depth * lanes > cap_words) lanes = (cap_words / depth) / tb * tb;
+  int cus = 0, dev = 0;
+  ANE_CHECK_HIP(hipGetDevice(&dev));
+  ANE_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (lanes < (int64_t)cus * tb)
+    return fail(ANE_ERR_UNSUPPORTED, "hierarchy depth %d needs %lld bytes of traversal stack per %lld lanes, "
+                "more than the %lld-byte scratch", s->depth, (long long)(depth * 4), (long long)((int64_t)cus * tb),
+                (long long)kScratchBytes);
+  hipStream_t st = S(stream);
+  // scratch words 0 ray counter, 2-5 the two 64-bit work counters, 6 the fault flag:
+  // cleared by a kernel (the array keeps the previous call's contents)
+  fill_u32(sp, 0u, 8, st);
+  TraceArgs a{};
+  a.sc = s->view;
+  a.g.t_min = t_min;
+  a.g.t_max = t_max;
+  a.g.rays = 1;
+  a.hits = hits;
+  a.work = sp;
+  a.stats = reinterpret_cast<unsigned long long*>(sp + 2);
+  a.overflow = sp + 6;
+  a.spill = reinterpret_cast<int32_t*>(sp + kScratchSpillWord);
+  a.spill_depth = (int32_t)depth;
+  a.total_lanes = (int32_t)lanes;
+  a.nrays = (int32_t)num_rays;
+  a.ray_orig = origins;
+  a.ray_dir = dirs;
+  a.t_min = t_min;
+  a.t_max = t_max;
+  launch_trace(a, any_hit != 0, true, counters2 != nullptr, st);
+  ANE_CHECK_HIP(hipGetLastError());
+  uint32_t ov = 0;
+  unsigned long long c2[2] = {0, 0};
+  ANE_CHECK_HIP(hipMemcpyAsync(&ov, sp + 6, 4, hipMemcpyDeviceToHost, st));
+  if (counters2) ANE_CHECK_HIP(hipMemcpyAsync(c2, sp + 2, 16, hipMemcpyDeviceToHost, st));
+  ANE_CHECK_HIP(hipStreamSynchronize(st));
+  if (ov) return fail(ANE_ERR_DEVICE, "traversal stack exceeded its spill capacity");
+  if (counters2) { counters2[0] = c2[0]; counters2[1] = c2[1]; }
+  return ANE_OK;
+}
+
 extern "C" int ane_abi_version(void) { return ANE_DDGI_ABI_VERSION; }
 extern "C" const char* ane_last_error(void) { return g_err.c_str(); }
 

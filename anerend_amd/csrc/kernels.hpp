@@ -118,7 +118,7 @@ struct TraceArgs {
   uint32_t step_limit;    // node visits allowed per ray; 0: 2 * ntri + 8 (never reached by a valid tree)
   int32_t* spill;         // [spill_depth][total_lanes]
   int32_t spill_depth;
-  int32_t total_lanes;
+  int32_t total_lanes;     // lanes of the launch (set by launch_trace; a caller's value > 0 caps it)
   int32_t nrays;          // probes_in_range * N
   int32_t probe_base;     // first probe of the range
   float sun[3];           // normalize(-light_dir)
@@ -166,6 +166,7 @@ void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_b
 
 int trace_lanes_per_launch();
 int trace_stack_lds();
+int trace_block_size();
 // rays = false: probe rays (closest hit, then the shadow ray of every hit);
 // rays = true: caller rays, closest or any hit
 void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipStream_t s);

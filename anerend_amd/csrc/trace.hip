@@ -92,6 +92,7 @@ static constexpr int kShadeUnroll = ANE_SHADE_UNROLL;
 bool trace_grid_supported() { return kLeafBatchProbe > 0; }
 
 int trace_stack_lds() { return kSS; }
+int trace_block_size() { return kTB; }
 
 // Direction-only part of the ray setup (RayPre without the origin), 48 bytes:
 //   a = (d.x, d.y, d.z, inv.x)  b = (inv.y, inv.z, sx, sy)  c = (sz, kx | ky << 2 | kz << 4, 0, 0)
@@ -601,7 +602,11 @@ int trace_lanes_per_launch()
 
 void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipStream_t s)
 {
-  const int blocks = trace_lanes_per_launch() / kTB;
+  // a persistent grid of every resident block, or fewer when the caller caps the
+  // lanes (a.total_lanes > 0: the spill area is sized for that many)
+  const int full = trace_lanes_per_launch() / kTB;
+  const int want = a.total_lanes > 0 ? a.total_lanes / kTB : full;
+  const int blocks = want < 1 ? 1 : (want < full ? want : full);
   TraceArgs b = a;
   b.total_lanes = blocks * kTB;
 #define ANE_TR(S, R, C)                                                                             \

@@ -97,6 +97,7 @@ _SIGS = [
     ("ane_scene_set_node_format", ctypes.c_int, [_vp, _i32, _vp]),
     ("ane_scene_trace_rays", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
     ("ane_ddgi_ray_directions", ctypes.c_int, [_i32, _vp, _vp]),
+    ("ane_scene_trace_rays_noalloc", ctypes.c_int, [_vp, _vp, _vp, _i64, _f32, _f32, _i32, _vp, _vp, _vp]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGS]

@@ -388,6 +388,20 @@ int ane_scene_trace_rays(const ane_scene* scene, const float* origins, const flo
                          int64_t num_rays, float t_min, float t_max, int32_t any_hit,
                          ane_hit* hits, uint64_t* counters2, void* stream);
 
+/* ane_scene_trace_rays without any device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc; it synchronizes the stream, so it is
+ * not meant for graph capture).  origins, dirs
+ * ([num_rays][3] float) and hits are DEVICE arrays the caller owns; counters2 is
+ * a nullable host array.  The traversal's scratch (ray counter, counters, fault
+ * flag and spilled stack) is one 16 MiB device array per GPU, held until the
+ * process ends: calls that overlap on one GPU race on it.  Scenes whose hierarchy
+ * is too deep for that stack at one block per CU (depth above about 140) are
+ * refused with ANE_ERR_UNSUPPORTED.  Waits for `stream` before returning. */
+int ane_scene_trace_rays_noalloc(const ane_scene* scene, const float* origins, const float* dirs,
+                                 int64_t num_rays, float t_min, float t_max, int32_t any_hit,
+                                 ane_hit* hits, uint64_t* counters2, void* stream);
+
 /* RT hard shadows (ShadowRayTracingPass, shadow.rgen:29-82): for every pixel of a
  * width x height depth buffer, world position = (invViewProj * (uv * 2 - 1, depth, 1)).xyz / w
  * with uv = pixelToUv(pixel) (helpers.glsl:1-4), then an any-hit ray from it along
