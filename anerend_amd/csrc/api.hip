@@ -386,9 +386,16 @@ struct ane_scene {
   int32_t* d_order = nullptr;   // internal nodes bucketed by depth
   uint64_t* d_lvl_key = nullptr;
   uint32_t* d_hist = nullptr;
-  std::vector<int32_t> level_begin;
+  int4* d_topo = nullptr;       // children, parent and leaf-range end of every internal node
+  int32_t* d_cross = nullptr;   // top nodes (more leaves than a fit treelet holds), by depth
+  int2* d_units = nullptr;      // leaf ranges of the fit treelets
+  int32_t* d_stray = nullptr;   // leaves directly under a top node
+  uint32_t* d_uscratch = nullptr;
+  int32_t nunits = 0, nstray = 0;
+  std::vector<int32_t> level_begin, cross_begin;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t depth = 0;
+  bool world_stale = false;  // d_world predates the last refit (rebuilt from d_tris on inspection)
   uint64_t version = 0;  // changes with every build or refit (keys derived data such as sun grids)
   mutable float build_ms = 0.0f;
   mutable bool build_ms_pending = false;  // refit timing not yet read back
@@ -440,7 +447,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_qnodes); dfree(s->d_scratch);
-  dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key);
+  dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key); dfree(s->d_topo); dfree(s->d_cross); dfree(s->d_units); dfree(s->d_stray); dfree(s->d_uscratch);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
   dfree(s->d_lt_lights); dfree(s->d_lt_orig); dfree(s->d_lt_dir); dfree(s->d_lt_tmax); dfree(s->d_lt_hits);
@@ -529,14 +536,17 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
   launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, s->d_cpart, s->d_cbounds, write_inst, st);
+  s->world_stale = false;
   launch_morton(s->d_world, n, s->d_cbounds, s->d_keys, s->d_ids, st);
   radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
-  launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, st);
+  launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, s->d_topo, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  if (launch_levels(s->d_nodes, n, s->d_lvl_key, s->d_keys2, s->d_ids2, s->d_order, s->d_scratch,
-                    s->d_hist, &s->level_begin, st) != 0)
+  if (launch_levels(s->d_nodes, s->d_topo, s->d_leaf_parent, n, s->d_lvl_key, s->d_keys2, s->d_ids2, s->d_order,
+                    s->d_scratch, s->d_uscratch, s->d_hist, s->d_cross, s->d_units, s->d_stray, &s->level_begin,
+                    &s->cross_begin, &s->nunits, &s->nstray, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
-  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, st);
   ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
@@ -553,10 +563,12 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   ++s->version;
   ANE_CHECK_HIP(hipEventRecord(s->ev0, st));
   const int n = s->ntri;
-  // a refit keeps the Morton order: no centroid bounds
-  launch_flatten(s->view, s->d_world, s->d_tri_inst, s->d_tri_rec, nullptr, nullptr, false, st);
-  launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_order, s->level_begin, st);
+  // a refit keeps the Morton order: the fit makes the leaf-order triangles straight
+  // from the triangle records (no centroid bounds, no input-order world copy)
+  const SceneK view = s->view;
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, st);
+  s->world_stale = true;
   ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
@@ -639,7 +651,12 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
   ANE_A(d_order, n);
   ANE_A(d_lvl_key, n);
-  ANE_A(d_hist, 128);
+  ANE_A(d_hist, 2 * 128 + 2);
+  ANE_A(d_topo, n > 1 ? n - 1 : 1);
+  ANE_A(d_cross, n > 1 ? n - 1 : 1);
+  ANE_A(d_units, n > 1 ? n - 1 : 1);
+  ANE_A(d_stray, n > 0 ? n : 1);
+  ANE_A(d_uscratch, 2 * n + 2);
 #undef ANE_A
   auto bail = [&](int code) { scene_free(s); delete s; return code; };
   hipError_t e = hipSuccess;
@@ -780,7 +797,14 @@ extern "C" int ane_scene_debug_bvh(const ane_scene* s, uint64_t* keys, uint32_t*
     if (right) right[i] = nodes[i].d.y;
     if (parent) parent[i] = nodes[i].d.z;
   }
-  if (boxes) ANE_CHECK_HIP(hipMemcpy(boxes, s->d_node_box, sizeof(float) * 6 * (n - 1), hipMemcpyDeviceToHost));
+  // a node's box is the union of its two child boxes in the traversal record
+  if (boxes)
+    for (int64_t i = 0; i < n - 1; ++i) {
+      const NodeBuild& q = nodes[i];
+      float* b = boxes + 6 * i;
+      b[0] = std::fmin(q.a.x, q.b.x); b[1] = std::fmin(q.a.z, q.b.z); b[2] = std::fmin(q.c.x, q.c.z);
+      b[3] = std::fmax(q.a.y, q.b.y); b[4] = std::fmax(q.a.w, q.b.w); b[5] = std::fmax(q.c.y, q.c.w);
+    }
   return ANE_OK;
 }
 
@@ -788,6 +812,11 @@ extern "C" int ane_scene_debug_world_tris(const ane_scene* s, float* tris9)
 {
   if (!s || !tris9) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
   ANE_CHECK_HIP(hipDeviceSynchronize());
+  if (s->world_stale) {
+    launch_world_from_tris(s->d_tris, s->ntri, s->d_world, nullptr);
+    ANE_CHECK_HIP(hipGetLastError());
+    ANE_CHECK_HIP(hipDeviceSynchronize());
+  }
   ANE_CHECK_HIP(hipMemcpy(tris9, s->d_world, sizeof(float) * 9 * (size_t)s->ntri, hipMemcpyDeviceToHost));
   return ANE_OK;
 }
@@ -1310,7 +1339,7 @@ extern "C" int ane_ddgi_translate(ane_ddgi_ctx* c, const float* cam, int32_t* sh
   if (!c || !cam) return fail(ANE_ERR_INVALID_ARG, "NULL argument");
   if (shifted) *shifted = 0;
   if (!std::isfinite(cam[0]) || !std::isfinite(cam[2])) return fail(ANE_ERR_INVALID_ARG, "non-finite camera");
-  // IrradianceProbeTranslationPass.cpp:206-222
+  // IrradianceProbeTranslationPass.cpp:60-76 (probe cell of the camera, delta to the last one)
   const int ix = (int)(cam[0] / c->g.step[0]);
   const int iz = (int)(cam[2] / c->g.step[2]);
   const int dx = ix - c->last_cell[0], dz = iz - c->last_cell[1];

@@ -341,54 +341,77 @@ __global__ __launch_bounds__(kBlock) void k_radix_digit_scan(uint32_t* hist, int
 }
 
 // Stable scatter of one tile: digit d of tile t goes to (digits below d in all
-// tiles) + (digit d in the tiles before t) + (rank within the tile).
+// tiles) + (digit d in the tiles before t) + (rank within the tile).  The tile is
+// first ordered by digit in LDS (ranks by ballots, as a stable counting sort), so
+// that consecutive threads write consecutive addresses of each digit's run.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* kin, const uint32_t* vin,
                                                          uint64_t* kout, uint32_t* vout, int64_t n,
                                                          int shift, const uint32_t* offs,
                                                          const uint32_t* totals, int64_t ntiles)
 {
-  __shared__ uint32_t run[256];
+  __shared__ uint32_t run[256];    // next tile-local position of each digit
+  __shared__ uint32_t gbase[256];  // global position of the tile's digit d, minus its local start
   __shared__ uint32_t wcnt[kBlock / 64][256];
+  __shared__ uint64_t skey[kTile];
+  __shared__ uint32_t sval[kTile];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int tn = (int)min((int64_t)kTile, n - base);
+  uint64_t key[kItems];
+  uint32_t val[kItems];
+  run[threadIdx.x] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const int64_t idx = base + r * kBlock + threadIdx.x;
+    key[r] = idx < n ? kin[idx] : 0ull;
+    val[r] = idx < n ? vin[idx] : 0u;
+    if (idx < n) atomicAdd(&run[(uint32_t)(key[r] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
   uint32_t tot;
+  const uint32_t lstart = block_exclusive_scan(run[threadIdx.x], &tot);
   const uint32_t below = block_exclusive_scan(totals[threadIdx.x], &tot);
-  run[threadIdx.x] = below + offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
-  int64_t base = (int64_t)blockIdx.x * kTile;
+  gbase[threadIdx.x] = below + offs[(int64_t)threadIdx.x * ntiles + blockIdx.x] - lstart;
+  run[threadIdx.x] = lstart;
   for (int r = 0; r < kItems; ++r) {
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) wcnt[w][threadIdx.x] = 0;
     __syncthreads();
-    int64_t idx = base + r * kBlock + threadIdx.x;
-    bool valid = idx < n;
-    uint64_t key = valid ? kin[idx] : 0ull;
-    uint32_t val = valid ? vin[idx] : 0u;
-    uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const bool valid = base + r * kBlock + threadIdx.x < n;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
     uint64_t mask = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      bool bit = (d >> b) & 1u;
-      uint64_t bal = __ballot(bit);
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
       mask &= bit ? bal : ~bal;
     }
-    uint32_t rank = (uint32_t)__popcll(mask & lt);
+    const uint32_t rank = (uint32_t)__popcll(mask & lt);
     if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(mask);
     __syncthreads();
     if (valid) {
-      uint32_t pre = run[d];
-      for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
-      uint32_t dst = pre + rank;
-      if (dst < (uint64_t)n) {
-        kout[dst] = key;
-        vout[dst] = val;
-      }
+      uint32_t q = run[d];
+      for (int w = 0; w < wave; ++w) q += wcnt[w][d];
+      q += rank;
+      skey[q] = key[r];
+      sval[q] = val[r];
     }
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) add += wcnt[w][threadIdx.x];
     run[threadIdx.x] += add;
-    __syncthreads();
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < tn; q += kBlock) {
+    const uint64_t k = skey[q];
+    const uint32_t dst = gbase[(uint32_t)(k >> shift) & 255u] + (uint32_t)q;
+    if (dst < (uint64_t)n) {
+      kout[dst] = k;
+      vout[dst] = sval[q];
+    }
   }
 }
 
@@ -437,7 +460,7 @@ __device__ __forceinline__ int kdelta(const uint64_t* k, int n, int i, int j)
 }
 
 __global__ __launch_bounds__(256) void k_karras(const uint64_t* k, int n, NodeBuild* nodes,
-                                                int32_t* leaf_parent)
+                                                int32_t* leaf_parent, int4* topo)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
@@ -465,9 +488,17 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* k, int n, NodeBu
   int R = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
   nodes[i].d.x = L;
   nodes[i].d.y = R;
-  if (i == 0) nodes[0].d.z = -1;
-  if (L < 0) leaf_parent[~L] = i; else nodes[L].d.z = i;
-  if (R < 0) leaf_parent[~R] = i; else nodes[R].d.z = i;
+  // compact topology for the fit: children, parent, and j, the other end of the
+  // node's leaf range [min(i, j), max(i, j)]
+  topo[i].x = L;
+  topo[i].y = R;
+  topo[i].w = j;
+  if (i == 0) {
+    nodes[0].d.z = -1;
+    topo[0].z = -1;
+  }
+  if (L < 0) leaf_parent[~L] = i; else { nodes[L].d.z = i; topo[L].z = i; }
+  if (R < 0) leaf_parent[~R] = i; else { nodes[R].d.z = i; topo[R].z = i; }
 }
 
 // ---------------------------------------------------------------------------
@@ -485,16 +516,35 @@ __global__ __launch_bounds__(256) void k_reorder(const float* world, const uint3
   tris[3 * (int64_t)p + 2] = make_float4(w[6], w[7], w[8], 0.0f);
 }
 
-__device__ __forceinline__ void write_child_box(NodeBuild* nd, int slot, const float* lo,
-                                                const float* hi)
+// Refit: the triangles in leaf order straight from their 64-byte records and the
+// moved instances (the flatten's transform, written at the triangle's sorted
+// position), without the input-order world copy a build sorts from.
+__global__ __launch_bounds__(256) void k_refit_tris(SceneK sc, const uint32_t* ids, int n, float4* tris)
 {
-  if (slot == 0) {
-    nd->a.x = lo[0]; nd->a.y = hi[0]; nd->a.z = lo[1]; nd->a.w = hi[1];
-    nd->c.x = lo[2]; nd->c.y = hi[2];
-  } else {
-    nd->b.x = lo[0]; nd->b.y = hi[0]; nd->b.z = lo[1]; nd->b.w = hi[1];
-    nd->c.z = lo[2]; nd->c.w = hi[2];
-  }
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t id = ids[p];
+  const float4* r = sc.tri_rec + 4 * (int64_t)id;
+  const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+  const float* m = sc.inst[__float_as_uint(d.x)].transform;
+  const f3 w0 = xform_point(m, mk(a.x, a.y, a.z));
+  const f3 w1 = xform_point(m, mk(b.x, b.y, b.z));
+  const f3 w2 = xform_point(m, mk(c.x, c.y, c.z));
+  tris[3 * (int64_t)p + 0] = make_float4(w0.x, w0.y, w0.z, __uint_as_float(id));
+  tris[3 * (int64_t)p + 1] = make_float4(w1.x, w1.y, w1.z, 0.0f);
+  tris[3 * (int64_t)p + 2] = make_float4(w2.x, w2.y, w2.z, 0.0f);
+}
+
+// The input-order world triangles from the leaf-order ones (for inspection after a refit).
+__global__ __launch_bounds__(256) void k_world_from_tris(const float4* tris, int n, float* world)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float4 v0 = tris[3 * (int64_t)p], v1 = tris[3 * (int64_t)p + 1], v2 = tris[3 * (int64_t)p + 2];
+  float* w = world + 9 * (int64_t)__float_as_uint(v0.w);
+  w[0] = v0.x; w[1] = v0.y; w[2] = v0.z;
+  w[3] = v1.x; w[4] = v1.y; w[5] = v1.z;
+  w[6] = v2.x; w[7] = v2.y; w[8] = v2.z;
 }
 
 // Box of one child: a leaf's triangle box or an internal node's stored box.
@@ -514,22 +564,6 @@ __device__ __forceinline__ void child_box(const float4* tris, const float* node_
   }
 }
 
-// Fit one internal node from its (already final) children: child boxes into
-// the traversal record, the union into node_box.
-__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, int i)
-{
-  NodeBuild* nd = &nodes[i];
-  const int L = nd->d.x, R = nd->d.y;
-  float l0[3], h0[3], l1[3], h1[3];
-  child_box(tris, node_box, L, l0, h0);
-  child_box(tris, node_box, R, l1, h1);
-  nd->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
-  nd->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
-  nd->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
-  float* nb = node_box + 6 * (int64_t)i;
-  nb[0] = fminf(l0[0], l1[0]); nb[1] = fminf(l0[1], l1[1]); nb[2] = fminf(l0[2], l1[2]);
-  nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
-}
 
 // Quantized nodes (layout next to NodeBuild): one thread per internal node, from
 // the fitted child boxes.  The same arithmetic restated on the CPU gives the same
@@ -538,13 +572,40 @@ __device__ int quant_exp(double extent)
 {
   int e = -126;
   if (extent > 0.0) {
-    int k;
-    (void)frexp(extent, &k);  // extent < 2^k
+    // frexp's exponent k (extent = m 2^k, m in [1/2, 1)) from the bits: extent is the
+    // difference of two binary32 values, so a normal binary64
+    const int k = (int)((__double_as_longlong(extent) >> 52) & 0x7FF) - 1022;  // extent < 2^k
     e = k - 9;                // 254 * 2^(k-9) < 2^k
-    while (254.0 * ldexp(1.0, e) < extent) ++e;
+    while (ldexp(254.0, e) < extent) ++e;
     if (e < -126) e = -126;
   }
   return e;
+}
+
+// The quantized record of one node from its two child boxes (lo, hi per axis).
+__device__ __forceinline__ void quantize_node(const float* l0, const float* h0, const float* l1,
+                                              const float* h1, int L, int R, uint4* q2)
+{
+  uint32_t by[12], eb = 0u, ob[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float o = l0[a] < l1[a] ? l0[a] : l1[a];
+    const float hi = h0[a] > h1[a] ? h0[a] : h1[a];
+    const int e = quant_exp((double)hi - (double)o);
+    // division by the power-of-two step 2^e, exact as a scaling by 2^-e
+    by[0 + a] = (uint32_t)floor(ldexp((double)l0[a] - (double)o, -e));
+    by[3 + a] = (uint32_t)ceil(ldexp((double)h0[a] - (double)o, -e));
+    by[6 + a] = (uint32_t)floor(ldexp((double)l1[a] - (double)o, -e));
+    by[9 + a] = (uint32_t)ceil(ldexp((double)h1[a] - (double)o, -e));
+    eb |= (uint32_t)(e + 128) << (8 * a);
+    ob[a] = __float_as_uint(o);
+  }
+  eb |= (L < 0 ? 1u : 0u) << 24 | (R < 0 ? 1u : 0u) << 25;
+  const uint32_t g = (uint32_t)(L >= 0 ? L : ~L);
+  q2[0] = make_uint4(ob[0], ob[1], ob[2], eb);
+  q2[1] = make_uint4(by[0] | by[1] << 8 | by[2] << 16 | by[3] << 24,
+                     by[4] | by[5] << 8 | by[6] << 16 | by[7] << 24,
+                     by[8] | by[9] << 8 | by[10] << 16 | by[11] << 24, g);
 }
 
 __global__ __launch_bounds__(256) void k_quantize_nodes(const NodeBuild* nodes, int n, uint4* q)
@@ -554,27 +615,7 @@ __global__ __launch_bounds__(256) void k_quantize_nodes(const NodeBuild* nodes, 
   const NodeBuild nd = nodes[i];
   const float l0[3] = {nd.a.x, nd.a.z, nd.c.x}, h0[3] = {nd.a.y, nd.a.w, nd.c.y};
   const float l1[3] = {nd.b.x, nd.b.z, nd.c.z}, h1[3] = {nd.b.y, nd.b.w, nd.c.w};
-  uint32_t by[12], eb = 0u, ob[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float o = l0[a] < l1[a] ? l0[a] : l1[a];
-    const float hi = h0[a] > h1[a] ? h0[a] : h1[a];
-    const int e = quant_exp((double)hi - (double)o);
-    const double step = ldexp(1.0, e);
-    by[0 + a] = (uint32_t)floor(((double)l0[a] - (double)o) / step);
-    by[3 + a] = (uint32_t)ceil(((double)h0[a] - (double)o) / step);
-    by[6 + a] = (uint32_t)floor(((double)l1[a] - (double)o) / step);
-    by[9 + a] = (uint32_t)ceil(((double)h1[a] - (double)o) / step);
-    eb |= (uint32_t)(e + 128) << (8 * a);
-    ob[a] = __float_as_uint(o);
-  }
-  const int L = nd.d.x, R = nd.d.y;
-  eb |= (L < 0 ? 1u : 0u) << 24 | (R < 0 ? 1u : 0u) << 25;
-  const uint32_t g = (uint32_t)(L >= 0 ? L : ~L);
-  q[2 * (int64_t)i] = make_uint4(ob[0], ob[1], ob[2], eb);
-  q[2 * (int64_t)i + 1] = make_uint4(by[0] | by[1] << 8 | by[2] << 16 | by[3] << 24,
-                                     by[4] | by[5] << 8 | by[6] << 16 | by[7] << 24,
-                                     by[8] | by[9] << 8 | by[10] << 16 | by[11] << 24, g);
+  quantize_node(l0, h0, l1, h1, nd.d.x, nd.d.y, q + 2 * (int64_t)i);
 }
 
 void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t s)
@@ -607,7 +648,234 @@ __global__ __launch_bounds__(kBlock) void k_level_offsets(const uint32_t* totals
   if (threadIdx.x < kMaxLevels) begin[threadIdx.x] = pre;
 }
 
-// One depth level: every child is a leaf or belongs to a deeper, finished level.
+// Fit one internal node from its (already final) children: child boxes into
+// the traversal record, the union into node_box.
+__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, int i)
+{
+  NodeBuild* nd = &nodes[i];
+  const int L = nd->d.x, R = nd->d.y;
+  float l0[3], h0[3], l1[3], h1[3];
+  child_box(tris, node_box, L, l0, h0);
+  child_box(tris, node_box, R, l1, h1);
+  nd->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
+  nd->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
+  nd->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+  float* nb = node_box + 6 * (int64_t)i;
+  nb[0] = fminf(l0[0], l1[0]); nb[1] = fminf(l0[1], l1[1]); nb[2] = fminf(l0[2], l1[2]);
+  nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Treelet fit.  A Karras node covers the leaves [min(i, j), max(i, j)] (i is one
+// end of its range).  The nodes covering at most kSpan leaves whose parent covers
+// more are the roots of treelets, which partition the hierarchy below the "top"
+// nodes (more than kSpan leaves, a fraction of a percent of all).  A workgroup per
+// treelet finishes it from LDS: the topology of the node indices in its leaf
+// range, its leaf boxes, the boxes it fits and arrival counters; each leaf walks
+// up, the first of the two arrivals at a node stops, the second fits it
+// (workgroup-scope ordering).  A walk ends at the first node whose range leaves
+// the treelet's.  The top nodes follow, deepest level first.  (Cutting the leaves
+// into fixed spans of kSpan instead leaves every ancestor of a span boundary to
+// the level pass: 7x the nodes and twice the levels.)
+// ---------------------------------------------------------------------------
+static constexpr int kSpan = 1024;
+static constexpr size_t kSpanLds = (size_t)kSpan * (sizeof(int4) + 12 * sizeof(float) + sizeof(uint32_t));
+
+__device__ __forceinline__ bool node_inside(int i, int j, int s, int e)
+{
+  return min(i, j) >= s && max(i, j) < e;
+}
+
+__device__ __forceinline__ int node_leaves(int i, int j) { return (i > j ? i - j : j - i) + 1; }
+
+__device__ __forceinline__ void span_child_box(const float* lb, const float* ib, int c, int s, float* lo,
+                                               float* hi)
+{
+  const float* b = c < 0 ? lb + (~c - s) : ib + (c - s);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = b[a * kSpan];
+    hi[a] = b[(3 + a) * kSpan];
+  }
+}
+
+// One thread per leaf of the span.  FROM_REC (refits): the leaf-order triangles
+// are first made from the triangle records and the moved instances (the
+// flatten's transform) and written to tris.
+template <bool FROM_REC>
+__global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* ids, float4* tris, NodeBuild* nodes,
+                                                    float* node_box, const int4* topo, const int32_t* leaf_parent,
+                                                    const int2* units, int n)
+{
+  extern __shared__ int4 span_lds[];
+  int4* nd = span_lds;                                       // [kSpan] topology of nodes s..
+  float* lb = reinterpret_cast<float*>(nd + kSpan);          // [6][kSpan] leaf boxes
+  float* ib = lb + 6 * kSpan;                                // [6][kSpan] boxes fitted here
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(ib + 6 * kSpan);  // [kSpan] arrivals
+  const int2 unit = units[blockIdx.x];  // the treelet's leaf range [x, y], at most kSpan leaves
+  const int s = unit.x;
+  const int e = unit.y + 1;
+  const int ni = min(e, n - 1) - s;  // node indices in the range (the treelet's and maybe an ancestor)
+  const int leaf = s + (int)threadIdx.x;
+  int p = leaf < e ? leaf_parent[leaf] : -1;
+  for (int k = threadIdx.x; k < kSpan; k += blockDim.x) cnt[k] = 0u;
+  for (int k = threadIdx.x; k < ni; k += blockDim.x) nd[k] = topo[s + k];
+  for (int k = threadIdx.x; k < e - s; k += blockDim.x) {
+    const int64_t p = s + k;
+    float4 v0, v1, v2;
+    if (FROM_REC) {
+      const uint32_t id = ids[p];
+      const float4* r = sc.tri_rec + 4 * (int64_t)id;
+      const float4 ra = r[0], rb = r[1], rc = r[2], rd = r[3];
+      const float* m = sc.inst[__float_as_uint(rd.x)].transform;
+      const f3 w0 = xform_point(m, mk(ra.x, ra.y, ra.z));
+      const f3 w1 = xform_point(m, mk(rb.x, rb.y, rb.z));
+      const f3 w2 = xform_point(m, mk(rc.x, rc.y, rc.z));
+      v0 = make_float4(w0.x, w0.y, w0.z, __uint_as_float(id));
+      v1 = make_float4(w1.x, w1.y, w1.z, 0.0f);
+      v2 = make_float4(w2.x, w2.y, w2.z, 0.0f);
+      tris[3 * p] = v0;
+      tris[3 * p + 1] = v1;
+      tris[3 * p + 2] = v2;
+    } else {
+      v0 = tris[3 * p];
+      v1 = tris[3 * p + 1];
+      v2 = tris[3 * p + 2];
+    }
+    lb[0 * kSpan + k] = fminf(fminf(v0.x, v1.x), v2.x);
+    lb[1 * kSpan + k] = fminf(fminf(v0.y, v1.y), v2.y);
+    lb[2 * kSpan + k] = fminf(fminf(v0.z, v1.z), v2.z);
+    lb[3 * kSpan + k] = fmaxf(fmaxf(v0.x, v1.x), v2.x);
+    lb[4 * kSpan + k] = fmaxf(fmaxf(v0.y, v1.y), v2.y);
+    lb[5 * kSpan + k] = fmaxf(fmaxf(v0.z, v1.z), v2.z);
+  }
+  __syncthreads();
+  {
+    while (p >= s && p < s + ni) {
+      const int4 d = nd[p - s];
+      if (!node_inside(p, d.w, s, e)) break;  // crossing: fitted later
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      const uint32_t old = atomicAdd(&cnt[p - s], 1u);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (old == 0u) break;  // the sibling subtree is not finished yet
+      float l0[3], h0[3], l1[3], h1[3];
+      span_child_box(lb, ib, d.x, s, l0, h0);
+      span_child_box(lb, ib, d.y, s, l1, h1);
+      NodeBuild* q = &nodes[p];
+      q->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
+      q->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
+      q->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+      float u[6];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        u[a] = fminf(l0[a], l1[a]);
+        u[3 + a] = fmaxf(h0[a], h1[a]);
+      }
+#pragma unroll
+      for (int a = 0; a < 6; ++a) ib[a * kSpan + (p - s)] = u[a];
+      // the box goes to global memory only where a crossing parent will read it
+      const int up = d.z;
+      if (!(up >= s && up < s + ni && node_inside(up, nd[up - s].w, s, e))) {
+        float* nb = node_box + 6 * (int64_t)p;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) nb[a] = u[a];
+      }
+      p = up;  // -1 past the root
+    }
+  }
+}
+
+// Treelet roots (flag[i], i < m) and the leaves hanging directly from a top node
+// (sflag[k], k < n); the last slot of each stays 0 so that the exclusive scans
+// end in the totals.
+__global__ __launch_bounds__(256) void k_unit_flags(const int4* topo, const int32_t* leaf_parent, int n,
+                                                    uint32_t* flag, uint32_t* sflag)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = n - 1;
+  if (k < m) {
+    const int i = (int)k;
+    const int4 t = topo[i];
+    const bool fits = node_leaves(i, t.w) <= kSpan;
+    const bool parent_top = t.z < 0 || node_leaves(t.z, topo[t.z].w) > kSpan;
+    flag[i] = (fits && parent_top) ? 1u : 0u;
+  } else if (k == m) {
+    flag[m] = 0u;
+  }
+  if (k < n) {
+    const int p = leaf_parent[k];
+    sflag[k] = node_leaves(p, topo[p].w) > kSpan ? 1u : 0u;
+  } else if (k == n) {
+    sflag[n] = 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_unit_scatter(const int4* topo, const int32_t* leaf_parent, int n,
+                                                      const uint32_t* pos, const uint32_t* spos, int2* units,
+                                                      int32_t* stray)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n - 1 && pos[k + 1] != pos[k]) {
+    const int i = (int)k, j = topo[i].w;
+    units[pos[k]] = make_int2(min(i, j), max(i, j));
+  }
+  if (k < n && spos[k + 1] != spos[k]) stray[spos[k]] = (int32_t)k;
+}
+
+// Refit: the leaf-order triangles of the leaves outside every treelet.
+__global__ __launch_bounds__(256) void k_refit_stray(SceneK sc, const uint32_t* ids, const int32_t* stray, int nstray,
+                                                     float4* tris)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nstray) return;
+  const int64_t p = stray[k];
+  const uint32_t id = ids[p];
+  const float4* r = sc.tri_rec + 4 * (int64_t)id;
+  const float4 ra = r[0], rb = r[1], rc = r[2], rd = r[3];
+  const float* mt = sc.inst[__float_as_uint(rd.x)].transform;
+  const f3 w0 = xform_point(mt, mk(ra.x, ra.y, ra.z));
+  const f3 w1 = xform_point(mt, mk(rb.x, rb.y, rb.z));
+  const f3 w2 = xform_point(mt, mk(rc.x, rc.y, rc.z));
+  tris[3 * p] = make_float4(w0.x, w0.y, w0.z, __uint_as_float(id));
+  tris[3 * p + 1] = make_float4(w1.x, w1.y, w1.z, 0.0f);
+  tris[3 * p + 2] = make_float4(w2.x, w2.y, w2.z, 0.0f);
+}
+
+// The top nodes of the depth-sorted order (order[k], k < m), kept in depth
+// order: flags, then (after an exclusive scan of the flags) their positions.
+__global__ __launch_bounds__(256) void k_cross_flags(const int32_t* order, int64_t m, const int4* topo,
+                                                     uint32_t* flag)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > m) return;
+  if (k == m) {
+    flag[m] = 0u;
+    return;
+  }
+  const int i = order[k];
+  const int j = topo[i].w;
+  flag[k] = node_leaves(i, j) > kSpan ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_cross_scatter(const int32_t* order, int64_t m, const int4* topo,
+                                                       const uint32_t* pos, int32_t* cross)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int i = order[k];
+  const int j = topo[i].w;
+  if (node_leaves(i, j) > kSpan) cross[pos[k]] = i;
+}
+
+// cross_begin[d] = crossing nodes shallower than depth d (level offsets in `cross`)
+__global__ void k_cross_offsets(const uint32_t* begin, const uint32_t* pos, uint32_t* cross_begin)
+{
+  const int d = threadIdx.x;
+  if (d < kMaxLevels) cross_begin[d] = pos[begin[d]];
+}
+
+// One depth level of the crossing nodes: every child is a leaf, inside a span
+// (fitted by k_fit_span) or a crossing node of a deeper, finished level.
 __global__ __launch_bounds__(256) void k_fit_level(const float4* tris, NodeBuild* nodes,
                                                    float* node_box, const int32_t* order, int begin,
                                                    int count)
@@ -661,12 +929,24 @@ void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t*
                      ids);
 }
 
-void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
+void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent, int4* topo,
                    hipStream_t s)
 {
   if (n < 2) return;
   hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, keys, n, nodes,
-                     leaf_parent);
+                     leaf_parent, topo);
+}
+
+void launch_refit_tris(const SceneK& sc, const uint32_t* ids, int n, float4* tris, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_refit_tris, dim3(blocks_for(n, 256)), dim3(256), 0, s, sc, ids, n, tris);
+}
+
+void launch_world_from_tris(const float4* tris, int n, float* world, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_world_from_tris, dim3(blocks_for(n, 256)), dim3(256), 0, s, tris, n, world);
 }
 
 void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s)
@@ -675,11 +955,22 @@ void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, ids, n, tris);
 }
 
-int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, uint32_t* val,
-                  int32_t* order, uint32_t* scratch, uint32_t* begin,
-                  std::vector<int32_t>* level_begin, hipStream_t s)
+// the treelet count and the stray-leaf count after the two offset tables
+__global__ void k_unit_counts(const uint32_t* pos, const uint32_t* spos, int n, uint32_t* out)
+{
+  out[0] = pos[n - 1];
+  out[1] = spos[n];
+}
+
+int launch_levels(const NodeBuild* nodes, const int4* topo, const int32_t* leaf_parent, int n, uint64_t* key,
+                  uint64_t* key2, uint32_t* val, int32_t* order, uint32_t* scratch, uint32_t* uscratch,
+                  uint32_t* begin, int32_t* cross, int2* units, int32_t* stray, std::vector<int32_t>* level_begin,
+                  std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s)
 {
   level_begin->clear();
+  cross_begin->clear();
+  *nunits = 0;
+  *nstray = 0;
   if (n < 2) return 0;
   const int64_t m = n - 1;
   hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(m, 256)), dim3(256), 0, s, nodes, n, key, val);
@@ -695,7 +986,26 @@ int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, 
                      reinterpret_cast<uint32_t*>(order), m, 0, (const uint32_t*)hist, (const uint32_t*)totals,
                      ntiles);
   hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(kBlock), 0, s, (const uint32_t*)totals, begin);
-  uint32_t h[kMaxLevels];
+  // crossing nodes in depth order: flags over the order, scan, scatter, level offsets
+  uint32_t* pos = reinterpret_cast<uint32_t*>(key);  // m + 1 words (the depth keys are consumed)
+  hipLaunchKernelGGL(k_cross_flags, dim3(blocks_for(m + 1, 256)), dim3(256), 0, s,
+                     (const int32_t*)order, m, topo, pos);
+  exclusive_scan(pos, m + 1, scratch, s);
+  hipLaunchKernelGGL(k_cross_scatter, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const int32_t*)order, m,
+                     topo, (const uint32_t*)pos, cross);
+  hipLaunchKernelGGL(k_cross_offsets, dim3(1), dim3(kMaxLevels), 0, s, (const uint32_t*)begin,
+                     (const uint32_t*)pos, begin + kMaxLevels);
+  // treelets (one fit workgroup each) and the leaves outside them
+  uint32_t* upos = uscratch;           // m + 1 words
+  uint32_t* spos = uscratch + n;       // n + 1 words
+  hipLaunchKernelGGL(k_unit_flags, dim3(blocks_for(n + 1, 256)), dim3(256), 0, s, topo, leaf_parent, n, upos, spos);
+  exclusive_scan(upos, m + 1, scratch, s);
+  exclusive_scan(spos, (int64_t)n + 1, scratch, s);
+  hipLaunchKernelGGL(k_unit_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, topo, leaf_parent, n,
+                     (const uint32_t*)upos, (const uint32_t*)spos, units, stray);
+  hipLaunchKernelGGL(k_unit_counts, dim3(1), dim3(1), 0, s, (const uint32_t*)upos, (const uint32_t*)spos, n,
+                     begin + 2 * kMaxLevels);
+  uint32_t h[2 * kMaxLevels + 2];
   if (hipMemcpyAsync(h, begin, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
   int levels = 0;
@@ -705,27 +1015,61 @@ int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, 
   }
   for (int d = 0; d < levels; ++d) level_begin->push_back((int32_t)h[d]);
   level_begin->push_back(levels < kMaxLevels ? (int32_t)h[levels] : (int32_t)m);
+  for (int d = 0; d <= levels; ++d)
+    cross_begin->push_back((int32_t)h[kMaxLevels + (d < kMaxLevels ? d : kMaxLevels - 1)]);
+  *nunits = (int)h[2 * kMaxLevels];
+  *nstray = (int)h[2 * kMaxLevels + 1];
   return 0;
 }
 
-void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
-                const std::vector<int32_t>& level_begin, hipStream_t s)
+void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
+                const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
+                const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
+                const uint32_t* ids, hipStream_t s)
 {
-  if (n < 2 || level_begin.size() < 2) return;
-  const int levels = (int)level_begin.size() - 1;
+  if (n < 2 || cross_begin.size() < 2) {
+    if (refit && n == 1) launch_refit_tris(*refit, ids, n, tris, s);
+    return;
+  }
+  static bool lds_set = false;
+  if (!lds_set) {
+    (void)hipFuncSetAttribute((const void*)k_fit_span<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kSpanLds);
+    (void)hipFuncSetAttribute((const void*)k_fit_span<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kSpanLds);
+    lds_set = true;
+  }
+  const SceneK none{};
+  if (refit && nstray > 0)
+    hipLaunchKernelGGL(k_refit_stray, dim3(blocks_for(nstray, 256)), dim3(256), 0, s, *refit, ids, stray, nstray,
+                       tris);
+  if (nunits > 0) {
+    if (refit)
+      hipLaunchKernelGGL(k_fit_span<true>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, *refit, ids, tris,
+                         nodes, node_box, topo, leaf_parent, units, n);
+    else
+      hipLaunchKernelGGL(k_fit_span<false>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, none,
+                         (const uint32_t*)nullptr, tris, nodes, node_box, topo, leaf_parent, units, n);
+  }
+  // the crossing nodes, deepest level first: a launch per level below the top
+  // ones, then the top levels in one workgroup
+  const int levels = (int)cross_begin.size() - 1;
   for (int d = levels - 1; d >= kTopLevels; --d) {
-    const int cnt = level_begin[d + 1] - level_begin[d];
+    const int cnt = cross_begin[d + 1] - cross_begin[d];
     if (cnt > 0)
-      hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, tris, nodes,
-                         node_box, order, level_begin[d], cnt);
+      hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, (const float4*)tris, nodes,
+                         node_box, cross, cross_begin[d], cnt);
   }
   TopLevels t{};
   t.num = levels < kTopLevels ? levels : kTopLevels;
+  int top = 0;
   for (int d = 0; d < t.num; ++d) {
-    t.begin[d] = level_begin[d];
-    t.count[d] = level_begin[d + 1] - level_begin[d];
+    t.begin[d] = cross_begin[d];
+    t.count[d] = cross_begin[d + 1] - cross_begin[d];
+    top += t.count[d];
   }
-  hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, tris, nodes, node_box, order, t);
+  if (top > 0)
+    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (const float4*)tris, nodes, node_box, cross, t);
 }
 
 

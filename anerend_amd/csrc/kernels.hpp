@@ -23,19 +23,34 @@ void launch_flatten(const SceneK& sc, float* world, uint32_t* tri_inst, float4* 
                     uint32_t* cbounds, bool write_inst, hipStream_t s);
 void launch_morton(const float* world, int n, const uint32_t* cbounds, uint64_t* keys,
                    uint32_t* ids, hipStream_t s);
-void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent,
+// topo[i] = (left, right, parent, j): node i covers the leaves [min(i, j), max(i, j)]
+void launch_karras(const uint64_t* keys, int n, NodeBuild* nodes, int32_t* leaf_parent, int4* topo,
                    hipStream_t s);
 void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris, hipStream_t s);
-// Buckets internal nodes by depth (syncs the stream once); level_begin gets
-// the offset of every depth in `order` plus a final end offset.
+// refit: leaf-order triangles from the triangle records and the instances (no world copy)
+void launch_refit_tris(const SceneK& sc, const uint32_t* ids, int n, float4* tris, hipStream_t s);
+// input-order world triangles [n][9] from the leaf-order ones
+void launch_world_from_tris(const float4* tris, int n, float* world, hipStream_t s);
+// Buckets internal nodes by depth, lists the top nodes (more than 1024 leaves) in
+// depth order, the treelets below them (leaf ranges) and the leaves directly under
+// a top node (syncs the stream once).  level_begin gets the offset of every depth
+// in `order` plus a final end offset, cross_begin the same for `cross`.
 // key/key2: n words of 64 bits, val: n words, scratch: radix_scratch_words(n),
-// begin: 128 words (the per-depth offsets read back to the host)
-int launch_levels(const NodeBuild* nodes, int n, uint64_t* key, uint64_t* key2, uint32_t* val,
-                  int32_t* order, uint32_t* scratch, uint32_t* begin,
-                  std::vector<int32_t>* level_begin, hipStream_t s);
-// Level-synchronous bottom-up boxes (no atomics, no inter-workgroup fences).
-void launch_fit(const float4* tris, int n, NodeBuild* nodes, float* node_box, const int32_t* order,
-                const std::vector<int32_t>& level_begin, hipStream_t s);
+// uscratch: 2 n + 2 words, begin: 258 words (read back to the host), cross and
+// units: n - 1 entries, stray: n entries.
+int launch_levels(const NodeBuild* nodes, const int4* topo, const int32_t* leaf_parent, int n, uint64_t* key,
+                  uint64_t* key2, uint32_t* val, int32_t* order, uint32_t* scratch, uint32_t* uscratch,
+                  uint32_t* begin, int32_t* cross, int2* units, int32_t* stray, std::vector<int32_t>* level_begin,
+                  std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s);
+// Bottom-up boxes: one launch for every treelet (LDS-local walks), then the top
+// nodes level by level (no inter-workgroup fences).
+// refit (refits): the leaf-order triangles are first made from the triangle
+// records and instances of that scene view (ids: the leaf order), else tris is
+// read as it is.
+void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
+                const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
+                const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
+                const uint32_t* ids, hipStream_t s);
 // trace.hip
 // Sun grid of the probe shadow rays (sungrid.hip): projected along the sun
 // direction, cell (cx, cy) of a point = (sun_cell(X, x0, icw, gx), sun_cell(Y, y0, ich, gy))

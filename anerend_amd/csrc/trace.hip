@@ -1221,7 +1221,7 @@ __global__ __launch_bounds__(256) void k_light_rays(const float* lights, int n, 
   tmax[i] = L[3];
 }
 
-// light_shadow.rchit:29-32 (hit distance), light_shadow.rmiss:12-15 (the range),
+// light_shadow.rchit:15-18 (hit distance), light_shadow.rmiss:12-15 (the range),
 // light_shadow.rgen:54-58 (disabled light: 0), stored at rgen:43-49, 84
 __global__ __launch_bounds__(256) void k_light_dist(const float* lights, int n, const uint32_t* order,
                                                     const ane_hit* hits, float* dist, float* dist_img)

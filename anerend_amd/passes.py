@@ -156,7 +156,7 @@ class IrradianceProbeTranslationPass(RenderPass):
                 return
             if rc.is_baking():
                 return
-            # camera cell delta and the shifted copies: IrradianceProbeTranslationPass.cpp:206-330
+            # camera cell delta and the shifted copies: IrradianceProbeTranslationPass.cpp:60-186
             rc.grid.translate(rc.get_camera().get_position())
         fgb.register_render_pass_exe("IrradianceProbeTrans", exe)
 

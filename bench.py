@@ -467,8 +467,14 @@ def run_dynamic(frames, warmup, rebuild_every, node_format, with_cpu):
     frs = [scenegen.dynamic_frame(5, f, cfg) for f in range(frames + 1)]
     ds = native.DeviceScene(scene)
     ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[node_format])
+    first_build_ms = ds.info()["build_ms"]
+    # warm: a rebuild and a refit of frame 0's instances (the same hierarchy and boxes)
+    ds.rebuild(insts[0])
     info = ds.info()
-    log(f"[bench] {scene.num_tris} triangles, LBVH {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
+    ds.refit(insts[0])
+    warm_refit_ms = ds.info()["build_ms"]
+    log(f"[bench] {scene.num_tris} triangles, LBVH build {info['build_ms']:.3f} ms (first {first_build_ms:.2f}), "
+        f"refit {warm_refit_ms:.3f} ms, depth {info['bvh_depth']}")
     grid = native.ProbeGrid(cfg)
     for f in range(warmup):  # warm-up frames outside the timed run, then restart the path
         grid.update(ds, frs[f])
@@ -516,6 +522,8 @@ def run_dynamic(frames, warmup, rebuild_every, node_format, with_cpu):
                           "rebuild every frame" if K == 1 else "refit every frame"),
            "stages_ms": {k: round(v, 4) for k, v in st["mean_ms"].items()},
            "last_bvh_update_ms": round(refit_ms, 4),
+           "lbvh_build_ms": round(info["build_ms"], 3),
+           "lbvh_refit_ms": round(warm_refit_ms, 3),
            "fp8_error_vs_fp64": err,
            "cpu_baseline": cpu}
     grid.close()
@@ -692,7 +700,14 @@ def main():
     ds = native.DeviceScene(scene)
     ds.set_node_format({"auto": 0, "full": 1, "quantized": 2}[args.node_format])
     info = ds.info()
-    log(f"[bench] LBVH on GPU: {info['build_ms']:.2f} ms, depth {info['bvh_depth']}")
+    first_build_ms = info["build_ms"]  # includes the first launch of every build kernel in the process
+    # warm figures: a rebuild and a refit from the same instances (the same hierarchy and boxes)
+    ds.rebuild(scene.instances)
+    info = ds.info()
+    ds.refit(scene.instances)
+    refit_ms = ds.info()["build_ms"]
+    log(f"[bench] LBVH on GPU: build {info['build_ms']:.3f} ms (first {first_build_ms:.2f}), "
+        f"refit {refit_ms:.3f} ms, depth {info['bvh_depth']}")
     grid = native.ProbeGrid(cfg)
     P = cfg.probes_x * cfg.probes_y * cfg.probes_z
     N = cfg.rays_per_probe
@@ -861,6 +876,8 @@ def main():
                    "bvh_nodes": "quantized 32 B" if node_format == 2 else "full 64 B",
                    "conv_precision": args.precision, "bvh_depth": info["bvh_depth"],
                    "lbvh_build_ms": round(info["build_ms"], 3),
+                   "lbvh_first_build_ms": round(first_build_ms, 3),
+                   "lbvh_refit_ms": round(refit_ms, 3),
                    "sun_grid_build_ms": round(sungrid_ms, 3),
                    "atlas_bytes": atlas_bytes,
                    "precision_note": "trace, shading and hysteresis blend in FP32; the convolution "

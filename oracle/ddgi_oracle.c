@@ -1307,7 +1307,7 @@ void ora_shade_hit(const ora_scene* s, const ora_config* c, const ora_frame* f, 
   shade_surface(s, c, f, atlas, origin, hit, 0, c->t_min, c->t_max, out, shadowed, counters, NULL);
 }
 
-/* mirror_reflections.rgen:35-86 + .rchit + .rmiss: per pixel of a depth buffer,
+/* mirror_reflections.rgen:35-82 + .rchit + .rmiss: per pixel of a depth buffer,
  * the world position (as shadow.rgen), the normal from normal4 (xyz of 4 floats,
  * normalized), a closest-hit ray along reflect(normalize(pos - cameraPos), normal)
  * over [0.1, 50], shaded as above; out4 = (irradiance, distance to the hit), a

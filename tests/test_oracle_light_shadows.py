@@ -1,5 +1,5 @@
 """Oracle pins for the point-light shadow moments: light_shadow.rgen:27-86,
-light_shadow.rchit:29-32, light_shadow.rmiss:12-15 (distances of 1024 Fibonacci
+light_shadow.rchit:15-18, light_shadow.rmiss:12-15 (distances of 1024 Fibonacci
 rays per light) and light_shadow_sum.comp:16-71 (8x8 octahedral moments).
 
 What pins them:
