@@ -383,16 +383,18 @@ struct ane_scene {
   uint4* d_qnodes = nullptr;    // quantized traversal nodes (allocated when first used)
   int32_t node_format = ANE_NODES_AUTO;
   uint32_t* d_scratch = nullptr;
-  int32_t* d_order = nullptr;   // internal nodes bucketed by depth
-  uint64_t* d_lvl_key = nullptr;
-  uint32_t* d_hist = nullptr;
+  uint32_t* d_info = nullptr;   // fit plan read back at a build: level offsets, counts, tree depth
   int4* d_topo = nullptr;       // children, parent and leaf-range end of every internal node
   int32_t* d_cross = nullptr;   // top nodes (more leaves than a fit treelet holds), by depth
+  int32_t* d_top = nullptr;     // top nodes, by index
+  uint8_t* d_ndepth = nullptr;  // depth of the top nodes
   int2* d_units = nullptr;      // leaf ranges of the fit treelets
+  int32_t* d_unit_root = nullptr;
+  uint8_t* d_uheight = nullptr; // height of each treelet's root
   int32_t* d_stray = nullptr;   // leaves directly under a top node
   uint32_t* d_uscratch = nullptr;
   int32_t nunits = 0, nstray = 0;
-  std::vector<int32_t> level_begin, cross_begin;
+  std::vector<int32_t> cross_begin;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t depth = 0;
   bool world_stale = false;  // d_world predates the last refit (rebuilt from d_tris on inspection)
@@ -447,7 +449,8 @@ static void scene_free(ane_scene* s)
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_qnodes); dfree(s->d_scratch);
-  dfree(s->d_order); dfree(s->d_hist); dfree(s->d_lvl_key); dfree(s->d_topo); dfree(s->d_cross); dfree(s->d_units); dfree(s->d_stray); dfree(s->d_uscratch);
+  dfree(s->d_info); dfree(s->d_topo); dfree(s->d_cross); dfree(s->d_top); dfree(s->d_ndepth); dfree(s->d_units);
+  dfree(s->d_unit_root); dfree(s->d_uheight); dfree(s->d_stray); dfree(s->d_uscratch);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
   dfree(s->d_lt_lights); dfree(s->d_lt_orig); dfree(s->d_lt_dir); dfree(s->d_lt_tmax); dfree(s->d_lt_hits);
@@ -541,18 +544,24 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   radix_sort_pairs(s->d_keys, s->d_ids, s->d_keys2, s->d_ids2, n, s->d_scratch, st);
   launch_karras(s->d_keys, n, s->d_nodes, s->d_leaf_parent, s->d_topo, st);
   launch_reorder(s->d_world, s->d_ids, n, s->d_tris, st);
-  if (launch_levels(s->d_nodes, s->d_topo, s->d_leaf_parent, n, s->d_lvl_key, s->d_keys2, s->d_ids2, s->d_order,
-                    s->d_scratch, s->d_uscratch, s->d_hist, s->d_cross, s->d_units, s->d_stray, &s->level_begin,
-                    &s->cross_begin, &s->nunits, &s->nstray, st) != 0)
-    return fail(ANE_ERR_DEVICE, "LBVH level bucketing failed");
+  if (launch_fit_plan(s->d_topo, s->d_leaf_parent, n, s->d_scratch, s->d_uscratch, s->d_info, s->d_ndepth,
+                      s->d_cross, s->d_top, s->d_units, s->d_unit_root, s->d_stray, &s->cross_begin, &s->nunits,
+                      &s->nstray, st) != 0)
+    return fail(ANE_ERR_DEVICE, "LBVH fit plan failed");
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, st);
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, s->d_uheight, st);
   ANE_TRY(scene_apply_node_format(s, st));
+  uint32_t* d_depth = s->d_info + 132;
+  if (launch_tree_depth(s->d_topo, n, s->d_unit_root, s->d_uheight, s->nunits, s->d_top, s->d_uscratch,
+                        s->d_ndepth, d_depth, st) != 0)
+    return fail(ANE_ERR_DEVICE, "LBVH depth failed");
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
+  uint32_t depth = 0;
+  if (n >= 2) ANE_CHECK_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, st));
   ANE_CHECK_HIP(hipStreamSynchronize(st));
-  // the deepest internal nodes have two leaf children: leaf depth = number of levels
-  s->depth = (n >= 2) ? (int32_t)s->level_begin.size() - 1 : 0;
+  // the deepest internal nodes have two leaf children: leaf depth = number of internal levels
+  s->depth = (int32_t)depth;
   ANE_CHECK_HIP(hipEventElapsedTime(&s->build_ms, s->ev0, s->ev1));
   s->build_ms_pending = false;
   return ANE_OK;
@@ -567,7 +576,7 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   // from the triangle records (no centroid bounds, no input-order world copy)
   const SceneK view = s->view;
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, st);
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, nullptr, st);
   s->world_stale = true;
   ANE_TRY(scene_apply_node_format(s, st));
   ANE_CHECK_HIP(hipGetLastError());
@@ -649,14 +658,16 @@ extern "C" int ane_scene_create(const float* vertices, int64_t num_vertices, con
   ANE_A(d_cbounds, 6);
   ANE_A(d_cpart, flatten_partial_words((int)n));
   ANE_A(d_scratch, radix_scratch_words(n > 0 ? n : 1));
-  ANE_A(d_order, n);
-  ANE_A(d_lvl_key, n);
-  ANE_A(d_hist, 2 * 128 + 2);
+  ANE_A(d_info, 136);
   ANE_A(d_topo, n > 1 ? n - 1 : 1);
   ANE_A(d_cross, n > 1 ? n - 1 : 1);
+  ANE_A(d_top, n > 1 ? n - 1 : 1);
+  ANE_A(d_ndepth, n > 1 ? n - 1 : 1);
   ANE_A(d_units, n > 1 ? n - 1 : 1);
+  ANE_A(d_unit_root, n > 1 ? n - 1 : 1);
+  ANE_A(d_uheight, n > 1 ? n - 1 : 1);
   ANE_A(d_stray, n > 0 ? n : 1);
-  ANE_A(d_uscratch, 2 * n + 2);
+  ANE_A(d_uscratch, 3 * n + 3);
 #undef ANE_A
   auto bail = [&](int code) { scene_free(s); delete s; return code; };
   hipError_t e = hipSuccess;

@@ -624,30 +624,6 @@ void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t 
   hipLaunchKernelGGL(k_quantize_nodes, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, s, nodes, n, q);
 }
 
-// Depth (root = 0) of every internal node as a radix key, node id as value.
-__global__ __launch_bounds__(256) void k_node_depth(const NodeBuild* nodes, int n, uint64_t* key,
-                                                    uint32_t* val)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  uint32_t d = 0;
-  int node = nodes[i].d.z;
-  while (node >= 0 && node < n - 1 && d < (uint32_t)kMaxLevels - 1) {
-    ++d;
-    node = nodes[node].d.z;
-  }
-  key[i] = d;
-  val[i] = (uint32_t)i;
-}
-
-// The start of depth level d in the sorted order: the count of all shallower nodes.
-__global__ __launch_bounds__(kBlock) void k_level_offsets(const uint32_t* totals, uint32_t* begin)
-{
-  uint32_t tot;
-  const uint32_t pre = block_exclusive_scan(totals[threadIdx.x], &tot);
-  if (threadIdx.x < kMaxLevels) begin[threadIdx.x] = pre;
-}
-
 // Fit one internal node from its (already final) children: child boxes into
 // the traversal record, the union into node_box.
 __device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, int i)
@@ -679,7 +655,7 @@ __device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, f
 // the level pass: 7x the nodes and twice the levels.)
 // ---------------------------------------------------------------------------
 static constexpr int kSpan = 1024;
-static constexpr size_t kSpanLds = (size_t)kSpan * (sizeof(int4) + 12 * sizeof(float) + sizeof(uint32_t));
+static constexpr size_t kSpanLds = (size_t)kSpan * (sizeof(int4) + 12 * sizeof(float) + sizeof(uint32_t) + 1);
 
 __device__ __forceinline__ bool node_inside(int i, int j, int s, int e)
 {
@@ -705,13 +681,14 @@ __device__ __forceinline__ void span_child_box(const float* lb, const float* ib,
 template <bool FROM_REC>
 __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* ids, float4* tris, NodeBuild* nodes,
                                                     float* node_box, const int4* topo, const int32_t* leaf_parent,
-                                                    const int2* units, int n)
+                                                    const int2* units, int n, uint8_t* uheight)
 {
   extern __shared__ int4 span_lds[];
   int4* nd = span_lds;                                       // [kSpan] topology of nodes s..
   float* lb = reinterpret_cast<float*>(nd + kSpan);          // [6][kSpan] leaf boxes
   float* ib = lb + 6 * kSpan;                                // [6][kSpan] boxes fitted here
   uint32_t* cnt = reinterpret_cast<uint32_t*>(ib + 6 * kSpan);  // [kSpan] arrivals
+  uint8_t* hg = reinterpret_cast<uint8_t*>(cnt + kSpan);      // [kSpan] heights of the fitted nodes
   const int2 unit = units[blockIdx.x];  // the treelet's leaf range [x, y], at most kSpan leaves
   const int s = unit.x;
   const int e = unit.y + 1;
@@ -773,23 +750,27 @@ __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* i
       }
 #pragma unroll
       for (int a = 0; a < 6; ++a) ib[a * kSpan + (p - s)] = u[a];
-      // the box goes to global memory only where a crossing parent will read it
+      const uint8_t h = (uint8_t)(1 + max(d.x < 0 ? 0 : (int)hg[d.x - s], d.y < 0 ? 0 : (int)hg[d.y - s]));
+      hg[p - s] = h;
+      // the treelet root: its box goes to global memory for its top parent, its height
+      // for the depth of the tree
       const int up = d.z;
       if (!(up >= s && up < s + ni && node_inside(up, nd[up - s].w, s, e))) {
         float* nb = node_box + 6 * (int64_t)p;
 #pragma unroll
         for (int a = 0; a < 6; ++a) nb[a] = u[a];
+        if (uheight) uheight[blockIdx.x] = h;
       }
       p = up;  // -1 past the root
     }
   }
 }
 
-// Treelet roots (flag[i], i < m) and the leaves hanging directly from a top node
-// (sflag[k], k < n); the last slot of each stays 0 so that the exclusive scans
-// end in the totals.
+// Treelet roots (flag[i], i < m), top nodes (tflag[i], i < m) and the leaves
+// hanging directly from a top node (sflag[k], k < n); the last slot of each stays
+// 0 so that the exclusive scans end in the totals.
 __global__ __launch_bounds__(256) void k_unit_flags(const int4* topo, const int32_t* leaf_parent, int n,
-                                                    uint32_t* flag, uint32_t* sflag)
+                                                    uint32_t* flag, uint32_t* sflag, uint32_t* tflag)
 {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int m = n - 1;
@@ -799,8 +780,10 @@ __global__ __launch_bounds__(256) void k_unit_flags(const int4* topo, const int3
     const bool fits = node_leaves(i, t.w) <= kSpan;
     const bool parent_top = t.z < 0 || node_leaves(t.z, topo[t.z].w) > kSpan;
     flag[i] = (fits && parent_top) ? 1u : 0u;
+    tflag[i] = fits ? 0u : 1u;
   } else if (k == m) {
     flag[m] = 0u;
+    tflag[m] = 0u;
   }
   if (k < n) {
     const int p = leaf_parent[k];
@@ -810,16 +793,93 @@ __global__ __launch_bounds__(256) void k_unit_flags(const int4* topo, const int3
   }
 }
 
-__global__ __launch_bounds__(256) void k_unit_scatter(const int4* topo, const int32_t* leaf_parent, int n,
-                                                      const uint32_t* pos, const uint32_t* spos, int2* units,
-                                                      int32_t* stray)
+__global__ __launch_bounds__(256) void k_unit_scatter(const int4* topo, int n, const uint32_t* pos,
+                                                      const uint32_t* spos, const uint32_t* tpos, int2* units,
+                                                      int32_t* unit_root, int32_t* stray, int32_t* top)
 {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n - 1 && pos[k + 1] != pos[k]) {
-    const int i = (int)k, j = topo[i].w;
-    units[pos[k]] = make_int2(min(i, j), max(i, j));
+  if (k < n - 1) {
+    const int i = (int)k;
+    if (pos[k + 1] != pos[k]) {
+      const int j = topo[i].w;
+      units[pos[k]] = make_int2(min(i, j), max(i, j));
+      unit_root[pos[k]] = i;
+    }
+    if (tpos[k + 1] != tpos[k]) top[tpos[k]] = i;
   }
   if (k < n && spos[k + 1] != spos[k]) stray[spos[k]] = (int32_t)k;
+}
+
+// Depth of every top node (its ancestors are all top nodes): the walk to the root.
+__global__ __launch_bounds__(256) void k_top_depth(const int32_t* top, const uint32_t* ntop, const int4* topo,
+                                                   uint8_t* ndepth)
+{
+  const uint32_t nt = *ntop;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) {
+    const int i = top[k];
+    int d = 0, p = topo[i].z;
+    while (p >= 0 && d < kMaxLevels - 1) {
+      ++d;
+      p = topo[p].z;
+    }
+    ndepth[i] = (uint8_t)d;
+  }
+}
+
+// The top nodes in depth order (a counting sort in one workgroup; the order
+// within a level is immaterial): cross, and out[d] = the start of depth d in it
+// for d <= kMaxLevels, out[kMaxLevels + 1] = the number of depth levels.
+__global__ __launch_bounds__(1024) void k_top_sort(const int32_t* top, const uint32_t* ntop, const uint8_t* ndepth,
+                                                   int32_t* cross, uint32_t* out)
+{
+  __shared__ uint32_t cnt[kMaxLevels], cur[kMaxLevels];
+  __shared__ uint32_t maxd;
+  const uint32_t nt = *ntop;
+  if (threadIdx.x < kMaxLevels) cnt[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) maxd = 0u;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) {
+    const uint32_t d = ndepth[top[k]];
+    atomicAdd(&cnt[d], 1u);
+    atomicMax(&maxd, d + 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0u;
+    for (int d = 0; d < kMaxLevels; ++d) {
+      cur[d] = run;
+      out[d] = run;
+      run += cnt[d];
+    }
+    out[kMaxLevels] = run;
+    out[kMaxLevels + 1] = maxd;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) {
+    const int i = top[k];
+    cross[atomicAdd(&cur[ndepth[i]], 1u)] = i;
+  }
+}
+
+// Levels of the tree (internal nodes on the longest root-leaf path): over the
+// treelets, the depth of the root plus the treelet's height; over the top nodes,
+// their depth + 1.
+__global__ __launch_bounds__(256) void k_tree_depth(const int32_t* unit_root, const uint8_t* uheight, int nunits,
+                                                    const int32_t* top, const uint32_t* ntop, const int4* topo,
+                                                    const uint8_t* ndepth, uint32_t* out)
+{
+  const uint32_t nt = *ntop;
+  uint32_t best = 0u;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (int64_t)nunits + nt;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    if (k < nunits) {
+      const int p = topo[unit_root[k]].z;
+      best = max(best, (p < 0 ? 0u : (uint32_t)ndepth[p] + 1u) + (uint32_t)uheight[k]);
+    } else {
+      best = max(best, (uint32_t)ndepth[top[k - nunits]] + 1u);
+    }
+  }
+  atomicMax(out, best);
 }
 
 // Refit: the leaf-order triangles of the leaves outside every treelet.
@@ -839,39 +899,6 @@ __global__ __launch_bounds__(256) void k_refit_stray(SceneK sc, const uint32_t* 
   tris[3 * p] = make_float4(w0.x, w0.y, w0.z, __uint_as_float(id));
   tris[3 * p + 1] = make_float4(w1.x, w1.y, w1.z, 0.0f);
   tris[3 * p + 2] = make_float4(w2.x, w2.y, w2.z, 0.0f);
-}
-
-// The top nodes of the depth-sorted order (order[k], k < m), kept in depth
-// order: flags, then (after an exclusive scan of the flags) their positions.
-__global__ __launch_bounds__(256) void k_cross_flags(const int32_t* order, int64_t m, const int4* topo,
-                                                     uint32_t* flag)
-{
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > m) return;
-  if (k == m) {
-    flag[m] = 0u;
-    return;
-  }
-  const int i = order[k];
-  const int j = topo[i].w;
-  flag[k] = node_leaves(i, j) > kSpan ? 1u : 0u;
-}
-
-__global__ __launch_bounds__(256) void k_cross_scatter(const int32_t* order, int64_t m, const int4* topo,
-                                                       const uint32_t* pos, int32_t* cross)
-{
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const int i = order[k];
-  const int j = topo[i].w;
-  if (node_leaves(i, j) > kSpan) cross[pos[k]] = i;
-}
-
-// cross_begin[d] = crossing nodes shallower than depth d (level offsets in `cross`)
-__global__ void k_cross_offsets(const uint32_t* begin, const uint32_t* pos, uint32_t* cross_begin)
-{
-  const int d = threadIdx.x;
-  if (d < kMaxLevels) cross_begin[d] = pos[begin[d]];
 }
 
 // One depth level of the crossing nodes: every child is a leaf, inside a span
@@ -955,77 +982,60 @@ void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris
   hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n, 256)), dim3(256), 0, s, world, ids, n, tris);
 }
 
-// the treelet count and the stray-leaf count after the two offset tables
-__global__ void k_unit_counts(const uint32_t* pos, const uint32_t* spos, int n, uint32_t* out)
+int launch_fit_plan(const int4* topo, const int32_t* leaf_parent, int n, uint32_t* scratch, uint32_t* uscratch,
+                    uint32_t* info, uint8_t* ndepth, int32_t* cross, int32_t* top, int2* units, int32_t* unit_root,
+                    int32_t* stray, std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s)
 {
-  out[0] = pos[n - 1];
-  out[1] = spos[n];
-}
-
-int launch_levels(const NodeBuild* nodes, const int4* topo, const int32_t* leaf_parent, int n, uint64_t* key,
-                  uint64_t* key2, uint32_t* val, int32_t* order, uint32_t* scratch, uint32_t* uscratch,
-                  uint32_t* begin, int32_t* cross, int2* units, int32_t* stray, std::vector<int32_t>* level_begin,
-                  std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s)
-{
-  level_begin->clear();
   cross_begin->clear();
   *nunits = 0;
   *nstray = 0;
   if (n < 2) return 0;
   const int64_t m = n - 1;
-  hipLaunchKernelGGL(k_node_depth, dim3(blocks_for(m, 256)), dim3(256), 0, s, nodes, n, key, val);
-  // one stable counting pass on the 8-bit depth digit (no global atomics)
-  const int64_t ntiles = (m + kTile - 1) / kTile;
-  uint32_t* hist = scratch;
-  uint32_t* totals = scratch + 256 * ntiles;
-  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, s, (const uint64_t*)key, m,
-                     0, hist, ntiles);
-  hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(kBlock), 0, s, hist, ntiles, totals);
-  hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, s,
-                     (const uint64_t*)key, (const uint32_t*)val, key2,
-                     reinterpret_cast<uint32_t*>(order), m, 0, (const uint32_t*)hist, (const uint32_t*)totals,
-                     ntiles);
-  hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(kBlock), 0, s, (const uint32_t*)totals, begin);
-  // crossing nodes in depth order: flags over the order, scan, scatter, level offsets
-  uint32_t* pos = reinterpret_cast<uint32_t*>(key);  // m + 1 words (the depth keys are consumed)
-  hipLaunchKernelGGL(k_cross_flags, dim3(blocks_for(m + 1, 256)), dim3(256), 0, s,
-                     (const int32_t*)order, m, topo, pos);
-  exclusive_scan(pos, m + 1, scratch, s);
-  hipLaunchKernelGGL(k_cross_scatter, dim3(blocks_for(m, 256)), dim3(256), 0, s, (const int32_t*)order, m,
-                     topo, (const uint32_t*)pos, cross);
-  hipLaunchKernelGGL(k_cross_offsets, dim3(1), dim3(kMaxLevels), 0, s, (const uint32_t*)begin,
-                     (const uint32_t*)pos, begin + kMaxLevels);
-  // treelets (one fit workgroup each) and the leaves outside them
-  uint32_t* upos = uscratch;           // m + 1 words
-  uint32_t* spos = uscratch + n;       // n + 1 words
-  hipLaunchKernelGGL(k_unit_flags, dim3(blocks_for(n + 1, 256)), dim3(256), 0, s, topo, leaf_parent, n, upos, spos);
+  uint32_t* upos = uscratch;              // m + 1 words
+  uint32_t* spos = uscratch + n;          // n + 1 words
+  uint32_t* tpos = uscratch + 2 * n + 1;  // m + 1 words
+  hipLaunchKernelGGL(k_unit_flags, dim3(blocks_for((int64_t)n + 1, 256)), dim3(256), 0, s, topo, leaf_parent, n,
+                     upos, spos, tpos);
   exclusive_scan(upos, m + 1, scratch, s);
   exclusive_scan(spos, (int64_t)n + 1, scratch, s);
-  hipLaunchKernelGGL(k_unit_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, topo, leaf_parent, n,
-                     (const uint32_t*)upos, (const uint32_t*)spos, units, stray);
-  hipLaunchKernelGGL(k_unit_counts, dim3(1), dim3(1), 0, s, (const uint32_t*)upos, (const uint32_t*)spos, n,
-                     begin + 2 * kMaxLevels);
-  uint32_t h[2 * kMaxLevels + 2];
-  if (hipMemcpyAsync(h, begin, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+  exclusive_scan(tpos, m + 1, scratch, s);
+  hipLaunchKernelGGL(k_unit_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, topo, n, (const uint32_t*)upos,
+                     (const uint32_t*)spos, (const uint32_t*)tpos, units, unit_root, stray, top);
+  const uint32_t* ntop = tpos + m;
+  hipLaunchKernelGGL(k_top_depth, dim3(1024), dim3(256), 0, s, (const int32_t*)top, ntop, topo, ndepth);
+  hipLaunchKernelGGL(k_top_sort, dim3(1), dim3(1024), 0, s, (const int32_t*)top, ntop, (const uint8_t*)ndepth,
+                     cross, info);
+  // info: [0, kMaxLevels] top-level offsets, [kMaxLevels + 1] levels, then the treelet
+  // and stray-leaf counts
+  hipMemcpyAsync(info + kMaxLevels + 2, upos + m, 4, hipMemcpyDeviceToDevice, s);
+  hipMemcpyAsync(info + kMaxLevels + 3, spos + n, 4, hipMemcpyDeviceToDevice, s);
+  uint32_t h[kMaxLevels + 4];
+  if (hipMemcpyAsync(h, info, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  int levels = 0;
-  for (int d = 0; d < kMaxLevels; ++d) {
-    const uint32_t next = (d + 1 < kMaxLevels) ? h[d + 1] : (uint32_t)m;
-    if (next > h[d]) levels = d + 1;
-  }
-  for (int d = 0; d < levels; ++d) level_begin->push_back((int32_t)h[d]);
-  level_begin->push_back(levels < kMaxLevels ? (int32_t)h[levels] : (int32_t)m);
-  for (int d = 0; d <= levels; ++d)
-    cross_begin->push_back((int32_t)h[kMaxLevels + (d < kMaxLevels ? d : kMaxLevels - 1)]);
-  *nunits = (int)h[2 * kMaxLevels];
-  *nstray = (int)h[2 * kMaxLevels + 1];
+  const int levels = (int)h[kMaxLevels + 1];
+  for (int d = 0; d <= levels && d <= kMaxLevels; ++d) cross_begin->push_back((int32_t)h[d]);
+  if (cross_begin->size() < 2) cross_begin->push_back(0);
+  *nunits = (int)h[kMaxLevels + 2];
+  *nstray = (int)h[kMaxLevels + 3];
+  return 0;
+}
+
+int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const uint8_t* uheight, int nunits,
+                      const int32_t* top, const uint32_t* uscratch, const uint8_t* ndepth, uint32_t* out,
+                      hipStream_t s)
+{
+  if (n < 2) return 0;
+  if (hipMemsetAsync(out, 0, 4, s) != hipSuccess) return -1;
+  const uint32_t* ntop = uscratch + 2 * n + 1 + (n - 1);
+  hipLaunchKernelGGL(k_tree_depth, dim3(256), dim3(256), 0, s, unit_root, uheight, nunits, top, ntop, topo, ndepth,
+                     out);
   return 0;
 }
 
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
-                const uint32_t* ids, hipStream_t s)
+                const uint32_t* ids, uint8_t* uheight, hipStream_t s)
 {
   if (n < 2 || cross_begin.size() < 2) {
     if (refit && n == 1) launch_refit_tris(*refit, ids, n, tris, s);
@@ -1046,10 +1056,10 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const in
   if (nunits > 0) {
     if (refit)
       hipLaunchKernelGGL(k_fit_span<true>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, *refit, ids, tris,
-                         nodes, node_box, topo, leaf_parent, units, n);
+                         nodes, node_box, topo, leaf_parent, units, n, uheight);
     else
       hipLaunchKernelGGL(k_fit_span<false>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, none,
-                         (const uint32_t*)nullptr, tris, nodes, node_box, topo, leaf_parent, units, n);
+                         (const uint32_t*)nullptr, tris, nodes, node_box, topo, leaf_parent, units, n, uheight);
   }
   // the crossing nodes, deepest level first: a launch per level below the top
   // ones, then the top levels in one workgroup

@@ -31,26 +31,31 @@ void launch_reorder(const float* world, const uint32_t* ids, int n, float4* tris
 void launch_refit_tris(const SceneK& sc, const uint32_t* ids, int n, float4* tris, hipStream_t s);
 // input-order world triangles [n][9] from the leaf-order ones
 void launch_world_from_tris(const float4* tris, int n, float* world, hipStream_t s);
-// Buckets internal nodes by depth, lists the top nodes (more than 1024 leaves) in
-// depth order, the treelets below them (leaf ranges) and the leaves directly under
-// a top node (syncs the stream once).  level_begin gets the offset of every depth
-// in `order` plus a final end offset, cross_begin the same for `cross`.
-// key/key2: n words of 64 bits, val: n words, scratch: radix_scratch_words(n),
-// uscratch: 2 n + 2 words, begin: 258 words (read back to the host), cross and
-// units: n - 1 entries, stray: n entries.
-int launch_levels(const NodeBuild* nodes, const int4* topo, const int32_t* leaf_parent, int n, uint64_t* key,
-                  uint64_t* key2, uint32_t* val, int32_t* order, uint32_t* scratch, uint32_t* uscratch,
-                  uint32_t* begin, int32_t* cross, int2* units, int32_t* stray, std::vector<int32_t>* level_begin,
-                  std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s);
+// The plan of a fit, once per topology (syncs the stream once): the treelets
+// (nodes of at most 1024 leaves under a node of more, the "top" nodes) as leaf
+// ranges and roots, the leaves directly under a top node, the top nodes in depth
+// order (cross) with the start of each depth level (cross_begin, one past the
+// last level at the end) and their depths (ndepth, by node).  scratch:
+// radix_scratch_words(n) words, uscratch: 3 n + 3 words, info: 136 words; cross,
+// top, units, unit_root: n - 1 entries, stray: n entries.
+int launch_fit_plan(const int4* topo, const int32_t* leaf_parent, int n, uint32_t* scratch, uint32_t* uscratch,
+                    uint32_t* info, uint8_t* ndepth, int32_t* cross, int32_t* top, int2* units, int32_t* unit_root,
+                    int32_t* stray, std::vector<int32_t>* cross_begin, int* nunits, int* nstray, hipStream_t s);
+// After a build's fit: the number of internal levels on the longest root-leaf path
+// into *out (device word), from the treelet heights and the top-node depths.
+int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const uint8_t* uheight, int nunits,
+                      const int32_t* top, const uint32_t* uscratch, const uint8_t* ndepth, uint32_t* out,
+                      hipStream_t s);
 // Bottom-up boxes: one launch for every treelet (LDS-local walks), then the top
 // nodes level by level (no inter-workgroup fences).
 // refit (refits): the leaf-order triangles are first made from the triangle
 // records and instances of that scene view (ids: the leaf order), else tris is
 // read as it is.
+// uheight (nullable): the height of each treelet's root, by treelet.
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
-                const uint32_t* ids, hipStream_t s);
+                const uint32_t* ids, uint8_t* uheight, hipStream_t s);
 // trace.hip
 // Sun grid of the probe shadow rays (sungrid.hip): projected along the sun
 // direction, cell (cx, cy) of a point = (sun_cell(X, x0, icw, gx), sun_cell(Y, y0, ich, gy))

