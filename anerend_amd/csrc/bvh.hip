@@ -1007,8 +1007,9 @@ int launch_fit_plan(const int4* topo, const int32_t* leaf_parent, int n, uint32_
                      cross, info);
   // info: [0, kMaxLevels] top-level offsets, [kMaxLevels + 1] levels, then the treelet
   // and stray-leaf counts
-  hipMemcpyAsync(info + kMaxLevels + 2, upos + m, 4, hipMemcpyDeviceToDevice, s);
-  hipMemcpyAsync(info + kMaxLevels + 3, spos + n, 4, hipMemcpyDeviceToDevice, s);
+  if (hipMemcpyAsync(info + kMaxLevels + 2, upos + m, 4, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(info + kMaxLevels + 3, spos + n, 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return -1;
   uint32_t h[kMaxLevels + 4];
   if (hipMemcpyAsync(h, info, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
   if (hipStreamSynchronize(s) != hipSuccess) return -1;

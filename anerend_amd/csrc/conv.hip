@@ -8,13 +8,17 @@
 //     E[t][p][c] = sum_r Wn[t][r] * L[p][c][r],   Wn = W / sum_r W
 // (M = 64 texels, K = N rays, columns = probes x 3 channels), followed by the
 // FP32 hysteresis epilogue (probe_conv.comp:63-89) written straight into the
-// atlas texels of each probe:
-//   k_conv_mfma  FP32 (v_mfma_f32_16x16x4_f32, the plain drop-in path), BF16
-//                (v_mfma_f32_16x16x32_bf16) or FP8-e4m3 (v_mfma_f32_16x16x32_fp8_fp8)
-//                operands, FP32 accumulate; FP8 carries a per-texel weight scale
-//                and an FP32 radiance scale per probe, channel and 32 rays
-//   k_conv_fp32  FP32 register-tiled kernel for ray counts that are not a
-//                multiple of 32
+// atlas texels of each probe.  Operands FP32 (v_mfma_f32_16x16x4_f32, the plain
+// drop-in path), BF16 / FP16 (v_mfma_f32_16x16x32_bf16 / _f16) or FP8-e4m3
+// (v_mfma_f32_16x16x32_fp8_fp8), FP32 accumulate; FP8 carries a per-texel weight
+// scale and an FP32 radiance scale per probe, channel and 32 rays.
+//   k_conv_stream  a whole update: 64 probes per block, the weights in LDS, each
+//                  wave streams its 16 probes over all rays
+//   k_conv_mfma    16 probes per block, 4 waves splitting the rays: weights too
+//                  large for LDS (FP32 beyond 640 rays, 16-bit beyond 1280)
+//   k_conv_jobs    the same blocks over the updates of a serving batch
+//   k_conv_fp32    FP32 register-tiled kernel for ray counts that are not a
+//                  multiple of 32
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -346,11 +350,323 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv_jobs(const ConvJob* jo
                    b - j.block_begin, part);
 }
 
-int conv_job_blocks(int num_probes) { return (num_probes + kConvProbes - 1) / kConvProbes; }
+// ---------------------------------------------------------------------------
+// Streaming MFMA convolution of a whole update.  Block = 4 waves x 16 probes.
+// The weights of every ray are staged once per block in LDS, in fragment order:
+// entry (s, rt, lane) holds the 8 k values A[rt*16 + lane%16][32 s + 8 (lane/16) ..]
+// that lane contracts in step s, so a wave's fragment read is 64 consecutive
+// lanes of LDS.  Each wave then contracts its own 16 probes over all rays (no
+// reduction across waves).  Its radiance is a ring of D steps: step s + D is
+// loaded while step s multiplies.  The loop body has no branches: D divides the
+// step count and the last D steps are peeled off (they load no radiance), so
+// the ring stays in the same registers from one iteration to the next.
+// Memory operations complete in issue order for the wait counters, so the
+// prologue issues the radiance ring, then the weights; writing the weights to
+// LDS waits for both, and step 0 needs its ring anyway.  The atlas texels of the
+// epilogue are loaded after that, during the contraction.  The epilogue blends
+// straight from the accumulator layout: lane (col, kq) holds texels
+// rt*16 + 4kq + i, i = 0..3, 4 consecutive texels (64 contiguous atlas bytes)
+// of tile row 2rt + kq/2 of probe col; its per-texel factors come from LDS.
+// ---------------------------------------------------------------------------
+#ifndef ANE_CONV_STREAM
+#define ANE_CONV_STREAM 1
+#endif
+// Radiance ring depth in 32-ray steps, per operand size (a sweep of 1, 2, 4 and 8:
+// deeper rings hold more registers; these were the fastest on the production shape).
+#ifndef ANE_CONV_DEPTH16
+#define ANE_CONV_DEPTH16 2
+#endif
+#ifndef ANE_CONV_DEPTH8
+#define ANE_CONV_DEPTH8 4
+#endif
+#ifndef ANE_CONV_DEPTH32
+#define ANE_CONV_DEPTH32 4
+#endif
+static constexpr int kStreamProbes = 64;
+static constexpr int kStreamLdsMax = 160 * 1024;
+static constexpr int kStreamFactorBytes = 2 * 64 * 4;  // per-texel scale and weight sum, after the weights
+static constexpr int kStreamStage = 16;                // weight entries per thread in flight at once
 
-void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, hipStream_t s)
+// One block's 64 probes (see above); blk is the block's index within its update.
+template <int PREC, int D>
+__device__ __forceinline__ void conv_stream_block(const GridK& g, const void* rad, const float* rad_scale,
+                                                  const void* wq, const float* wscale, const float* wsum,
+                                                  float4* atlas, int nprobes, int probe_base, int32_t bake,
+                                                  int blk)
+{
+  using F = ConvFrag<PREC>;
+  constexpr int EB = F::EB, FB = 8 * EB;
+  extern __shared__ __align__(16) uint8_t wl[];
+  const int n = g.rays, S = n / 32;  // S % D == 0
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  const int pw = blk * kStreamProbes + wave * 16;  // the wave's first probe
+  const bool live = pw + col < nprobes;
+  // lanes past the last probe read the last probe: their columns are never stored
+  const int pc = live ? pw + col : nprobes - 1;
+  const uint8_t* radp = (const uint8_t*)rad + ((int64_t)pc * 3 * n + 8 * kq) * EB;
+  const float* rsp = (PREC == ANE_PRECISION_FP8) ? rad_scale + (int64_t)pc * 3 * S : nullptr;
+
+  F b[D][3];
+  float bs[D][3];
+  auto load_b = [&](int s, int j) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[j][c].load(radp + ((int64_t)c * n + 32 * s) * EB);
+    if constexpr (PREC == ANE_PRECISION_FP8) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bs[j][c] = rsp[c * S + s];
+    }
+  };
+#pragma unroll
+  for (int j = 0; j < D; ++j) load_b(j, j);
+
+  // weights to LDS in fragment order, kStreamStage entries per thread in flight.
+  // Entries past the last step load and store the last step's entry of the same
+  // (row tile, lane) again: identical bytes, so no branch is needed.
+  const uint8_t* wp = (const uint8_t*)wq;
+  float* fac = reinterpret_cast<float*>(wl + (size_t)64 * n * EB);
+  for (int m0 = 0; m0 < S; m0 += kStreamStage) {
+    uint4 x[kStreamStage][FB == 32 ? 2 : 1];
+#pragma unroll
+    for (int m = 0; m < kStreamStage; ++m) {
+      const int s = min(m0 + m, S - 1), l = tid & 63, rt = tid >> 6;
+      const uint8_t* src = wp + ((int64_t)(rt * 16 + (l & 15)) * n + 32 * s + 8 * (l >> 4)) * EB;
+      if constexpr (FB == 32) {
+        x[m][0] = reinterpret_cast<const uint4*>(src)[0];
+        x[m][1] = reinterpret_cast<const uint4*>(src)[1];
+      } else if constexpr (FB == 16) {
+        x[m][0] = *reinterpret_cast<const uint4*>(src);
+      } else {
+        const uint2 y = *reinterpret_cast<const uint2*>(src);
+        x[m][0] = make_uint4(y.x, y.y, 0u, 0u);
+      }
+    }
+    // the per-texel factors (the weight-sum correction or FP8 weight scale, and the weight sum)
+    const int tf = tid & 63;
+    const float f_sc = (PREC == ANE_PRECISION_FP8) ? wscale[tf]
+                       : (PREC == ANE_PRECISION_BF16 || PREC == ANE_PRECISION_FP16) ? wscale[64 + tf] : 1.0f;
+    const float f_ws = wsum[tf];
+#pragma unroll
+    for (int m = 0; m < kStreamStage; ++m) {
+      const size_t e = (size_t)min(m0 + m, S - 1) * 256 + tid;
+      if constexpr (FB == 32) {
+        reinterpret_cast<uint4*>(wl + e * FB)[0] = x[m][0];
+        reinterpret_cast<uint4*>(wl + e * FB)[1] = x[m][1];
+      } else if constexpr (FB == 16) {
+        *reinterpret_cast<uint4*>(wl + e * FB) = x[m][0];
+      } else {
+        *reinterpret_cast<uint2*>(wl + e * FB) = make_uint2(x[m][0].x, x[m][0].y);
+      }
+    }
+    if (m0 == 0 && tid < 64) {
+      fac[tid] = f_sc;
+      fac[64 + tid] = f_ws;
+    }
+  }
+  __syncthreads();
+  if (pw >= nprobes) return;  // no barrier follows
+
+  // atlas texels of the epilogue, in flight during the contraction
+  const int q = probe_base + pc;
+  const int px = q % g.nx, pz = (q / g.nx) % g.nz, py = q / (g.nx * g.nz);
+  const int64_t tile = ((int64_t)10 * ((int64_t)py * g.nz + pz) + 1) * g.atlas_w + 10 * px + 1 + 4 * (kq & 1);
+  float4 old[4][4];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const int64_t row = tile + (int64_t)(2 * rt + (kq >> 1)) * g.atlas_w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) old[rt][i] = atlas[row + i];
+  }
+
+  f32x4 acc[4][3];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[rt][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  auto load_a = [&](int s, F (&a)[4]) {
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) a[rt].load(wl + ((size_t)(s * 4 + rt) * 64 + lane) * FB);
+  };
+  // step j of the ring: contract the step's radiance b[j] with the weights a
+  auto contract = [&](const F (&a)[4], int j) {
+    if constexpr (PREC == ANE_PRECISION_FP8) {
+      // each 32-ray step carries its own radiance scale: all 12 products first,
+      // then they are scaled into the accumulators
+      f32x4 t[4][3];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[rt][c] = F::mma(a[rt], b[j][c], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[rt][c] += t[rt][c] * bs[j][c];
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[rt][c] = F::mma(a[rt], b[j][c], acc[rt][c]);
+    }
+  };
+  // the weights of step s + 1 are read from LDS while step s multiplies; the
+  // last D steps load no more radiance, so nothing is in flight at the epilogue
+  F a[4];
+  load_a(0, a);
+  for (int s0 = 0; s0 < S - D; s0 += D) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      F an[4];
+      load_a(s0 + j + 1, an);
+      contract(a, j);
+      load_b(s0 + j + D, j);
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) a[rt] = an[rt];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    F an[4];
+    if (j + 1 < D) load_a(S - D + j + 1, an);
+    contract(a, j);
+    if (j + 1 < D) {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) a[rt] = an[rt];
+    }
+  }
+  if (!live) return;
+  const float h = g.hysteresis;
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const int64_t row = tile + (int64_t)(2 * rt + (kq >> 1)) * g.atlas_w;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = rt * 16 + 4 * kq + i;
+      const float sc = fac[t];
+      atlas[row + i] = blend_texel(old[rt][i], acc[rt][0][i] * sc, acc[rt][1][i] * sc, acc[rt][2][i] * sc,
+                                   fac[64 + t], h, bake != 0);
+    }
+  }
+}
+
+template <int PREC, int D>
+__global__ __launch_bounds__(256) void k_conv_stream(GridK g, const void* rad, const float* rad_scale,
+                                                     const void* wq, const float* wscale, const float* wsum,
+                                                     float4* atlas, int nprobes, int probe_base, int32_t bake)
+{
+  conv_stream_block<PREC, D>(g, rad, rad_scale, wq, wscale, wsum, atlas, nprobes, probe_base, bake, blockIdx.x);
+}
+
+// The streaming convolutions of many updates in one launch (the request loop's
+// fused stage): block b runs block b - block_begin of the last job whose
+// block_begin <= b; the dynamic LDS fits the jobs' largest weight table.
+template <int PREC, int D>
+__global__ __launch_bounds__(256) void k_conv_stream_jobs(const ConvJob* jobs, int njobs)
+{
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block_begin <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const ConvJob& j = jobs[lo];
+  conv_stream_block<PREC, D>(j.g, j.rad, j.rad_scale, j.wq, j.wscale, j.wsum, j.atlas, j.nprobes, j.probe_base,
+                             j.bake, b - j.block_begin);
+}
+
+static int conv_operand_bytes(int precision)
+{
+  return precision == ANE_PRECISION_FP32 ? 4 : precision == ANE_PRECISION_FP8 ? 1 : 2;
+}
+
+static size_t conv_stream_lds(int precision, int rays)
+{
+  return (size_t)64 * rays * conv_operand_bytes(precision) + kStreamFactorBytes;
+}
+
+bool conv_stream_fits(int precision, int rays)
+{
+  return ANE_CONV_STREAM && rays > 0 && rays % 32 == 0 && conv_stream_lds(precision, rays) <= (size_t)kStreamLdsMax;
+}
+
+// The ring depth: the largest power of two up to the precision's depth that
+// divides the step count S (all of them, for a job launch).  The results do not
+// depend on it: every step is accumulated in order whatever the depth.
+template <int PREC, int D>
+static void launch_stream_d(int S, bool jobs, const GridK& g, const void* rad, const float* rad_scale,
+                            const void* wq, const float* wscale, const float* wsum, float4* atlas, int num_probes,
+                            int probe_base, bool bake, const ConvJob* job_table, int njobs, int blocks, size_t lds,
+                            hipStream_t s)
+{
+  if constexpr (D > 1) {
+    if (S % D != 0) {
+      launch_stream_d<PREC, D / 2>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes, probe_base,
+                                   bake, job_table, njobs, blocks, lds, s);
+      return;
+    }
+  }
+  static bool lds_set = false;
+  if (!lds_set) {
+    (void)hipFuncSetAttribute((const void*)k_conv_stream<PREC, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kStreamLdsMax);
+    (void)hipFuncSetAttribute((const void*)k_conv_stream_jobs<PREC, D>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kStreamLdsMax);
+    lds_set = true;
+  }
+  if (jobs) {
+    hipLaunchKernelGGL((k_conv_stream_jobs<PREC, D>), dim3((unsigned)blocks), dim3(256), lds, s, job_table, njobs);
+  } else {
+    hipLaunchKernelGGL((k_conv_stream<PREC, D>), dim3((unsigned)blocks), dim3(256), lds, s, g, rad, rad_scale, wq,
+                       wscale, wsum, atlas, num_probes, probe_base, (int32_t)bake);
+  }
+}
+
+template <int PREC>
+static void launch_conv_stream(int S, bool jobs, const GridK& g, const void* rad, const float* rad_scale,
+                               const void* wq, const float* wscale, const float* wsum, float4* atlas,
+                               int num_probes, int probe_base, bool bake, const ConvJob* job_table, int njobs,
+                               int blocks, size_t lds, hipStream_t s)
+{
+  constexpr int D = PREC == ANE_PRECISION_FP32 ? ANE_CONV_DEPTH32
+                    : PREC == ANE_PRECISION_FP8 ? ANE_CONV_DEPTH8 : ANE_CONV_DEPTH16;
+  launch_stream_d<PREC, D>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes, probe_base, bake,
+                           job_table, njobs, blocks, lds, s);
+}
+
+static void launch_conv_stream_any(int precision, int S, bool jobs, const GridK& g, const void* rad,
+                                   const float* rad_scale, const void* wq, const float* wscale, const float* wsum,
+                                   float4* atlas, int num_probes, int probe_base, bool bake,
+                                   const ConvJob* job_table, int njobs, int blocks, size_t lds, hipStream_t s)
+{
+  if (precision == ANE_PRECISION_FP32)
+    launch_conv_stream<ANE_PRECISION_FP32>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes,
+                                           probe_base, bake, job_table, njobs, blocks, lds, s);
+  else if (precision == ANE_PRECISION_FP16)
+    launch_conv_stream<ANE_PRECISION_FP16>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes,
+                                           probe_base, bake, job_table, njobs, blocks, lds, s);
+  else if (precision == ANE_PRECISION_BF16)
+    launch_conv_stream<ANE_PRECISION_BF16>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes,
+                                           probe_base, bake, job_table, njobs, blocks, lds, s);
+  else
+    launch_conv_stream<ANE_PRECISION_FP8>(S, jobs, g, rad, rad_scale, wq, wscale, wsum, atlas, num_probes,
+                                          probe_base, bake, job_table, njobs, blocks, lds, s);
+}
+
+int conv_job_blocks(int precision, int rays, int num_probes)
+{
+  const int per = conv_stream_fits(precision, rays) ? kStreamProbes : kConvProbes;
+  return (num_probes + per - 1) / per;
+}
+
+void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, bool stream, int max_rays,
+                      int rays_gcd, hipStream_t s)
 {
   if (njobs <= 0 || total_blocks <= 0) return;
+  if (stream) {
+    launch_conv_stream_any(precision, rays_gcd / 32, true, GridK{}, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, 0, 0, false, jobs, njobs, total_blocks, conv_stream_lds(precision, max_rays), s);
+    return;
+  }
   const dim3 grid((unsigned)total_blocks), block(64 * kConvWaves);
   if (precision == ANE_PRECISION_FP32)
     hipLaunchKernelGGL(k_conv_jobs<ANE_PRECISION_FP32>, grid, block, 0, s, jobs, njobs);
@@ -371,6 +687,14 @@ void launch_convolve(const GridK& g, const FrameK& f, int precision, const void*
   if (num_probes <= 0) return;
   float4* a4 = reinterpret_cast<float4*>(atlas);
   const unsigned blocks = (unsigned)((num_probes + kConvProbes - 1) / kConvProbes);
+  if (conv_stream_fits(precision, g.rays)) {
+    const void* wq = precision == ANE_PRECISION_FP32 ? (const void*)w32
+                     : precision == ANE_PRECISION_FP8 ? (const void*)wfp8 : (const void*)wbf16;
+    const int blocks = (num_probes + kStreamProbes - 1) / kStreamProbes;
+    launch_conv_stream_any(precision, g.rays / 32, false, g, rad, rad_scale, wq, wfp8_scale, wsum, a4, num_probes,
+                           probe_base, bake, nullptr, 0, blocks, conv_stream_lds(precision, g.rays), s);
+    return;
+  }
   if (precision == ANE_PRECISION_FP32) {
     if (g.rays % 32 != 0) {  // the register-tiled kernel takes any ray count
       const unsigned b32 = (unsigned)((num_probes + kPB - 1) / kPB);

@@ -176,13 +176,20 @@ struct ConvJob {
   const float* wsum;
   float4* atlas;
   int32_t nprobes, probe_base, bake;
-  int32_t block_begin;  // conv_job_blocks(nprobes) blocks
+  int32_t block_begin;  // conv_job_blocks(precision, g.rays, nprobes) blocks
 };
 // max_rays: the largest ray count among the jobs (FP8 stages a probe in LDS)
 void launch_shade_jobs(int precision, const ShadeJob* jobs, int njobs, int total_blocks, int max_rays,
                        hipStream_t s);
-int conv_job_blocks(int num_probes);
-void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, hipStream_t s);
+// Whether a convolution of this precision and ray count takes the streaming
+// kernel (its weights fit LDS); the request loop launches such jobs apart from
+// the others so that every update runs the kernel its own launch would.
+bool conv_stream_fits(int precision, int rays);
+int conv_job_blocks(int precision, int rays, int num_probes);
+// stream: every job fits the streaming kernel; max_rays: their largest ray
+// count (LDS size); rays_gcd: the greatest common divisor of their ray counts
+void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, bool stream, int max_rays,
+                      int rays_gcd, hipStream_t s);
 
 int trace_lanes_per_launch();
 int trace_stack_lds();

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -161,24 +162,34 @@ void ane_serve::launch_fused(std::vector<Launch>& batch, int slot,
       (void)hipStreamSynchronize(fs);
     }
   }
-  // job tables grouped by precision, each group contiguous in the slot
+  // job tables grouped by precision, each group contiguous in the slot; within a
+  // group the convolutions that take the streaming kernel come first (one launch
+  // each kind, so that every update runs the kernel its own launch would)
   struct Group {
-    int first = 0, count = 0, shade_blocks = 0, conv_blocks = 0, max_rays = 0;
+    int first = 0, count = 0, shade_blocks = 0, max_rays = 0;
+    int conv_count[2] = {0, 0}, conv_blocks[2] = {0, 0}, conv_max_rays[2] = {0, 0}, conv_gcd[2] = {0, 0};
   } grp[4];
   int k = 0;
   for (int p = 0; p < 4; ++p) {
     grp[p].first = k;
-    for (size_t i = 0; i < batch.size(); ++i) {
-      if (prec[i] != p) continue;
-      sj[i].block_begin = grp[p].shade_blocks;
-      cj[i].block_begin = grp[p].conv_blocks;
-      grp[p].shade_blocks += cj[i].nprobes;
-      grp[p].conv_blocks += ane::conv_job_blocks(cj[i].nprobes);
-      grp[p].max_rays = std::max(grp[p].max_rays, (int)sj[i].g.rays);
-      hs[k] = sj[i];
-      hc[k] = cj[i];
-      ++k;
-      ++grp[p].count;
+    for (int kind = 0; kind < 2; ++kind) {
+      for (size_t i = 0; i < batch.size(); ++i) {
+        if (prec[i] != p) continue;
+        const int rays = cj[i].g.rays;
+        if ((ane::conv_stream_fits(p, rays) ? 0 : 1) != kind) continue;
+        sj[i].block_begin = grp[p].shade_blocks;
+        cj[i].block_begin = grp[p].conv_blocks[kind];
+        grp[p].shade_blocks += cj[i].nprobes;
+        grp[p].conv_blocks[kind] += ane::conv_job_blocks(p, rays, cj[i].nprobes);
+        grp[p].max_rays = std::max(grp[p].max_rays, (int)sj[i].g.rays);
+        grp[p].conv_max_rays[kind] = std::max(grp[p].conv_max_rays[kind], rays);
+        grp[p].conv_gcd[kind] = std::gcd(grp[p].conv_gcd[kind], rays);
+        hs[k] = sj[i];
+        hc[k] = cj[i];
+        ++k;
+        ++grp[p].count;
+        ++grp[p].conv_count[kind];
+      }
     }
   }
   hipError_t e = hipSuccess;
@@ -187,8 +198,12 @@ void ane_serve::launch_fused(std::vector<Launch>& batch, int slot,
       e = hipMemcpyAsync(dc, hc, sizeof(ane::ConvJob) * k, hipMemcpyHostToDevice, st);
     for (int p = 0; p < 4 && e == hipSuccess; ++p)
       ane::launch_shade_jobs(p, ds + grp[p].first, grp[p].count, grp[p].shade_blocks, grp[p].max_rays, st);
-    for (int p = 0; p < 4 && e == hipSuccess; ++p)
-      ane::launch_conv_jobs(p, dc + grp[p].first, grp[p].count, grp[p].conv_blocks, st);
+    for (int p = 0; p < 4 && e == hipSuccess; ++p) {
+      ane::launch_conv_jobs(p, dc + grp[p].first, grp[p].conv_count[0], grp[p].conv_blocks[0], true,
+                            grp[p].conv_max_rays[0], grp[p].conv_gcd[0], st);
+      ane::launch_conv_jobs(p, dc + grp[p].first + grp[p].conv_count[0], grp[p].conv_count[1],
+                            grp[p].conv_blocks[1], false, grp[p].conv_max_rays[1], grp[p].conv_gcd[1], st);
+    }
     if (e == hipSuccess) e = hipGetLastError();
   }
   for (size_t i = 0; i < batch.size(); ++i) {

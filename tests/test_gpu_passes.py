@@ -116,6 +116,37 @@ def test_convolution_precisions_vs_float64(precision, bound):
     assert rel.max() <= bound
 
 
+# Few rays do not average rounding out, so the ragged shapes take the worst case of a
+# weighted mean whose radiance and weights are both rounded (half an ulp each) and whose
+# weights are renormalised: 3 half-ulps of the operand format (BF16 2^-9, FP16 2^-12); FP8
+# e4m3 two half-ulps of 2^-4.
+RAGGED_BOUND = {PRECISION_FP32: RTOL, PRECISION_BF16: 3 * 2.0 ** -9, PRECISION_FP16: 3 * 2.0 ** -12,
+                PRECISION_FP8: 2.0 ** -3}
+
+
+@pytest.mark.parametrize("precision", [PRECISION_FP32, PRECISION_BF16, PRECISION_FP16, PRECISION_FP8])
+@pytest.mark.parametrize("dims,n", [((5, 3, 7), 96), ((9, 2, 5), 160), ((3, 1, 2), 32), ((7, 2, 5), 672)])
+def test_convolution_ragged_shapes(precision, dims, n):
+    """Probe counts that leave the last 64-probe block partly or wholly empty, ray counts
+    that are not a multiple of the radiance prefetch depth (and 672 FP32 rays, whose
+    weights no longer fit the streaming kernel's LDS), hysteresis 0.5 over a random atlas:
+    the whole atlas (blended interiors, untouched borders) against the oracle."""
+    bound = RAGGED_BOUND[precision]
+    cfg = Config.make(dims, step=(1, 2, 1), rays=n, hysteresis=0.5, precision=precision)
+    fr = scenegen.make_frame(7, config=cfg, iteration=3)
+    g = scenegen.PCG32(11, np.arange(cfg.num_probes * 3 * n))
+    L = g.log_uniform(1e-3, 50.0).reshape(cfg.num_probes, 3, n).astype(np.float32)
+    atlas = np.random.default_rng(9).uniform(0, 1, cfg.atlas_shape).astype(np.float32)
+    grid = native.ProbeGrid(cfg)
+    grid.atlas = atlas
+    grid.set_radiance(fr, L)
+    grid.convolve(fr)
+    O.convolve_blend(cfg, fr, O.rotate_dirs(fr.rotation, O.fibonacci_dirs(n)), L, atlas)
+    err = rel_err(grid.atlas, atlas)
+    print(f"precision {precision} dims {dims} rays {n}: max rel {err:.3e}")
+    assert err <= bound
+
+
 def _bake_frame(cfg, it):
     f = scenegen.make_frame(1, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it,
                             flags=FLAG_DDGI | FLAG_DDGI_MULTI | FLAG_BAKE)
