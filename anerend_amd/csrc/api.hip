@@ -234,7 +234,9 @@ struct StageTimer {
   }
 };
 
-enum { kStSetup = 0, kStTrace, kStSunGrid, kStShade, kStConv, kStTrans, kStages };
+enum { kStSetup = 0, kStTrace, kStSunGrid, kStShade, kStConv, kStTrans, kStShadow, kStages };
+static_assert(kStages == sizeof(((ane_ddgi_stats*)nullptr)->mean_ms) / sizeof(float),
+              "one stats entry per stage timer");
 
 }  // namespace ane
 
@@ -1185,10 +1187,16 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
     ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use, &c->timers[kStSunGrid]));
     if (use) a.sg = c->sgc.view();
   }
-  // primary and shadow rays in one persistent launch (the trace stage)
+  // primary rays in one persistent launch (with the shadow rays of their hits
+  // unless a sun grid is in use), then the sun-grid shadow rays
   ANE_TRY(c->timers[kStTrace].begin(st));
   launch_trace(a, false, false, c->counting, st);
   ANE_TRY(c->timers[kStTrace].end(st));
+  if (a.sg.list) {
+    ANE_TRY(c->timers[kStShadow].begin(st));
+    launch_probe_shadow_grid(a, c->counting, st);
+    ANE_TRY(c->timers[kStShadow].end(st));
+  }
   ANE_TRY(c->timers[kStShade].begin(st));
   launch_shade(s->view, c->g, fk, c->d_probe_org, c->d_hits, c->d_shadow,
                c->d_atlas[c->cur], c->cfg.precision, c->d_rad, c->d_rad_scale, nprobes,
@@ -1267,6 +1275,7 @@ int ddgi_update_front(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame*
     if (use) a.sg = c->sgc.view();
   }
   launch_trace(a, false, false, c->counting, st);
+  launch_probe_shadow_grid(a, c->counting, st);
   ANE_CHECK_HIP(hipGetLastError());
   *sj = ShadeJob{};
   sj->sc = s->view;

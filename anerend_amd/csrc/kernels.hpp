@@ -191,6 +191,10 @@ int conv_job_blocks(int precision, int rays, int num_probes);
 void launch_conv_jobs(int precision, const ConvJob* jobs, int njobs, int total_blocks, bool stream, int max_rays,
                       int rays_gcd, hipStream_t s);
 
+// With a sun grid view in a.sg (probe rays): the shadow rays of the hits of the
+// preceding launch_trace (k_probe_shadow_grid; it does nothing when the grid is
+// not in use on the device, in which case the trace ran them).
+void launch_probe_shadow_grid(const TraceArgs& a, bool count, hipStream_t s);
 int trace_lanes_per_launch();
 int trace_stack_lds();
 int trace_block_size();

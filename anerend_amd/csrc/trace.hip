@@ -10,11 +10,14 @@
 //                    finished lanes by ballot between traversal steps; short stack in
 //                    LDS with a global spill area; leaves tested in wave-wide batches.
 //                    A probe ray's closest hit (rgen:90-106) is followed in the same
-//                    lane by its any-hit shadow ray from the hit position (rchit:30-66),
-//                    against the triangles of its sun-grid cell when the context has a
-//                    current grid (sungrid.hip), else through the hierarchy.  Q = true
-//                    reads the scene's quantized 32-byte nodes (large scenes), else the
-//                    64-byte nodes; caller rays may carry a per-ray t_max
+//                    lane by its any-hit shadow ray from the hit position (rchit:30-66)
+//                    through the hierarchy, unless the context has a current sun grid
+//                    (sungrid.hip).  Q = true reads the scene's quantized 32-byte nodes
+//                    (large scenes), else the 64-byte nodes; caller rays may carry a
+//                    per-ray t_max
+//   k_probe_shadow_grid  with a sun grid: the shadow rays of the probe hits against
+//                    the triangles of their cells, the candidates of a wave's 64 rays
+//                    spread over its lanes
 //   k_shade          one block per probe: attribute fetch, Cook-Torrance direct light,
 //                    indirect light from the previous atlas, emissive, sky on miss
 //                    (rchit:24-110, rmiss:17-21, pbr_light.glsl); radiance stored in the
@@ -279,7 +282,8 @@ __device__ __forceinline__ f3 hit_geom_pos(const SceneK& sc, int32_t id, float u
 // traversal
 //   RAYS = false: probe rays.  A lane traces its primary ray (closest hit,
 //     rgen:90-106); when it hits, the same lane continues at once with the
-//     shadow ray from the hit position toward the sun (any hit, rchit:44-66).
+//     shadow ray from the hit position toward the sun (any hit, rchit:44-66),
+//     except with a sun grid in use: then k_probe_shadow_grid runs them next.
 //   RAYS = true: caller rays, closest hit or (ANY) any hit.
 // ---------------------------------------------------------------------------
 template <bool ANY, bool RAYS, bool COUNT, bool Q>
@@ -288,7 +292,6 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   constexpr int32_t kDone = 0x7FFFFFFF;  // node sentinel: the lane's traversal has ended
   constexpr int32_t kPop = 0x7FFFFFFE;   // step result: continue from the stack
   constexpr int32_t kWait = 0x7FFFFFFD;  // stack empty, the last leaf still waits for its test
-  constexpr int32_t kGridRay = 0x7FFFFFFC;  // shadow ray walking its sun-grid cell (sp: cursor, steps_left: end)
   // (internal nodes are 0 .. ntri - 2 < kWait; leaves are ~position < 0)
   // Leaf batching: a lane that reaches a leaf parks it in `pend`, pops its next
   // node and waits; the triangle tests of all parked leaves run together once
@@ -301,9 +304,9 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   constexpr bool SPEC = ANE_TRACE_SPEC && !RAYS && !COUNT;
   constexpr int kLeafBatch = RAYS ? kLeafBatchRays : (Q ? ANE_TRACE_LEAFBATCH_Q : kLeafBatchProbe);
   constexpr int kUnroll = Q ? ANE_TRACE_UNROLL_Q : ANE_TRACE_UNROLL;
-  // probe shadow rays test the triangles of their sun-grid cell (parked one at a
-  // time as batch leaves) instead of traversing the hierarchy
-  const bool sgrid = !RAYS && kLeafBatch > 0 && a.sg.list != nullptr && a.sg.params->valid != 0u;
+  // with a sun grid in use, the probe shadow rays are k_probe_shadow_grid's
+  // (launched after this kernel): the lanes here end at the primary hit
+  const bool sgrid = !RAYS && a.sg.list != nullptr && a.sg.params->valid != 0u;
   __shared__ int32_t stk[kSS][kTB];
   if (RAYS && a.skip_if_set && *a.skip_if_set != 0u) return;  // a sun grid answered these rays
   const int tid = threadIdx.x;
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         h.v = (bid >= 0) ? bv : 0.0f;
         a.hits[out] = h;
       }
-      if (!RAYS && !sph && bid >= 0) {
+      if (!RAYS && !sph && bid >= 0 && !sgrid) {
         // shadow ray from the hit position toward the sun (rchit:44-66), same lane
         HitGeom hg;
         const f3 p = hit_geom_pos(sc, bid, bu, bv, hg);
@@ -349,23 +352,11 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
         sph = true;
         tbest = tmax0;
         bid = -1;
-        if (sgrid) {
-          const SunGridParams* gq = a.sg.params;
-          const float X = comp(p, gq->kx) - gq->sx * comp(p, gq->kz);
-          const float Y = comp(p, gq->ky) - gq->sy * comp(p, gq->kz);
-          const int64_t cell = (int64_t)sun_cell(Y, gq->y0, gq->ich, gq->gy) * gq->gx +
-                               sun_cell(X, gq->x0, gq->icw, gq->gx);
-          sp = (int32_t)a.sg.start[cell];
-          steps_left = a.sg.start[cell + 1];
-          node = (uint32_t)sp < steps_left ? kGridRay : kDone;
-          if (COUNT) ++sn;  // one cell lookup
-        } else {
-          sp = 0;
-          steps_left = budget;
-          node = root;
-        }
+        sp = 0;
+        steps_left = budget;
+        node = root;
       } else {
-        if (!RAYS) a.shadow[out] = sph ? (uint8_t)(bid >= 0 ? 1 : 0) : (uint8_t)255;
+        if (!RAYS && !sgrid) a.shadow[out] = sph ? (uint8_t)(bid >= 0 ? 1 : 0) : (uint8_t)255;
         ray = -1;
       }
     }
@@ -427,20 +418,8 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
     const bool go = ray >= 0 && node != kDone &&
-                    (kLeafBatch == 0 || (node != kWait && ((SPEC && node != kGridRay) ? !(node < 0 && pend != 0)
-                                                                                         : pend == 0)));
-    if (!RAYS && go && node == kGridRay) {
-      // the next candidate of the shadow ray's cell waits for the batch
-      const uint32_t pos = a.sg.list[(uint32_t)sp];
-      ++sp;
-      if (pos >= (uint32_t)sc.ntri) {
-        *a.overflow = 1u;
-        node = kDone;
-      } else {
-        pend = ~(int32_t)pos;
-        node = (uint32_t)sp < steps_left ? kGridRay : kWait;
-      }
-    } else if (go) {
+                    (kLeafBatch == 0 || (node != kWait && (SPEC ? !(node < 0 && pend != 0) : pend == 0)));
+    if (go) {
       int32_t next = kPop;
       // guards: a malformed hierarchy ends the ray and raises the error flag
       // instead of reading out of bounds or looping forever
@@ -543,7 +522,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
       const uint64_t pm = __ballot(pend != 0);
       if (pm != 0ull) {
         const bool can = ray >= 0 && node != kDone && node != kWait &&
-                         ((SPEC && node != kGridRay) ? !(node < 0 && pend != 0) : pend == 0);
+                         (SPEC ? !(node < 0 && pend != 0) : pend == 0);
         if (__popcll(pm) >= kLeafBatch || __ballot(can) == 0ull) {
           if (pend != 0) {
             if (COUNT) {
@@ -585,6 +564,145 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
   }
 }
 
+// ---------------------------------------------------------------------------
+// Probe shadow rays through the sun grid (rchit:44-66), after the traversal of
+// the primary rays: the shadow ray of every hit tests the triangles of its
+// sun-grid cell in list order, and any hit shadows it.
+// A wave takes 64 consecutive output slots and spreads the candidate tests of
+// their rays over all its lanes: the candidates of the 64 rays (ray after ray,
+// list order within a ray) are tested 64 at a time, so every lane runs a test
+// instead of walking its own ray's list.  Per ray the earliest event in list
+// order is kept in LDS (key 2 pos for a hit, 2 pos + 1 for a list entry out of
+// range; 2 count + 1 for none), and the tests of candidates after a known event
+// are skipped.
+// The outcome is that of the in-order walk that stops at its first hit: the ray
+// is shadowed iff its first event is a hit, after pos + 1 tests (pos without
+// a hit).  Every shadow flag of the launch is written here (255: no hit).
+// ---------------------------------------------------------------------------
+#ifndef ANE_SHADOW_U
+#define ANE_SHADOW_U 2  // 1, 2, 4 and 8 swept: 238, 230, 250 and 383 µs on terrain1m
+#endif
+static constexpr int kShadowU = ANE_SHADOW_U;  // candidates per lane and round
+
+template <bool COUNT>
+__device__ __forceinline__ void probe_shadow_group(const TraceArgs& a, int64_t first)
+{
+  __shared__ uint32_t key[256];
+  const SunGridParams* gq = a.sg.params;
+  const int tid = threadIdx.x, lane = tid & 63, wb = tid & ~63;
+  const int64_t i = first + tid;
+  const bool in = i < (int64_t)a.nrays;
+  const SceneK& sc = a.sc;
+  ane_hit h;
+  h.tri = -1;
+  h.u = h.v = 0.0f;
+  if (in) h = a.hits[i];
+  const bool has = in && h.tri >= 0;
+  f3 p = mk(0.0f, 0.0f, 0.0f);
+  uint32_t b = 0u, c = 0u;
+  if (has) {
+    HitGeom hg;
+    p = hit_geom_pos(sc, h.tri, h.u, h.v, hg);
+    const float X = comp(p, gq->kx) - gq->sx * comp(p, gq->kz);
+    const float Y = comp(p, gq->ky) - gq->sy * comp(p, gq->kz);
+    const int64_t cell =
+        (int64_t)sun_cell(Y, gq->y0, gq->ich, gq->gy) * gq->gx + sun_cell(X, gq->x0, gq->icw, gq->gx);
+    b = a.sg.start[cell];
+    c = a.sg.start[cell + 1] - b;
+  }
+  key[tid] = 2u * c + 1u;
+  // candidates before each ray of the wave
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t excl = incl - c;
+  RayPre r;
+  load_dirpre(r, a.dirpre + 3 * a.g.rays, p);  // the sun direction's ray setup
+  const float tmin = a.g.t_min, tmax = a.g.t_max;
+  // kShadowU candidates per lane and round, every load of the round issued
+  // before any test (lanes past the end load the last candidate and drop it)
+  constexpr int U = kShadowU;
+  for (uint32_t t0 = 0; t0 < total; t0 += 64u * U) {
+    uint32_t pos[U], e[U];
+    int jj[U];
+    f3 o[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = t0 + 64u * u + (uint32_t)lane;
+      const uint32_t tc = t < total ? t : total - 1u;
+      // the ray of candidate tc: the first lane whose inclusive count exceeds it
+      int j = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const uint32_t v = __shfl(incl, j + step - 1, 64);
+        if (v <= tc) j += step;
+      }
+      const uint32_t ej = __shfl(excl, j, 64), bj = __shfl(b, j, 64);
+      o[u] = mk(__shfl(p.x, j, 64), __shfl(p.y, j, 64), __shfl(p.z, j, 64));
+      jj[u] = t < total ? j : -1;
+      pos[u] = tc - ej;
+      e[u] = a.sg.list[bj + pos[u]];
+    }
+    float4 v0[U], v1[U], v2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = e[u] < (uint32_t)sc.ntri ? (int64_t)e[u] : 0;
+      v0[u] = sc.tris[3 * q];
+      v1[u] = sc.tris[3 * q + 1];
+      v2[u] = sc.tris[3 * q + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (jj[u] >= 0 && 2u * pos[u] < key[wb + jj[u]]) {
+        if (e[u] >= (uint32_t)sc.ntri) {
+          *a.overflow = 1u;
+          atomicMin(&key[wb + jj[u]], 2u * pos[u] + 1u);
+        } else {
+          float tt, uu, vv;
+          r.o = o[u];
+          if (tri_test(r, mk(v0[u].x, v0[u].y, v0[u].z), mk(v1[u].x, v1[u].y, v1[u].z),
+                       mk(v2[u].x, v2[u].y, v2[u].z), tmin, tmax, tt, uu, vv))
+            atomicMin(&key[wb + jj[u]], 2u * pos[u]);
+        }
+      }
+    }
+  }
+  const uint32_t k = key[tid];
+  if (in) a.shadow[i] = has ? (uint8_t)((k & 1u) ? 0 : 1) : (uint8_t)255;
+  if (COUNT) {
+    // one cell lookup per hit; the tests of the in-order walk; one atomic per block
+    __shared__ unsigned long long part[2][4];
+    unsigned long long sn = has ? 1ull : 0ull;
+    unsigned long long st = has ? (unsigned long long)((k & 1u) ? (k >> 1) : (k >> 1) + 1u) : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      sn += __shfl_xor(sn, d, 64);
+      st += __shfl_xor(st, d, 64);
+    }
+    if (lane == 0) {
+      part[0][tid >> 6] = sn;
+      part[1][tid >> 6] = st;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&a.stats[2], part[0][0] + part[0][1] + part[0][2] + part[0][3]);
+      atomicAdd(&a.stats[3], part[1][0] + part[1][1] + part[1][2] + part[1][3]);
+    }
+  }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_probe_shadow_grid(TraceArgs a)
+{
+  const SunGridParams* gq = a.sg.params;
+  if (gq->valid == 0u) return;  // no grid in use: the traversal ran the shadow rays
+  probe_shadow_group<COUNT>(a, (int64_t)blockIdx.x * 256);
+}
+
 static int g_lanes = 0;
 
 int trace_lanes_per_launch()
@@ -621,6 +739,16 @@ void launch_trace(const TraceArgs& a, bool any_hit, bool rays, bool count, hipSt
     if (count) ANE_TR(false, false, true); else ANE_TR(false, false, false);
   }
 #undef ANE_TR
+}
+
+void launch_probe_shadow_grid(const TraceArgs& a, bool count, hipStream_t s)
+{
+  if (!a.sg.list || a.nrays <= 0) return;
+  // a block per 256 output slots (blocks looping over several groups measured 1.5-4%
+  // slower: their waves' chains of dependent loads then run one group after another)
+  const dim3 grid((unsigned)(((int64_t)a.nrays + 255) / 256));
+  if (count) hipLaunchKernelGGL(k_probe_shadow_grid<true>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_probe_shadow_grid<false>, grid, dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

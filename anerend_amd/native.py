@@ -30,12 +30,12 @@ class SceneInfo(ctypes.Structure):
                 ("build_ms", ctypes.c_float), ("node_format", ctypes.c_int32)]
 
 
-STAGES = ("setup", "trace", "sungrid", "shade", "convolve", "translate")
+STAGES = ("setup", "trace", "sungrid", "shade", "convolve", "translate", "shadow")
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("last_ms", ctypes.c_float * 6), ("mean_ms", ctypes.c_float * 6),
-                ("calls", ctypes.c_uint64 * 6),
+    _fields_ = [("last_ms", ctypes.c_float * 7), ("mean_ms", ctypes.c_float * 7),
+                ("calls", ctypes.c_uint64 * 7),
                 ("primary_nodes", ctypes.c_uint64), ("primary_tris", ctypes.c_uint64),
                 ("shadow_nodes", ctypes.c_uint64), ("shadow_tris", ctypes.c_uint64),
                 ("rays", ctypes.c_uint64), ("device_bytes", ctypes.c_int64),

@@ -172,12 +172,13 @@ typedef struct ane_ddgi_stats {
   /* device time per stage from hipEvent pairs, ms: the last call and the mean
    * over all calls since the last ane_ddgi_stats_reset.  Stage order:
    * setup, trace (primary rays and, in the same launch, the shadow ray of
-   * every hit), sun grid (the build of the shadow rays' candidate lists, only in
-   * the updates that build one: a new scene version or sun), shade, convolve,
-   * translate. */
-  float last_ms[6];
-  float mean_ms[6];
-  uint64_t calls[6];
+   * every hit unless a sun grid serves them), sun grid (the build of the shadow
+   * rays' candidate lists, only in the updates that build one: a new scene
+   * version or sun), shade, convolve, translate, shadow (the shadow rays of the
+   * hits against their sun-grid cells, in the updates that have a grid). */
+  float last_ms[7];
+  float mean_ms[7];
+  uint64_t calls[7];
   /* traversal work of the last trace (only while counting is enabled) */
   uint64_t primary_nodes, primary_tris, shadow_nodes, shadow_tris;
   uint64_t rays;          /* primary rays of the last trace */
