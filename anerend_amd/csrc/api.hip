@@ -266,7 +266,7 @@ struct SunGridCache {
   uint32_t* cur = nullptr;    // [cells] scatter cursors
   uint32_t* list = nullptr;   // [cap] candidates
   uint32_t* scan = nullptr;   // scan scratch
-  float* est = nullptr;       // [2 * 2048] candidate-estimate partials
+  float* est = nullptr;       // candidate-estimate partials, sun_grid_estimate_words(ntri)
   int g = 0;
   uint32_t cap = 0;
   int64_t bytes = 0;          // device bytes of the buffers above
@@ -306,7 +306,7 @@ struct SunGridCache {
     ANE_TRY(dalloc(&cur, ncell, &bytes));
     ANE_TRY(dalloc(&list, capl, &bytes));
     ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, &bytes));
-    ANE_TRY(dalloc(&est, 2 * 2048, &bytes));
+    ANE_TRY(dalloc(&est, sun_grid_estimate_words(ntri), &bytes));
     g = gg;
     cap = (uint32_t)capl;
     return ANE_OK;

@@ -112,11 +112,12 @@ void launch_shadow_mask_grid(const float* depth, int W, int H, const float* ivp1
                              const SunGridK& sg, const SceneK& sc, float* vis, hipStream_t s);
 void launch_anyhit_grid(const float* orig, const float* dirs, int64_t n, const SunGridK& sg, const SceneK& sc,
                         float tmin, float tmax, ane_hit* hits, hipStream_t s);
-// est: [2 * 2048] partial sums of the candidate estimate
+// est: [sun_grid_estimate_words(ntri)] partial sums of the candidate estimate
+int sun_grid_estimate_words(int32_t ntri);
 void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
                            uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
                            float* est, hipStream_t s);
-// probe traces can take a sun grid (needs leaf batching)
+// probe traces can take a sun grid (k_probe_shadow_grid runs their shadow rays)
 bool trace_grid_supported();
 
 struct TraceArgs {

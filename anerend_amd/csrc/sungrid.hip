@@ -184,21 +184,70 @@ __device__ __forceinline__ bool sun_tri_overlaps_cell(const SunGridParams& g, co
   return true;
 }
 
+// Every (triangle, cell of its rectangle) pair of a wave's 64 triangles, spread
+// over the wave's lanes 64 at a time: f(j, cj, x, y) runs for the pair of lane
+// j's triangle cj and cell (x, y), in every lane that has a pair.  Lanes whose
+// `has` is false have no pairs.  Neighbouring lanes mostly take neighbouring
+// cells of one triangle, so the per-cell atomics of a wave rarely meet.
+template <class F>
+__device__ __forceinline__ void sun_wave_pairs(const CellRange& c, bool has, F&& f)
+{
+  const int lane = threadIdx.x & 63;
+  const int w = has ? c.cx1 - c.cx0 + 1 : 0;
+  const uint32_t r = has ? (uint32_t)w * (uint32_t)(c.cy1 - c.cy0 + 1) : 0u;
+  uint32_t incl = r;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t excl = incl - r;
+  for (uint32_t t0 = 0; t0 < total; t0 += 64u) {
+    const uint32_t t = t0 + (uint32_t)lane;
+    const uint32_t tc = t < total ? t : total - 1u;
+    int j = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const uint32_t v = __shfl(incl, j + step - 1, 64);
+      if (v <= tc) j += step;
+    }
+    CellRange cj;
+    cj.cx0 = __shfl(c.cx0, j, 64);
+    cj.cy0 = __shfl(c.cy0, j, 64);
+    cj.cx1 = __shfl(c.cx1, j, 64);
+    cj.cy1 = __shfl(c.cy1, j, 64);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      cj.px[k] = __shfl(c.px[k], j, 64);
+      cj.py[k] = __shfl(c.py[k], j, 64);
+    }
+    const uint32_t q = tc - __shfl(excl, j, 64);
+    const uint32_t wj = (uint32_t)(cj.cx1 - cj.cx0 + 1);
+    const int y = cj.cy0 + (int)(q / wj), x = cj.cx0 + (int)(q % wj);
+    if (t < total) f(j, cj, x, y);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t ntri, SunGridParams* gp,
                                                    uint32_t* cnt)
 {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const SunGridParams g = *gp;
-  if (p >= ntri || !g.valid) return;
-  const CellRange c = sun_tri_cells(g, tris, p);
-  const int64_t n = (int64_t)(c.cx1 - c.cx0 + 1) * (c.cy1 - c.cy0 + 1);
-  if (n > kMaxCellsPerTri) {
-    gp->valid = 0u;  // a triangle too large for the grid: the traversal takes the shadow rays
-    return;
+  if (!g.valid) return;
+  CellRange c{};
+  bool has = p < ntri;
+  if (has) {
+    c = sun_tri_cells(g, tris, p);
+    const int64_t n = (int64_t)(c.cx1 - c.cx0 + 1) * (c.cy1 - c.cy0 + 1);
+    if (n > kMaxCellsPerTri) {
+      gp->valid = 0u;  // a triangle too large for the grid: the traversal takes the shadow rays
+      has = false;
+    }
   }
-  for (int y = c.cy0; y <= c.cy1; ++y)
-    for (int x = c.cx0; x <= c.cx1; ++x)
-      if (sun_tri_overlaps_cell(g, c, x, y)) atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
+  sun_wave_pairs(c, has, [&](int, const CellRange& cj, int x, int y) {
+    if (sun_tri_overlaps_cell(g, cj, x, y)) atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
+  });
 }
 
 // after the scan: start[g.gx * g.gy] entries in total; the cursors begin at the starts
@@ -210,53 +259,42 @@ __global__ __launch_bounds__(256) void k_sun_cursor(const uint32_t* start, int64
   if (i == 0 && start[ncell] > cap) gp->valid = 0u;
 }
 
+// The scatter of the candidate lists, and the candidate estimate from the same
+// pairs: a shadow ray starts on a triangle and tests the candidates of the cell
+// its origin falls in, one of the triangle's cells; per triangle the mean count
+// of its cells (integers: the sums are exact in any order), then the mean over
+// the triangles, summed per block here and over the blocks by k_sun_decide.
 __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t ntri, const SunGridParams* gp,
-                                                     uint32_t* cur, uint32_t* list, uint32_t cap)
+                                                     const uint32_t* start, uint32_t* cur, uint32_t* list,
+                                                     uint32_t cap, float* part)
 {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const SunGridParams g = *gp;
-  if (p >= ntri || !g.valid) return;
-  const CellRange c = sun_tri_cells(g, tris, p);
-  for (int y = c.cy0; y <= c.cy1; ++y)
-    for (int x = c.cx0; x <= c.cx1; ++x)
-      if (sun_tri_overlaps_cell(g, c, x, y)) {
-        const uint32_t k = atomicAdd(&cur[(int64_t)y * g.gx + x], 1u);
-        if (k < cap) list[k] = (uint32_t)p;
-      }
-}
-
-static constexpr int kEstBlocks = 2048;
-
-// Candidate estimate: a shadow ray starts on a triangle and tests the
-// candidates of the cell its origin falls in, one of the triangle's cells; per
-// triangle the mean count of its cells, then the mean over the triangles.
-__global__ __launch_bounds__(256) void k_sun_estimate(const float4* tris, int32_t ntri, const SunGridParams* gp,
-                                                      const uint32_t* start, float* part)
-{
+  __shared__ float ts[256], tk[256];  // per lane's triangle: the counts of its cells, and their number
   __shared__ float rs[256], rn[256];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = threadIdx.x, wb = t & ~63;
   const SunGridParams g = *gp;
-  float sum = 0.0f, num = 0.0f;
-  if (g.valid) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ntri;
-         p += (int64_t)gridDim.x * blockDim.x) {
-      const CellRange c = sun_tri_cells(g, tris, p);
-      float s = 0.0f, k = 0.0f;
-      for (int y = c.cy0; y <= c.cy1; ++y)
-        for (int x = c.cx0; x <= c.cx1; ++x)
-          if (sun_tri_overlaps_cell(g, c, x, y)) {
-            const int64_t cell = (int64_t)y * g.gx + x;
-            s += (float)(start[cell + 1] - start[cell]);
-            k += 1.0f;
-          }
-      if (k > 0.0f) {
-        sum += s / k;
-        num += 1.0f;
-      }
-    }
+  if (!g.valid) {
+    if (t == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0f;
+    return;
   }
-  const int t = threadIdx.x;
-  rs[t] = sum;
-  rn[t] = num;
+  const bool has = p < ntri;
+  CellRange c{};
+  if (has) c = sun_tri_cells(g, tris, p);
+  ts[t] = 0.0f;
+  tk[t] = 0.0f;
+  const int64_t p0 = p - (t & 63);  // the wave's first triangle
+  sun_wave_pairs(c, has, [&](int j, const CellRange& cj, int x, int y) {
+    if (sun_tri_overlaps_cell(g, cj, x, y)) {
+      const int64_t cell = (int64_t)y * g.gx + x;
+      const uint32_t k = atomicAdd(&cur[cell], 1u);
+      if (k < cap) list[k] = (uint32_t)(p0 + j);
+      atomicAdd(&ts[wb + j], (float)(start[cell + 1] - start[cell]));
+      atomicAdd(&tk[wb + j], 1.0f);
+    }
+  });
+  const float k = tk[t];
+  rs[t] = (has && k > 0.0f) ? ts[t] / k : 0.0f;
+  rn[t] = (has && k > 0.0f) ? 1.0f : 0.0f;
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
     if (t < h) {
@@ -271,12 +309,12 @@ __global__ __launch_bounds__(256) void k_sun_estimate(const float4* tris, int32_
   }
 }
 
-__global__ __launch_bounds__(256) void k_sun_decide(const float* part, SunGridParams* gp)
+__global__ __launch_bounds__(256) void k_sun_decide(const float* part, int nblocks, SunGridParams* gp)
 {
   __shared__ float rs[256], rn[256];
   const int t = threadIdx.x;
   float s = 0.0f, n = 0.0f;
-  for (int b = t; b < kEstBlocks; b += 256) {
+  for (int b = t; b < nblocks; b += 256) {
     s += part[2 * b];
     n += part[2 * b + 1];
   }
@@ -329,12 +367,13 @@ void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridPara
   exclusive_scan(cnt, ncell + 1, scan_scratch, s);
   hipLaunchKernelGGL(k_sun_cursor, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
                      (const uint32_t*)cnt, ncell, cap, d_params, cur);
-  hipLaunchKernelGGL(k_sun_estimate, dim3(kEstBlocks), dim3(256), 0, s, sc.tris, sc.ntri,
-                     (const SunGridParams*)d_params, (const uint32_t*)cnt, est);
-  hipLaunchKernelGGL(k_sun_decide, dim3(1), dim3(256), 0, s, (const float*)est, d_params);
-  if (sc.ntri > 0)
+  if (sc.ntri > 0) {
     hipLaunchKernelGGL(k_sun_scatter, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri,
-                       (const SunGridParams*)d_params, cur, list, cap);
+                       (const SunGridParams*)d_params, (const uint32_t*)cnt, cur, list, cap, est);
+    hipLaunchKernelGGL(k_sun_decide, dim3(1), dim3(256), 0, s, (const float*)est, (int)tb, d_params);
+  }
 }
+
+int sun_grid_estimate_words(int32_t ntri) { return 2 * (int)((std::max(ntri, 1) + 255) / 256); }
 
 }  // namespace ane

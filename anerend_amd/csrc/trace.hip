@@ -92,7 +92,7 @@ static constexpr int kLeafBatchProbe = ANE_TRACE_LEAFBATCH;
 static constexpr int kLeafBatchRays = ANE_TRACE_LEAFBATCH_RAYS;
 static constexpr int kShadeUnroll = ANE_SHADE_UNROLL;
 
-bool trace_grid_supported() { return kLeafBatchProbe > 0; }
+bool trace_grid_supported() { return true; }
 
 int trace_stack_lds() { return kSS; }
 int trace_block_size() { return kTB; }
