@@ -522,14 +522,26 @@ static bool scene_wants_qnodes(const ane_scene* s)
   return s->node_format == ANE_NODES_AUTO && (int64_t)s->ntri >= ANE_NODES_QUANTIZED_MIN_TRIS;
 }
 
-static int scene_apply_node_format(ane_scene* s, hipStream_t st)
+// The quantized records a fit should write (allocated on first use), or null
+// when the scene traverses the full nodes.
+static int scene_fit_qnodes(ane_scene* s, uint4** out)
+{
+  *out = nullptr;
+  if (!scene_wants_qnodes(s)) return ANE_OK;
+  if (!s->d_qnodes) ANE_TRY(dalloc(&s->d_qnodes, 2 * (int64_t)(s->ntri - 1), &s->bytes));
+  *out = s->d_qnodes;
+  return ANE_OK;
+}
+
+// fitted: the fit that just ran wrote the quantized records (scene_fit_qnodes)
+static int scene_apply_node_format(ane_scene* s, hipStream_t st, bool fitted = false)
 {
   if (!scene_wants_qnodes(s)) {
     s->view.qnodes = nullptr;
     return ANE_OK;
   }
   if (!s->d_qnodes) ANE_TRY(dalloc(&s->d_qnodes, 2 * (int64_t)(s->ntri - 1), &s->bytes));
-  launch_quantize_nodes(s->d_nodes, s->ntri, s->d_qnodes, st);
+  if (!fitted) launch_quantize_nodes(s->d_nodes, s->ntri, s->d_qnodes, st);
   s->view.qnodes = s->d_qnodes;
   return ANE_OK;
 }
@@ -550,9 +562,11 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
                       s->d_cross, s->d_top, s->d_units, s->d_unit_root, s->d_stray, &s->cross_begin, &s->nunits,
                       &s->nstray, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH fit plan failed");
-  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
+  uint4* qn = nullptr;
+  ANE_TRY(scene_fit_qnodes(s, &qn));
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
              s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, s->d_uheight, st);
-  ANE_TRY(scene_apply_node_format(s, st));
+  ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));
   uint32_t* d_depth = s->d_info + 132;
   if (launch_tree_depth(s->d_topo, n, s->d_unit_root, s->d_uheight, s->nunits, s->d_top, s->d_uscratch,
                         s->d_ndepth, d_depth, st) != 0)
@@ -577,10 +591,12 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   // a refit keeps the Morton order: the fit makes the leaf-order triangles straight
   // from the triangle records (no centroid bounds, no input-order world copy)
   const SceneK view = s->view;
-  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
+  uint4* qn = nullptr;
+  ANE_TRY(scene_fit_qnodes(s, &qn));
+  launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
              s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, nullptr, st);
   s->world_stale = true;
-  ANE_TRY(scene_apply_node_format(s, st));
+  ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));
   ANE_CHECK_HIP(hipGetLastError());
   ANE_CHECK_HIP(hipEventRecord(s->ev1, st));
   // the refit stays asynchronous; its device time is read when it is asked for

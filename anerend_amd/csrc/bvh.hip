@@ -625,8 +625,9 @@ void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t 
 }
 
 // Fit one internal node from its (already final) children: child boxes into
-// the traversal record, the union into node_box.
-__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, int i)
+// the traversal record (and its quantized record when qn is set), the union
+// into node_box.
+__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, uint4* qn, int i)
 {
   NodeBuild* nd = &nodes[i];
   const int L = nd->d.x, R = nd->d.y;
@@ -636,6 +637,7 @@ __device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, f
   nd->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
   nd->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
   nd->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+  if (qn) quantize_node(l0, h0, l1, h1, L, R, qn + 2 * (int64_t)i);
   float* nb = node_box + 6 * (int64_t)i;
   nb[0] = fminf(l0[0], l1[0]); nb[1] = fminf(l0[1], l1[1]); nb[2] = fminf(l0[2], l1[2]);
   nb[3] = fmaxf(h0[0], h1[0]); nb[4] = fmaxf(h0[1], h1[1]); nb[5] = fmaxf(h0[2], h1[2]);
@@ -680,8 +682,9 @@ __device__ __forceinline__ void span_child_box(const float* lb, const float* ib,
 // flatten's transform) and written to tris.
 template <bool FROM_REC>
 __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* ids, float4* tris, NodeBuild* nodes,
-                                                    float* node_box, const int4* topo, const int32_t* leaf_parent,
-                                                    const int2* units, int n, uint8_t* uheight)
+                                                    float* node_box, uint4* qn, const int4* topo,
+                                                    const int32_t* leaf_parent, const int2* units, int n,
+                                                    uint8_t* uheight)
 {
   extern __shared__ int4 span_lds[];
   int4* nd = span_lds;                                       // [kSpan] topology of nodes s..
@@ -762,6 +765,19 @@ __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* i
         if (uheight) uheight[blockIdx.x] = h;
       }
       p = up;  // -1 past the root
+    }
+  }
+  if (qn) {
+    // the quantized records of the treelet's nodes, all at once from the boxes in
+    // LDS once every walk has ended (off the walks' serial path)
+    __syncthreads();
+    for (int k = threadIdx.x; k < ni; k += blockDim.x) {
+      const int4 d = nd[k];
+      if (!node_inside(s + k, d.w, s, e)) continue;
+      float l0[3], h0[3], l1[3], h1[3];
+      span_child_box(lb, ib, d.x, s, l0, h0);
+      span_child_box(lb, ib, d.y, s, l1, h1);
+      quantize_node(l0, h0, l1, h1, d.x, d.y, qn + 2 * (int64_t)(s + k));
     }
   }
 }
@@ -904,12 +920,12 @@ __global__ __launch_bounds__(256) void k_refit_stray(SceneK sc, const uint32_t* 
 // One depth level of the crossing nodes: every child is a leaf, inside a span
 // (fitted by k_fit_span) or a crossing node of a deeper, finished level.
 __global__ __launch_bounds__(256) void k_fit_level(const float4* tris, NodeBuild* nodes,
-                                                   float* node_box, const int32_t* order, int begin,
+                                                   float* node_box, uint4* qn, const int32_t* order, int begin,
                                                    int count)
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
-  fit_node(tris, nodes, node_box, order[begin + k]);
+  fit_node(tris, nodes, node_box, qn, order[begin + k]);
 }
 
 // The top levels (at most 2^d nodes at depth d) in one workgroup, deepest first.
@@ -920,11 +936,11 @@ struct TopLevels {
 };
 
 __global__ __launch_bounds__(1024) void k_fit_top(const float4* tris, NodeBuild* nodes,
-                                                  float* node_box, const int32_t* order, TopLevels t)
+                                                  float* node_box, uint4* qn, const int32_t* order, TopLevels t)
 {
   for (int l = t.num - 1; l >= 0; --l) {
     for (int k = threadIdx.x; k < t.count[l]; k += blockDim.x)
-      fit_node(tris, nodes, node_box, order[t.begin[l] + k]);
+      fit_node(tris, nodes, node_box, qn, order[t.begin[l] + k]);
     __threadfence_block();
     __syncthreads();
   }
@@ -1033,7 +1049,7 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
   return 0;
 }
 
-void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
+void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
                 const uint32_t* ids, uint8_t* uheight, hipStream_t s)
@@ -1057,10 +1073,11 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const in
   if (nunits > 0) {
     if (refit)
       hipLaunchKernelGGL(k_fit_span<true>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, *refit, ids, tris,
-                         nodes, node_box, topo, leaf_parent, units, n, uheight);
+                         nodes, node_box, qnodes, topo, leaf_parent, units, n, uheight);
     else
       hipLaunchKernelGGL(k_fit_span<false>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, none,
-                         (const uint32_t*)nullptr, tris, nodes, node_box, topo, leaf_parent, units, n, uheight);
+                         (const uint32_t*)nullptr, tris, nodes, node_box, qnodes, topo, leaf_parent, units, n,
+                         uheight);
   }
   // the crossing nodes, deepest level first: a launch per level below the top
   // ones, then the top levels in one workgroup
@@ -1069,7 +1086,7 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const in
     const int cnt = cross_begin[d + 1] - cross_begin[d];
     if (cnt > 0)
       hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, (const float4*)tris, nodes,
-                         node_box, cross, cross_begin[d], cnt);
+                         node_box, qnodes, cross, cross_begin[d], cnt);
   }
   TopLevels t{};
   t.num = levels < kTopLevels ? levels : kTopLevels;
@@ -1080,7 +1097,7 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const in
     top += t.count[d];
   }
   if (top > 0)
-    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (const float4*)tris, nodes, node_box, cross, t);
+    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (const float4*)tris, nodes, node_box, qnodes, cross, t);
 }
 
 

@@ -52,7 +52,9 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
 // records and instances of that scene view (ids: the leaf order), else tris is
 // read as it is.
 // uheight (nullable): the height of each treelet's root, by treelet.
-void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, const int4* topo,
+// qnodes (nullable): the quantized records are written too, from the same child
+// boxes (the bytes launch_quantize_nodes makes from the fitted nodes).
+void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
                 const uint32_t* ids, uint8_t* uheight, hipStream_t s);
