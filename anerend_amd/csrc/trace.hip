@@ -584,6 +584,79 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
 #endif
 static constexpr int kShadowU = ANE_SHADOW_U;  // candidates per lane and round
 
+// The candidates of a wave's 64 rays spread over its lanes: lane l's ray (origin
+// p, direction of r, list [b, b + c) of the sun grid) has its candidates tested
+// with the watertight any-hit test, 64 at a time, ray after ray and in list order
+// within a ray (kShadowU per lane and round, every load of a round issued before
+// any test).  key[l] (this wave's 64 LDS slots, set to 2 c + 1 by the caller)
+// ends as the earliest event of lane l's list in list order: 2 pos for a hit,
+// 2 pos + 1 for an entry out of range when STOP_ON_BAD (which also raises
+// *overflow; otherwise such entries are skipped).  Tests after a known event are
+// skipped.  All lanes of the wave must call it (shuffles).
+template <bool STOP_ON_BAD>
+__device__ __forceinline__ void grid_wave_events(const SunGridK& sg, const float4* tris, int32_t ntri, RayPre r,
+                                                 f3 p, uint32_t b, uint32_t c, float tmin, float tmax,
+                                                 uint32_t* key, uint32_t* overflow)
+{
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t excl = incl - c;
+  constexpr int U = kShadowU;
+  for (uint32_t t0 = 0; t0 < total; t0 += 64u * U) {
+    uint32_t pos[U], e[U];
+    int jj[U];
+    f3 o[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = t0 + 64u * u + (uint32_t)lane;
+      const uint32_t tc = t < total ? t : total - 1u;
+      // the ray of candidate tc: the first lane whose inclusive count exceeds it
+      int j = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const uint32_t v = __shfl(incl, j + step - 1, 64);
+        if (v <= tc) j += step;
+      }
+      const uint32_t ej = __shfl(excl, j, 64), bj = __shfl(b, j, 64);
+      o[u] = mk(__shfl(p.x, j, 64), __shfl(p.y, j, 64), __shfl(p.z, j, 64));
+      jj[u] = t < total ? j : -1;
+      pos[u] = tc - ej;
+      e[u] = sg.list[bj + pos[u]];
+    }
+    float4 v0[U], v1[U], v2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = e[u] < (uint32_t)ntri ? (int64_t)e[u] : 0;
+      v0[u] = tris[3 * q];
+      v1[u] = tris[3 * q + 1];
+      v2[u] = tris[3 * q + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (jj[u] >= 0 && 2u * pos[u] < key[jj[u]]) {
+        if (e[u] >= (uint32_t)ntri) {
+          if (STOP_ON_BAD) {
+            *overflow = 1u;
+            atomicMin(&key[jj[u]], 2u * pos[u] + 1u);
+          }
+        } else {
+          float tt, uu, vv;
+          r.o = o[u];
+          if (tri_test(r, mk(v0[u].x, v0[u].y, v0[u].z), mk(v1[u].x, v1[u].y, v1[u].z),
+                       mk(v2[u].x, v2[u].y, v2[u].z), tmin, tmax, tt, uu, vv))
+            atomicMin(&key[jj[u]], 2u * pos[u]);
+        }
+      }
+    }
+  }
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void probe_shadow_group(const TraceArgs& a, int64_t first)
 {
@@ -611,66 +684,9 @@ __device__ __forceinline__ void probe_shadow_group(const TraceArgs& a, int64_t f
     c = a.sg.start[cell + 1] - b;
   }
   key[tid] = 2u * c + 1u;
-  // candidates before each ray of the wave
-  uint32_t incl = c;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
-  const uint32_t total = __shfl(incl, 63, 64);
-  const uint32_t excl = incl - c;
   RayPre r;
   load_dirpre(r, a.dirpre + 3 * a.g.rays, p);  // the sun direction's ray setup
-  const float tmin = a.g.t_min, tmax = a.g.t_max;
-  // kShadowU candidates per lane and round, every load of the round issued
-  // before any test (lanes past the end load the last candidate and drop it)
-  constexpr int U = kShadowU;
-  for (uint32_t t0 = 0; t0 < total; t0 += 64u * U) {
-    uint32_t pos[U], e[U];
-    int jj[U];
-    f3 o[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t t = t0 + 64u * u + (uint32_t)lane;
-      const uint32_t tc = t < total ? t : total - 1u;
-      // the ray of candidate tc: the first lane whose inclusive count exceeds it
-      int j = 0;
-#pragma unroll
-      for (int step = 32; step >= 1; step >>= 1) {
-        const uint32_t v = __shfl(incl, j + step - 1, 64);
-        if (v <= tc) j += step;
-      }
-      const uint32_t ej = __shfl(excl, j, 64), bj = __shfl(b, j, 64);
-      o[u] = mk(__shfl(p.x, j, 64), __shfl(p.y, j, 64), __shfl(p.z, j, 64));
-      jj[u] = t < total ? j : -1;
-      pos[u] = tc - ej;
-      e[u] = a.sg.list[bj + pos[u]];
-    }
-    float4 v0[U], v1[U], v2[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t q = e[u] < (uint32_t)sc.ntri ? (int64_t)e[u] : 0;
-      v0[u] = sc.tris[3 * q];
-      v1[u] = sc.tris[3 * q + 1];
-      v2[u] = sc.tris[3 * q + 2];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (jj[u] >= 0 && 2u * pos[u] < key[wb + jj[u]]) {
-        if (e[u] >= (uint32_t)sc.ntri) {
-          *a.overflow = 1u;
-          atomicMin(&key[wb + jj[u]], 2u * pos[u] + 1u);
-        } else {
-          float tt, uu, vv;
-          r.o = o[u];
-          if (tri_test(r, mk(v0[u].x, v0[u].y, v0[u].z), mk(v1[u].x, v1[u].y, v1[u].z),
-                       mk(v2[u].x, v2[u].y, v2[u].z), tmin, tmax, tt, uu, vv))
-            atomicMin(&key[wb + jj[u]], 2u * pos[u]);
-        }
-      }
-    }
-  }
+  grid_wave_events<true>(a.sg, sc.tris, sc.ntri, r, p, b, c, a.g.t_min, a.g.t_max, key + wb, a.overflow);
   const uint32_t k = key[tid];
   if (in) a.shadow[i] = has ? (uint8_t)((k & 1u) ? 0 : 1) : (uint8_t)255;
   if (COUNT) {
@@ -1447,29 +1463,39 @@ void launch_shadow_resolve(const float* depth, const ane_hit* hits, int64_t n, f
                      skip_if_set);
 }
 
-// RT hard shadows through a sun grid built for the rays' direction: one thread
-// per pixel, the world position and ray setup of k_shadow_rays and the trace,
-// the cell's candidates in the watertight test (the any-hit answer of the
-// traversal); nothing unless the grid is valid
+// RT hard shadows through a sun grid built for the rays' direction: a thread per
+// pixel sets up its ray (the world position and ray setup of k_shadow_rays and the
+// trace) and looks up its cell; the candidates of a wave's 64 pixels are then
+// spread over its lanes (grid_wave_events), each in the watertight test: any hit
+// is the any-hit answer of the traversal.  Nothing unless the grid is valid.
 __global__ __launch_bounds__(256) void k_shadow_mask_grid(const float* depth, int W, int H, Mat4 ivp, f3 dir,
                                                           SunGridK sg, SceneK sc, float* vis)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)W * H) return;
+  __shared__ uint32_t key[256];
   const SunGridParams g = *sg.params;
   if (!g.valid) return;
-  const float d = depth[i];
-  const bool none = d == 1.0f || (dir.x == 0.0f && dir.y == 0.0f && dir.z == 0.0f) ||
-                    !__builtin_isfinite(dir.x) || !__builtin_isfinite(dir.y) || !__builtin_isfinite(dir.z);
-  if (none) {
-    vis[i] = 1.0f;
-    return;
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+  const bool in = i < (int64_t)W * H;
+  const bool dir_ok = !(dir.x == 0.0f && dir.y == 0.0f && dir.z == 0.0f) && __builtin_isfinite(dir.x) &&
+                      __builtin_isfinite(dir.y) && __builtin_isfinite(dir.z);
+  const float d = in ? depth[i] : 1.0f;
+  const bool ray = in && d != 1.0f && dir_ok;
+  f3 p = mk(0.0f, 0.0f, 0.0f);
+  uint32_t b = 0u, c = 0u;
+  if (ray) {
+    p = pixel_world_pos(i, W, H, d, ivp);
+    const float X = comp(p, g.kx) - g.sx * comp(p, g.kz);
+    const float Y = comp(p, g.ky) - g.sy * comp(p, g.kz);
+    const int64_t cell = (int64_t)sun_cell(Y, g.y0, g.ich, g.gy) * g.gx + sun_cell(X, g.x0, g.icw, g.gx);
+    b = sg.start[cell];
+    c = sg.start[cell + 1] - b;
   }
-  const f3 p = pixel_world_pos(i, W, H, d, ivp);
+  key[tid] = 2u * c + 1u;
   RayPre r;
-  ray_setup(r, p, dir);
-  ane_hit h;
-  vis[i] = grid_any_hit(sg, g, sc.tris, sc.ntri, r, p, 0.1f, 50.0f, &h) ? 0.0f : 1.0f;  // shadow.rgen:66-67
+  ray_setup(r, p, dir_ok ? dir : mk(0.0f, 0.0f, 1.0f));
+  grid_wave_events<false>(sg, sc.tris, sc.ntri, r, p, b, c, 0.1f, 50.0f, key + (tid & ~63), nullptr);
+  if (in) vis[i] = (ray && !(key[tid] & 1u)) ? 0.0f : 1.0f;  // shadow.rgen:66-67
 }
 
 // any hit of caller rays along the grid's direction (hits: tri -1 on a miss)
