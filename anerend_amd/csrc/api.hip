@@ -247,7 +247,11 @@ using namespace ane;
 // for any-hit rays that all share it, and the key they were built for.  A key
 // is first only remembered; the second request with it builds the grid on the
 // request's stream, so a direction or scene that changes with every call keeps
-// the traversal and never pays a build.
+// the traversal and never pays a build.  One exception: when a grid of the same
+// scene version (another direction) was built and found valid, and the request
+// has at least ANE_SUNGRID_EAGER_RAYS rays per triangle (a build costs about as
+// much as the traversal of 5 probe rays per triangle saves), the first request
+// with a new direction builds: a sun that moves every frame over a static scene.
 // ===========================================================================
 #ifndef ANE_SUNGRID_DENSITY
 #define ANE_SUNGRID_DENSITY 1.0  // cells per triangle
@@ -258,6 +262,9 @@ using namespace ane;
 #ifndef ANE_SUNGRID_EAGER
 #define ANE_SUNGRID_EAGER 0  // 1: build on the first request with a key
 #endif
+#ifndef ANE_SUNGRID_EAGER_RAYS
+#define ANE_SUNGRID_EAGER_RAYS 6  // rays per triangle from which a new direction alone builds at once
+#endif
 
 struct SunGridCache {
   SunGridParams* params = nullptr;
@@ -266,7 +273,8 @@ struct SunGridCache {
   uint32_t* cur = nullptr;    // [cells] scatter cursors
   uint32_t* list = nullptr;   // [cap] candidates
   uint32_t* scan = nullptr;   // scan scratch
-  float* est = nullptr;       // candidate-estimate partials, sun_grid_estimate_words(ntri)
+  float* est = nullptr;       // build scratch, sun_grid_scratch_words(ntri)
+  int64_t est_words = 0;
   int g = 0;
   uint32_t cap = 0;
   int64_t bytes = 0;          // device bytes of the buffers above
@@ -281,14 +289,50 @@ struct SunGridCache {
              dir[1] == o.dir[1] && dir[2] == o.dir[2];
     }
   } built, seen;
+  // the validity flag of a build, read back without blocking: copied to pinned
+  // memory behind an event (not under stream capture) and taken once the event
+  // has completed.  `known` is the build whose flag last_valid holds; a newer
+  // build of the same scene version is predicted to be valid alike.
+  uint32_t* h_valid = nullptr;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+  Key pend, known;
+  int last_valid = -1;
 
   void release()
   {
     dfree(params); dfree(acc); dfree(start); dfree(cur); dfree(list); dfree(scan); dfree(est);
     g = 0;
     cap = 0;
+    est_words = 0;
     bytes = 0;
     built = Key{};
+    pending = false;
+    known = Key{};
+    last_valid = -1;
+  }
+  // release() and the host-side read-back state (teardown)
+  void destroy()
+  {
+    release();
+    if (ev) (void)hipEventDestroy(ev);
+    if (h_valid) (void)hipHostFree(h_valid);
+    ev = nullptr;
+    h_valid = nullptr;
+  }
+  void poll()
+  {
+    if (!pending) return;
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipErrorNotReady) {
+      (void)hipGetLastError();  // not an error: the copy has not landed yet
+      return;
+    }
+    pending = false;
+    if (q == hipSuccess) {
+      last_valid = *reinterpret_cast<volatile uint32_t*>(h_valid) ? 1 : 0;
+      known = pend;
+    }
   }
   // device buffers for a scene of ntri triangles: about ANE_SUNGRID_DENSITY cells
   // per triangle (at most 4096 x 4096), ANE_SUNGRID_CAPACITY entries per triangle
@@ -297,7 +341,8 @@ struct SunGridCache {
     int gg = (int)std::ceil(std::sqrt((double)std::max(ntri, 1) * ANE_SUNGRID_DENSITY));
     gg = std::min(std::max(gg, 16), 4096);
     const int64_t capl = std::min<int64_t>((int64_t)ANE_SUNGRID_CAPACITY * ntri + 1024, (int64_t)1 << 30);
-    if (gg <= g && capl <= (int64_t)cap) return ANE_OK;
+    const int64_t ew = sun_grid_scratch_words(ntri);
+    if (gg <= g && capl <= (int64_t)cap && ew <= est_words) return ANE_OK;
     release();
     const int64_t ncell = (int64_t)gg * gg;
     ANE_TRY(dalloc(&params, 1, &bytes));
@@ -306,7 +351,8 @@ struct SunGridCache {
     ANE_TRY(dalloc(&cur, ncell, &bytes));
     ANE_TRY(dalloc(&list, capl, &bytes));
     ANE_TRY(dalloc(&scan, (ncell + 1) / 1024 + 256, &bytes));
-    ANE_TRY(dalloc(&est, sun_grid_estimate_words(ntri), &bytes));
+    ANE_TRY(dalloc(&est, ew, &bytes));
+    est_words = ew;
     g = gg;
     cap = (uint32_t)capl;
     return ANE_OK;
@@ -318,8 +364,9 @@ struct SunGridCache {
   // watertight test would touch on an edge, so a grid's answer could differ
   // from the traversal's there.
   // timer (nullable) brackets a build's launches on the stream
+  // rays: the rays of the request (the eager-build rule)
   int update(const SceneK& view, int32_t ntri, const void* scene, uint64_t version, const float* dir3,
-             hipStream_t st, bool* use, StageTimer* timer = nullptr)
+             int64_t rays, hipStream_t st, bool* use, StageTimer* timer = nullptr)
   {
     *use = false;
     for (int k = 0; k < 3; ++k)
@@ -333,7 +380,10 @@ struct SunGridCache {
       *use = true;
       return ANE_OK;
     }
-    if (!ANE_SUNGRID_EAGER && !(seen == k)) {
+    poll();
+    const bool eager = known.set && known.scene == scene && known.version == version && last_valid == 1 &&
+                       rays >= (int64_t)ANE_SUNGRID_EAGER_RAYS * ntri;
+    if (!ANE_SUNGRID_EAGER && !eager && !(seen == k)) {
       seen = k;
       return ANE_OK;
     }
@@ -345,6 +395,16 @@ struct SunGridCache {
     if (timer) ANE_TRY(timer->end(st));
     ANE_CHECK_HIP(hipGetLastError());
     built = k;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ANE_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      if (!h_valid) ANE_CHECK_HIP(hipHostMalloc((void**)&h_valid, sizeof(uint32_t), hipHostMallocDefault));
+      if (!ev) ANE_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      ANE_CHECK_HIP(hipMemcpyAsync(h_valid, &params->valid, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      ANE_CHECK_HIP(hipEventRecord(ev, st));
+      pend = k;
+      pending = true;
+    }
     *use = true;
     return ANE_OK;
   }
@@ -459,7 +519,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_lt_dist); dfree(s->d_lt_dirs); dfree(s->d_lt_wt); dfree(s->d_lt_idx); dfree(s->d_lt_order); dfree(s->d_lt_work);
   dfree(s->d_lt_stats);
   dfree(s->d_lt_ov); dfree(s->d_lt_spill);
-  s->sgc.release();
+  s->sgc.destroy();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (int k = 0; k < 2; ++k) {
@@ -918,7 +978,7 @@ static void ctx_free(ane_ddgi_ctx* c)
   dfree(c->d_stage); dfree(c->d_work); dfree(c->d_stats); dfree(c->d_overflow); dfree(c->d_spill);
   dfree(c->d_tiles);
   dfree(c->d_baked);
-  c->sgc.release();
+  c->sgc.destroy();
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (int k = 0; k < 2; ++k) { dfree(c->d_mr_orig[k]); dfree(c->d_mr_dir[k]); dfree(c->d_mr_hits[k]); }
   dfree(c->d_mr_work); dfree(c->d_mr_stats); dfree(c->d_mr_ov); dfree(c->d_mr_spill);
@@ -1200,7 +1260,7 @@ extern "C" int ane_ddgi_trace(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddg
   a.work = c->d_work;
   if (sun_grid_wanted(c)) {
     bool use = false;
-    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use, &c->timers[kStSunGrid]));
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, a.nrays, st, &use, &c->timers[kStSunGrid]));
     if (use) a.sg = c->sgc.view();
   }
   // primary rays in one persistent launch (with the shadow rays of their hits
@@ -1287,7 +1347,7 @@ int ddgi_update_front(ane_ddgi_ctx* c, const ane_scene* s, const ane_ddgi_frame*
   a.work = c->d_work;
   if (sun_grid_wanted(c)) {
     bool use = false;
-    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, st, &use));
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, a.sun, a.nrays, st, &use));
     if (use) a.sg = c->sgc.view();
   }
   launch_trace(a, false, false, c->counting, st);
@@ -1604,7 +1664,7 @@ extern "C" int ane_ddgi_mirror_reflections(ane_ddgi_ctx* c, const ane_scene* s, 
   const uint32_t* skip = nullptr;
   if (!counters4 && sun_grid_wanted(c)) {
     bool use = false;
-    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, sun3, st, &use));
+    ANE_TRY(c->sgc.update(s->view, s->ntri, s, s->version, sun3, n, st, &use));
     if (use) {
       launch_anyhit_grid(c->d_mr_orig[1], c->d_mr_dir[1], n, c->sgc.view(), s->view, 0.1f, 50.0f, c->d_mr_hits[1],
                          st);
@@ -1860,7 +1920,7 @@ extern "C" int ane_scene_shadow_mask(ane_scene* s, const float* depth, int32_t w
   if (!counters2) {
     const float dir[3] = {-light3[0], -light3[1], -light3[2]};
     bool use = false;
-    ANE_TRY(s->sgc.update(s->view, s->ntri, s, s->version, dir, st, &use));
+    ANE_TRY(s->sgc.update(s->view, s->ntri, s, s->version, dir, (int64_t)width * height, st, &use));
     if (use) {
       launch_shadow_mask_grid(depth, width, height, ivp16, light3, s->sgc.view(), s->view, out_vis, st);
       skip = s->sgc.valid_flag();

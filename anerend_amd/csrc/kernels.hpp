@@ -114,8 +114,9 @@ void launch_shadow_mask_grid(const float* depth, int W, int H, const float* ivp1
                              const SunGridK& sg, const SceneK& sc, float* vis, hipStream_t s);
 void launch_anyhit_grid(const float* orig, const float* dirs, int64_t n, const SunGridK& sg, const SceneK& sc,
                         float tmin, float tmax, ane_hit* hits, hipStream_t s);
-// est: [sun_grid_estimate_words(ntri)] partial sums of the candidate estimate
-int sun_grid_estimate_words(int32_t ntri);
+// est: [sun_grid_scratch_words(ntri)] build scratch (partial sums of the candidate
+// estimate, per-block cell counts)
+int64_t sun_grid_scratch_words(int32_t ntri);
 void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridParams* d_params, uint32_t* acc,
                            uint32_t* cnt, uint32_t* cur, uint32_t* list, uint32_t cap, uint32_t* scan_scratch,
                            float* est, hipStream_t s);

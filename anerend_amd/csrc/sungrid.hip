@@ -185,10 +185,13 @@ __device__ __forceinline__ bool sun_tri_overlaps_cell(const SunGridParams& g, co
 }
 
 // Every (triangle, cell of its rectangle) pair of a wave's 64 triangles, spread
-// over the wave's lanes 64 at a time: f(j, cj, x, y) runs for the pair of lane
-// j's triangle cj and cell (x, y), in every lane that has a pair.  Lanes whose
-// `has` is false have no pairs.  Neighbouring lanes mostly take neighbouring
-// cells of one triangle, so the per-cell atomics of a wave rarely meet.
+// over the wave's lanes 64 at a time: f(ok, j, cj, x, y) runs in every lane of
+// the wave, round after round; ok says whether the lane has a pair in the round,
+// that of lane j's triangle cj and cell (x, y).  The pairs of one triangle take
+// consecutive lanes (j does not decrease over the lanes that have pairs, which
+// come first).  Lanes whose `has` is false have no pairs.  Neighbouring lanes
+// mostly take neighbouring cells of one triangle, so the per-cell atomics of a
+// wave rarely meet.
 template <class F>
 __device__ __forceinline__ void sun_wave_pairs(const CellRange& c, bool has, F&& f)
 {
@@ -225,16 +228,73 @@ __device__ __forceinline__ void sun_wave_pairs(const CellRange& c, bool has, F&&
     const uint32_t q = tc - __shfl(excl, j, 64);
     const uint32_t wj = (uint32_t)(cj.cx1 - cj.cx0 + 1);
     const int y = cj.cy0 + (int)(q / wj), x = cj.cx0 + (int)(q % wj);
-    if (t < total) f(j, cj, x, y);
+    f(t < total, j, cj, x, y);
   }
 }
 
-__global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t ntri, SunGridParams* gp,
-                                                   uint32_t* cnt)
+// Per-block aggregation of the per-cell atomics.  The triangles are in leaf
+// (Morton) order, so a block's 256 triangles cover a compact window of cells,
+// each cell several times.  When the window's bounding rectangle has at most
+// kWin cells, the block counts its pairs in LDS and adds one global atomic per
+// touched cell; otherwise every pair takes its own global atomic.
+#ifndef ANE_SUNGRID_WIN
+#define ANE_SUNGRID_WIN 1024
+#endif
+static constexpr int kWin = ANE_SUNGRID_WIN;
+
+struct CellWindow {
+  int x0, y0, w;  // first cell and width of the rectangle
+  int n;          // cells in it, or 0: the block does not aggregate
+};
+
+// The block's window over the cell ranges of its triangles (has: the thread's
+// triangle takes part).  All threads of the block call it.
+__device__ __forceinline__ CellWindow sun_block_window(const CellRange& c, bool has)
 {
+  __shared__ int red[4][4];
+  int v[4] = {has ? c.cx0 : INT_MAX, has ? c.cy0 : INT_MAX, has ? -c.cx1 : INT_MAX, has ? -c.cy1 : INT_MAX};
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = min(v[k], __shfl_xor(v[k], d, 64));
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][wv] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = min(min(red[k][0], red[k][1]), min(red[k][2], red[k][3]));
+  __syncthreads();  // red is reused by the next call
+  CellWindow w{0, 0, 1, 0};
+  if (v[0] == INT_MAX) return w;  // no triangle in the block
+  const int64_t ww = (int64_t)(-v[2]) - v[0] + 1, wh = (int64_t)(-v[3]) - v[1] + 1;
+  if (ww * wh > kWin) return w;
+  w.x0 = v[0];
+  w.y0 = v[1];
+  w.w = (int)ww;
+  w.n = (int)(ww * wh);
+  return w;
+}
+
+// the validity flag as one value for the whole block (k_sun_count clears it
+// while other blocks read it)
+__device__ __forceinline__ bool sun_block_valid(const SunGridParams* gp)
+{
+  __shared__ uint32_t v;
+  if (threadIdx.x == 0) v = gp->valid;
+  __syncthreads();
+  return v != 0u;
+}
+
+// An aggregating block also keeps its window's counts in tab (kWin per block,
+// at most 256 per cell) for the scatter.
+__global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t ntri, SunGridParams* gp,
+                                                   uint32_t* cnt, uint16_t* tab)
+{
+  __shared__ uint32_t wc[kWin];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!sun_block_valid(gp)) return;
   const SunGridParams g = *gp;
-  if (!g.valid) return;
   CellRange c{};
   bool has = p < ntri;
   if (has) {
@@ -245,9 +305,21 @@ __global__ __launch_bounds__(256) void k_sun_count(const float4* tris, int32_t n
       has = false;
     }
   }
-  sun_wave_pairs(c, has, [&](int, const CellRange& cj, int x, int y) {
-    if (sun_tri_overlaps_cell(g, cj, x, y)) atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
+  const CellWindow win = sun_block_window(c, has);
+  for (int i = threadIdx.x; i < win.n; i += 256) wc[i] = 0u;
+  __syncthreads();
+  sun_wave_pairs(c, has, [&](bool ok, int, const CellRange& cj, int x, int y) {
+    if (ok && sun_tri_overlaps_cell(g, cj, x, y)) {
+      if (win.n) atomicAdd(&wc[(y - win.y0) * win.w + (x - win.x0)], 1u);
+      else atomicAdd(&cnt[(int64_t)y * g.gx + x], 1u);
+    }
   });
+  __syncthreads();
+  uint16_t* bt = tab + (int64_t)blockIdx.x * kWin;
+  for (int i = threadIdx.x; i < win.n; i += 256) {
+    bt[i] = (uint16_t)wc[i];
+    if (wc[i]) atomicAdd(&cnt[(int64_t)(win.y0 + i / win.w) * g.gx + win.x0 + i % win.w], wc[i]);
+  }
 }
 
 // after the scan: start[g.gx * g.gy] entries in total; the cursors begin at the starts
@@ -264,12 +336,19 @@ __global__ __launch_bounds__(256) void k_sun_cursor(const uint32_t* start, int64
 // its origin falls in, one of the triangle's cells; per triangle the mean count
 // of its cells (integers: the sums are exact in any order), then the mean over
 // the triangles, summed per block here and over the blocks by k_sun_decide.
+// An aggregating block (the same window as in k_sun_count, whose per-cell counts
+// it reads back) reserves each cell's run with one global atomic on the cell's
+// cursor, then places its pairs in the runs from LDS cursors; the others take a
+// cursor atomic per pair.  Both reserve from the same cursors, so the runs never
+// overlap.
 __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t ntri, const SunGridParams* gp,
                                                      const uint32_t* start, uint32_t* cur, uint32_t* list,
-                                                     uint32_t cap, float* part)
+                                                     uint32_t cap, const uint16_t* tab, float* part)
 {
   __shared__ float ts[256], tk[256];  // per lane's triangle: the counts of its cells, and their number
   __shared__ float rs[256], rn[256];
+  __shared__ uint32_t wcur[kWin];  // an aggregating block's per-cell counts, then its cursors
+  __shared__ float wtot[kWin];     // and the cells' entry counts (for the estimate)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int t = threadIdx.x, wb = t & ~63;
   const SunGridParams g = *gp;
@@ -282,14 +361,47 @@ __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t
   if (has) c = sun_tri_cells(g, tris, p);
   ts[t] = 0.0f;
   tk[t] = 0.0f;
+  const CellWindow win = sun_block_window(c, has);
+  if (win.n) {
+    const uint16_t* bt = tab + (int64_t)blockIdx.x * kWin;
+    for (int i = t; i < win.n; i += 256) {
+      const uint32_t n = bt[i];
+      if (n) {
+        const int64_t cell = (int64_t)(win.y0 + i / win.w) * g.gx + win.x0 + i % win.w;
+        wcur[i] = atomicAdd(&cur[cell], n);
+        wtot[i] = (float)(start[cell + 1] - start[cell]);
+      }
+    }
+    __syncthreads();
+  }
   const int64_t p0 = p - (t & 63);  // the wave's first triangle
-  sun_wave_pairs(c, has, [&](int j, const CellRange& cj, int x, int y) {
-    if (sun_tri_overlaps_cell(g, cj, x, y)) {
+  const int lane = t & 63;
+  sun_wave_pairs(c, has, [&](bool ok, int j, const CellRange& cj, int x, int y) {
+    float sc = 0.0f, sk = 0.0f;
+    if (ok && sun_tri_overlaps_cell(g, cj, x, y)) {
       const int64_t cell = (int64_t)y * g.gx + x;
-      const uint32_t k = atomicAdd(&cur[cell], 1u);
+      const int l = (y - win.y0) * win.w + (x - win.x0);
+      const uint32_t k = win.n ? atomicAdd(&wcur[l], 1u) : atomicAdd(&cur[cell], 1u);
       if (k < cap) list[k] = (uint32_t)(p0 + j);
-      atomicAdd(&ts[wb + j], (float)(start[cell + 1] - start[cell]));
-      atomicAdd(&tk[wb + j], 1.0f);
+      sc = win.n ? wtot[l] : (float)(start[cell + 1] - start[cell]);
+      sk = 1.0f;
+    }
+    // per triangle, the sums over its lanes (a segmented scan: its lanes are
+    // consecutive), added by its last lane
+    const int key = ok ? j : -1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float uc = __shfl_up(sc, d, 64), uk = __shfl_up(sk, d, 64);
+      const int kk = __shfl_up(key, d, 64);
+      if (lane >= d && kk == key) {
+        sc += uc;
+        sk += uk;
+      }
+    }
+    const int next = __shfl_down(key, 1, 64);
+    if (ok && (lane == 63 || next != key)) {
+      ts[wb + j] += sc;
+      tk[wb + j] += sk;
     }
   });
   const float k = tk[t];
@@ -362,18 +474,24 @@ void launch_sun_grid_build(const SceneK& sc, const SunGridParams& g, SunGridPara
   hipLaunchKernelGGL(k_sun_bounds, dim3(kBoundBlocks), dim3(256), 0, s, sc.tris, sc.ntri, g, acc);
   hipLaunchKernelGGL(k_sun_params, dim3(1), dim3(kBoundBlocks), 0, s, g, (const uint32_t*)acc, d_params);
   const unsigned tb = (unsigned)((sc.ntri + 255) / 256);
+  uint16_t* tab = reinterpret_cast<uint16_t*>(est + 2 * (int64_t)tb);  // after the estimate partials
   if (sc.ntri > 0)
-    hipLaunchKernelGGL(k_sun_count, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri, d_params, cnt);
+    hipLaunchKernelGGL(k_sun_count, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri, d_params, cnt, tab);
   exclusive_scan(cnt, ncell + 1, scan_scratch, s);
   hipLaunchKernelGGL(k_sun_cursor, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s,
                      (const uint32_t*)cnt, ncell, cap, d_params, cur);
   if (sc.ntri > 0) {
     hipLaunchKernelGGL(k_sun_scatter, dim3(tb), dim3(256), 0, s, sc.tris, sc.ntri,
-                       (const SunGridParams*)d_params, (const uint32_t*)cnt, cur, list, cap, est);
+                       (const SunGridParams*)d_params, (const uint32_t*)cnt, cur, list, cap, (const uint16_t*)tab,
+                       est);
     hipLaunchKernelGGL(k_sun_decide, dim3(1), dim3(256), 0, s, (const float*)est, (int)tb, d_params);
   }
 }
 
-int sun_grid_estimate_words(int32_t ntri) { return 2 * (int)((std::max(ntri, 1) + 255) / 256); }
+int64_t sun_grid_scratch_words(int32_t ntri)
+{
+  const int64_t nb = (std::max(ntri, 1) + 255) / 256;
+  return 2 * nb + nb * (kWin / 2);  // estimate partials, then the blocks' window counts (16-bit)
+}
 
 }  // namespace ane

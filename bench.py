@@ -777,7 +777,8 @@ def main():
         if g is not grid:
             g.close()
     ref32.close()
-    # a sun that moves every update: the sun grid is never current, shadow rays traverse
+    # a sun that moves every update over the static scene: each update builds the grid of
+    # its sun (the scene's grid was valid and there are many rays per triangle)
     def moving_sun_frame(i):
         return scenegen.make_frame(500 + i, camera=tuple(frame.camera_pos[:3]), config=cfg, iteration=i)
     ms_ms, st_ms = time_updates(grid, [moving_sun_frame(i) for i in range(args.warmup + args.steps)],

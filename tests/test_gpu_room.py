@@ -78,9 +78,12 @@ def test_room_updates_parity(room):
 @pytest.mark.parametrize("family", ["room", "terrain"])
 def test_sun_grid_shadow_rays_match_oracle(family):
     """Probe shadow rays through the sun grid: the grid of a scene and sun is built
-    on the second update that sees them, so updates 0 and 3 (a new sun) traverse
-    the hierarchy and updates 1, 2 and 4 test the triangles of the sun-grid cells.
-    Hits, shadow decisions, radiance and atlas equal the oracle's at every update."""
+    on the second update that sees them, so update 0 traverses the hierarchy and
+    updates 1 and 2 test the triangles of the sun-grid cells.  Update 3 has a new
+    sun over the same scene: the room (168 probe rays per triangle) builds its grid
+    at once, the terrain (4.7 per triangle, under the 6 that pay for a build)
+    traverses and builds at update 4.  Hits, shadow decisions, radiance and atlas
+    equal the oracle's at every update."""
     if family == "room":
         scene, cfg, _ = scenegen.workload("room8")
         cam = (0.5, 1.5, 0.5)
@@ -92,6 +95,7 @@ def test_sun_grid_shadow_rays_match_oracle(family):
     grid = native.ProbeGrid(cfg)
     atlas = O.new_atlas(cfg)
     lit = shadowed = 0
+    builds = []
     for it in range(5):
         fr = scenegen.make_frame(6, camera=cam, config=cfg, iteration=it)
         if it >= 3:
@@ -106,7 +110,37 @@ def test_sun_grid_shadow_rays_match_oracle(family):
         assert rel_err(grid.atlas, atlas) <= RTOL
         lit += int((sho == 0).sum())
         shadowed += int((sho == 1).sum())
+        builds.append(int(grid.stats()["calls"]["sungrid"]))
     assert lit > 0 and shadowed > 0
+    assert builds == ([0, 1, 1, 2, 2] if family == "room" else [0, 1, 1, 1, 2]), builds
+
+
+def test_sun_grid_moving_sun_builds_every_update():
+    """A sun that moves every update over a static scene: once a grid of the scene
+    has been built and found valid, each new sun builds its grid in its first
+    update (many probe rays per triangle), and every update equals the oracle."""
+    scene, cfg, _ = scenegen.workload("room8")
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    builds = []
+    for it in range(6):
+        fr = scenegen.make_frame(6, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        if it >= 2:
+            fr.light_dir[0] += 0.05 * it   # a new sun every update from update 2 on
+        print(f"moving sun update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), f"hit ids differ at update {it}"
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.radiance(fr), rado) <= RTOL
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        st = grid.stats()
+        builds.append(int(st["calls"]["sungrid"]))
+        if it >= 1:
+            assert st["sun_grid_state"] == 1
+    assert builds == [0, 1, 2, 3, 4, 5], builds
 
 
 @pytest.mark.parametrize("light", [(0.0, -1.0, 0.0), (1.0, 0.0, 0.0), (0.0, -0.6, 0.8)])
