@@ -380,7 +380,10 @@ struct SunGridCache {
       *use = true;
       return ANE_OK;
     }
-    poll();
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ANE_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (!capturing) poll();  // no event queries under capture
     const bool eager = known.set && known.scene == scene && known.version == version && last_valid == 1 &&
                        rays >= (int64_t)ANE_SUNGRID_EAGER_RAYS * ntri;
     if (!ANE_SUNGRID_EAGER && !eager && !(seen == k)) {
@@ -395,9 +398,7 @@ struct SunGridCache {
     if (timer) ANE_TRY(timer->end(st));
     ANE_CHECK_HIP(hipGetLastError());
     built = k;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    ANE_CHECK_HIP(hipStreamIsCapturing(st, &cs));
-    if (cs == hipStreamCaptureStatusNone) {
+    if (!capturing) {
       if (!h_valid) ANE_CHECK_HIP(hipHostMalloc((void**)&h_valid, sizeof(uint32_t), hipHostMallocDefault));
       if (!ev) ANE_CHECK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
       ANE_CHECK_HIP(hipMemcpyAsync(h_valid, &params->valid, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
