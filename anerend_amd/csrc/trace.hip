@@ -57,6 +57,9 @@ namespace ane {
 #ifndef ANE_SHADE_WAVES
 #define ANE_SHADE_WAVES 1  // waves per SIMD the shading kernels are register-limited to (1: compiler's choice)
 #endif
+#ifndef ANE_SHADE_TB
+#define ANE_SHADE_TB 512  // threads per shading block for probes of 512 rays or more (256: 0.357 against 0.348 ms)
+#endif
 #ifndef ANE_SHADE_UNROLL
 #define ANE_SHADE_UNROLL 1  // probe corners of sampleProbe unrolled together (register use vs loads in flight)
 #endif
@@ -1235,8 +1238,8 @@ void launch_shade(const SceneK& sc, const GridK& g, const FrameK& f, const float
   if (num_probes <= 0) return;
   const float4* a4 = reinterpret_cast<const float4*>(atlas);
   if (g.rays >= 512)
-    shade_launch<512>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
-                      num_probes, probe_base, s);
+    shade_launch<ANE_SHADE_TB>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
+                               num_probes, probe_base, s);
   else
     shade_launch<256>(sc, g, f, probe_org, hits, shadow, a4, precision, rad, rad_scale,
                       num_probes, probe_base, s);
