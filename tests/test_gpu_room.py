@@ -143,6 +143,42 @@ def test_sun_grid_moving_sun_builds_every_update():
     assert builds == [0, 1, 2, 3, 4, 5], builds
 
 
+@pytest.mark.parametrize("boxes", [100, 200])
+def test_sun_grid_blocks_over_the_cell_window(boxes):
+    """The sun-grid build counts and places the pairs of a block of 256 triangles in
+    LDS when the block's cells fit a window of 1024, else with a global atomic per
+    pair.  The room's own floor, walls and ceiling project over most of the grid
+    (1225 cells for 100 boxes, 2500 for 200), so the blocks holding them take the
+    per-pair path (blocks of small triangles only, which aggregate, make up the
+    terrain scenes).  Probe updates and the shadow mask through the grid equal the
+    oracle."""
+    from tests.test_oracle_shading import _view_proj
+    scene = scenegen.room_scene(3, num_boxes=boxes)
+    cfg = scenegen.small_room_config()
+    ds, osc = native.DeviceScene(scene), O.OracleScene(scene)
+    grid = native.ProbeGrid(cfg)
+    atlas = O.new_atlas(cfg)
+    _, ivp = _view_proj((0.3, 1.8, 4.2), (0.0, 0.5, 0.0))
+    ivp16 = ivp.T.ravel().astype(np.float32)
+    depth = np.random.default_rng(4).uniform(0.95, 0.9995, (30, 40)).astype(np.float32)
+    light = np.array([0.35, -0.8, 0.45], np.float32)
+    vo, _ = osc.shadow_mask(depth, ivp16, light)
+    for it in range(3):
+        fr = scenegen.make_frame(8, camera=(0.5, 1.5, 0.5), config=cfg, iteration=it)
+        fr.light_dir[:3] = light
+        print(f"window room {boxes} update {it}", flush=True)
+        ho, sho, rado, _ = osc.update(cfg, fr, atlas)
+        grid.update(ds, fr)
+        hg, shg = grid.debug_rays(fr)
+        assert np.array_equal(hg["tri"], ho["tri"]), f"hits differ at update {it}"
+        assert np.array_equal(shg, sho), f"shadow decisions differ at update {it}"
+        assert rel_err(grid.atlas, atlas) <= RTOL
+        assert np.array_equal(ds.shadow_mask(depth, ivp16, light), vo), f"shadow mask differs at call {it}"
+    st = grid.stats()
+    print(f"sun grid estimate {st['sun_grid_candidates']:.2f}", flush=True)
+    assert st["sun_grid_state"] == 1 and st["calls"]["sungrid"] == 1, st
+
+
 @pytest.mark.parametrize("light", [(0.0, -1.0, 0.0), (1.0, 0.0, 0.0), (0.0, -0.6, 0.8)])
 def test_sun_grid_axis_aligned_suns(light):
     """Suns along an axis or in a coordinate plane.  Their rays can lie exactly
