@@ -444,6 +444,7 @@ struct ane_scene {
   uint32_t* d_cbounds = nullptr;
   uint32_t* d_cpart = nullptr;  // per-block centroid-bound partials of a build
   uint4* d_qnodes = nullptr;    // quantized traversal nodes (allocated when first used)
+  bool boxes_stale = false;     // the last fit left the 64-byte records' boxes out (quantized traversal)
   int32_t node_format = ANE_NODES_AUTO;
   uint32_t* d_scratch = nullptr;
   uint32_t* d_info = nullptr;   // fit plan read back at a build: level offsets, counts, tree depth
@@ -594,9 +595,23 @@ static int scene_fit_qnodes(ane_scene* s, uint4** out)
   return ANE_OK;
 }
 
+// The 64-byte records' boxes, when the last fit left them out: a fit from the
+// current leaf-order triangles that writes them (the traversal of quantized
+// records does not read them; a change of node format and the BVH read-back do).
+static int scene_full_boxes(ane_scene* s, hipStream_t st)
+{
+  if (!s->boxes_stale) return ANE_OK;
+  launch_fit(s->d_tris, s->ntri, s->d_nodes, s->d_node_box, nullptr, s->d_topo, s->d_leaf_parent, s->d_units,
+             s->nunits, s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, nullptr, st, true);
+  ANE_CHECK_HIP(hipGetLastError());
+  s->boxes_stale = false;
+  return ANE_OK;
+}
+
 // fitted: the fit that just ran wrote the quantized records (scene_fit_qnodes)
 static int scene_apply_node_format(ane_scene* s, hipStream_t st, bool fitted = false)
 {
+  if (!fitted) ANE_TRY(scene_full_boxes(s, st));
   if (!scene_wants_qnodes(s)) {
     s->view.qnodes = nullptr;
     return ANE_OK;
@@ -626,7 +641,8 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
   uint4* qn = nullptr;
   ANE_TRY(scene_fit_qnodes(s, &qn));
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, s->d_uheight, st);
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, s->d_uheight, st, qn == nullptr);
+  s->boxes_stale = qn != nullptr;
   ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));
   uint32_t* d_depth = s->d_info + 132;
   if (launch_tree_depth(s->d_topo, n, s->d_unit_root, s->d_uheight, s->nunits, s->d_top, s->d_uscratch,
@@ -655,7 +671,8 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   uint4* qn = nullptr;
   ANE_TRY(scene_fit_qnodes(s, &qn));
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, nullptr, st);
+             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, nullptr, st, qn == nullptr);
+  s->boxes_stale = qn != nullptr;
   s->world_stale = true;
   ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));
   ANE_CHECK_HIP(hipGetLastError());
@@ -876,6 +893,9 @@ extern "C" int ane_scene_debug_bvh(const ane_scene* s, uint64_t* keys, uint32_t*
 {
   if (!s) return fail(ANE_ERR_INVALID_ARG, "scene is NULL");
   const int64_t n = s->ntri;
+  // the boxes come from the 64-byte records, refitted first if the last fit left
+  // them out (derived data: the scene is unchanged otherwise)
+  if (boxes) ANE_TRY(scene_full_boxes(const_cast<ane_scene*>(s), nullptr));
   ANE_CHECK_HIP(hipDeviceSynchronize());
   if (keys) ANE_CHECK_HIP(hipMemcpy(keys, s->d_keys, 8 * n, hipMemcpyDeviceToHost));
   if (ids) ANE_CHECK_HIP(hipMemcpy(ids, s->d_ids, 4 * n, hipMemcpyDeviceToHost));

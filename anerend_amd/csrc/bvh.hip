@@ -684,7 +684,7 @@ template <bool FROM_REC>
 __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* ids, float4* tris, NodeBuild* nodes,
                                                     float* node_box, uint4* qn, const int4* topo,
                                                     const int32_t* leaf_parent, const int2* units, int n,
-                                                    uint8_t* uheight)
+                                                    uint8_t* uheight, int full)
 {
   extern __shared__ int4 span_lds[];
   int4* nd = span_lds;                                       // [kSpan] topology of nodes s..
@@ -741,10 +741,12 @@ __global__ __launch_bounds__(kSpan) void k_fit_span(SceneK sc, const uint32_t* i
       float l0[3], h0[3], l1[3], h1[3];
       span_child_box(lb, ib, d.x, s, l0, h0);
       span_child_box(lb, ib, d.y, s, l1, h1);
-      NodeBuild* q = &nodes[p];
-      q->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
-      q->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
-      q->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+      if (full) {  // the 64-byte record's boxes (a scene that traverses quantized records skips them)
+        NodeBuild* q = &nodes[p];
+        q->a = make_float4(l0[0], h0[0], l0[1], h0[1]);
+        q->b = make_float4(l1[0], h1[0], l1[1], h1[1]);
+        q->c = make_float4(l0[2], h0[2], l1[2], h1[2]);
+      }
       float u[6];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -1052,7 +1054,7 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
-                const uint32_t* ids, uint8_t* uheight, hipStream_t s)
+                const uint32_t* ids, uint8_t* uheight, hipStream_t s, bool full_boxes)
 {
   if (n < 2 || cross_begin.size() < 2) {
     if (refit && n == 1) launch_refit_tris(*refit, ids, n, tris, s);
@@ -1073,11 +1075,11 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* q
   if (nunits > 0) {
     if (refit)
       hipLaunchKernelGGL(k_fit_span<true>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, *refit, ids, tris,
-                         nodes, node_box, qnodes, topo, leaf_parent, units, n, uheight);
+                         nodes, node_box, qnodes, topo, leaf_parent, units, n, uheight, full_boxes ? 1 : 0);
     else
       hipLaunchKernelGGL(k_fit_span<false>, dim3((unsigned)nunits), dim3(kSpan), kSpanLds, s, none,
                          (const uint32_t*)nullptr, tris, nodes, node_box, qnodes, topo, leaf_parent, units, n,
-                         uheight);
+                         uheight, full_boxes ? 1 : 0);
   }
   // the crossing nodes, deepest level first: a launch per level below the top
   // ones, then the top levels in one workgroup

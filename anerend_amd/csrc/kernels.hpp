@@ -54,10 +54,13 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
 // uheight (nullable): the height of each treelet's root, by treelet.
 // qnodes (nullable): the quantized records are written too, from the same child
 // boxes (the bytes launch_quantize_nodes makes from the fitted nodes).
+// full_boxes false: the treelet nodes' boxes in the 64-byte records are left
+// as they were (a scene that traverses the quantized records; the top nodes'
+// are written either way).
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
                 const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
-                const uint32_t* ids, uint8_t* uheight, hipStream_t s);
+                const uint32_t* ids, uint8_t* uheight, hipStream_t s, bool full_boxes = true);
 // trace.hip
 // Sun grid of the probe shadow rays (sungrid.hip): projected along the sun
 // direction, cell (cx, cy) of a point = (sun_cell(X, x0, icw, gx), sun_cell(Y, y0, ich, gy))
