@@ -450,6 +450,8 @@ struct ane_scene {
   uint32_t* d_info = nullptr;   // fit plan read back at a build: level offsets, counts, tree depth
   int4* d_topo = nullptr;       // children, parent and leaf-range end of every internal node
   int32_t* d_cross = nullptr;   // top nodes (more leaves than a fit treelet holds), by depth
+  int2* d_cross_kids = nullptr; // their children, in the same order (cross_kids_cap entries)
+  int64_t cross_kids_cap = 0;
   int32_t* d_top = nullptr;     // top nodes, by index
   uint8_t* d_ndepth = nullptr;  // depth of the top nodes
   int2* d_units = nullptr;      // leaf ranges of the fit treelets
@@ -513,7 +515,7 @@ static void scene_free(ane_scene* s)
   dfree(s->d_world); dfree(s->d_keys); dfree(s->d_keys2); dfree(s->d_ids); dfree(s->d_ids2);
   dfree(s->d_nodes); dfree(s->d_leaf_parent); dfree(s->d_tris);
   dfree(s->d_node_box); dfree(s->d_cbounds); dfree(s->d_cpart); dfree(s->d_qnodes); dfree(s->d_scratch);
-  dfree(s->d_info); dfree(s->d_topo); dfree(s->d_cross); dfree(s->d_top); dfree(s->d_ndepth); dfree(s->d_units);
+  dfree(s->d_info); dfree(s->d_topo); dfree(s->d_cross); dfree(s->d_cross_kids); dfree(s->d_top); dfree(s->d_ndepth); dfree(s->d_units);
   dfree(s->d_unit_root); dfree(s->d_uheight); dfree(s->d_stray); dfree(s->d_uscratch);
   dfree(s->d_sh_orig); dfree(s->d_sh_dir); dfree(s->d_sh_hits); dfree(s->d_sh_work);
   dfree(s->d_sh_stats); dfree(s->d_sh_ov); dfree(s->d_sh_spill);
@@ -602,7 +604,7 @@ static int scene_full_boxes(ane_scene* s, hipStream_t st)
 {
   if (!s->boxes_stale) return ANE_OK;
   launch_fit(s->d_tris, s->ntri, s->d_nodes, s->d_node_box, nullptr, s->d_topo, s->d_leaf_parent, s->d_units,
-             s->nunits, s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, nullptr, st, true);
+             s->nunits, s->d_stray, s->nstray, s->d_cross, s->d_cross_kids, s->cross_begin, nullptr, nullptr, nullptr, st, true);
   ANE_CHECK_HIP(hipGetLastError());
   s->boxes_stale = false;
   return ANE_OK;
@@ -638,10 +640,19 @@ static int scene_build(ane_scene* s, hipStream_t st, bool write_inst)
                       s->d_cross, s->d_top, s->d_units, s->d_unit_root, s->d_stray, &s->cross_begin, &s->nunits,
                       &s->nstray, st) != 0)
     return fail(ANE_ERR_DEVICE, "LBVH fit plan failed");
+  const int64_t ncross = s->cross_begin.empty() ? 0 : s->cross_begin.back();
+  if (ncross > s->cross_kids_cap) {  // the plan synchronized the stream
+    dfree(s->d_cross_kids);
+    s->bytes -= s->cross_kids_cap * (int64_t)sizeof(int2);
+    s->cross_kids_cap = 0;
+    ANE_TRY(dalloc(&s->d_cross_kids, ncross, &s->bytes));
+    s->cross_kids_cap = ncross;
+  }
+  launch_cross_kids(s->d_cross, s->d_topo, (int)ncross, s->d_cross_kids, st);
   uint4* qn = nullptr;
   ANE_TRY(scene_fit_qnodes(s, &qn));
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, nullptr, nullptr, s->d_uheight, st, qn == nullptr);
+             s->d_stray, s->nstray, s->d_cross, s->d_cross_kids, s->cross_begin, nullptr, nullptr, s->d_uheight, st, qn == nullptr);
   s->boxes_stale = qn != nullptr;
   ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));
   uint32_t* d_depth = s->d_info + 132;
@@ -671,7 +682,7 @@ static int scene_refit_dev(ane_scene* s, hipStream_t st)
   uint4* qn = nullptr;
   ANE_TRY(scene_fit_qnodes(s, &qn));
   launch_fit(s->d_tris, n, s->d_nodes, s->d_node_box, qn, s->d_topo, s->d_leaf_parent, s->d_units, s->nunits,
-             s->d_stray, s->nstray, s->d_cross, s->cross_begin, &view, s->d_ids, nullptr, st, qn == nullptr);
+             s->d_stray, s->nstray, s->d_cross, s->d_cross_kids, s->cross_begin, &view, s->d_ids, nullptr, st, qn == nullptr);
   s->boxes_stale = qn != nullptr;
   s->world_stale = true;
   ANE_TRY(scene_apply_node_format(s, st, qn != nullptr));

@@ -627,10 +627,12 @@ void launch_quantize_nodes(const NodeBuild* nodes, int n, uint4* q, hipStream_t 
 // Fit one internal node from its (already final) children: child boxes into
 // the traversal record (and its quantized record when qn is set), the union
 // into node_box.
-__device__ __forceinline__ void fit_node(const float4* tris, NodeBuild* nodes, float* node_box, uint4* qn, int i)
+// A crossing node i with children (L, R): its 64-byte record's boxes, its
+// quantized record (qn non-null) and its own box.
+__device__ __forceinline__ void fit_cross(const float4* tris, NodeBuild* nodes, float* node_box, uint4* qn, int i,
+                                          int L, int R)
 {
   NodeBuild* nd = &nodes[i];
-  const int L = nd->d.x, R = nd->d.y;
   float l0[3], h0[3], l1[3], h1[3];
   child_box(tris, node_box, L, l0, h0);
   child_box(tris, node_box, R, l1, h1);
@@ -920,14 +922,26 @@ __global__ __launch_bounds__(256) void k_refit_stray(SceneK sc, const uint32_t* 
 }
 
 // One depth level of the crossing nodes: every child is a leaf, inside a span
-// (fitted by k_fit_span) or a crossing node of a deeper, finished level.
+// (fitted by k_fit_span) or a crossing node of a deeper, finished level.  kids:
+// the children of order[k] (gathered once per build), so that a level's loads
+// wait on one round trip (the child boxes), not three.
 __global__ __launch_bounds__(256) void k_fit_level(const float4* tris, NodeBuild* nodes,
-                                                   float* node_box, uint4* qn, const int32_t* order, int begin,
-                                                   int count)
+                                                   float* node_box, uint4* qn, const int32_t* order, const int2* kids,
+                                                   int begin, int count)
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
-  fit_node(tris, nodes, node_box, qn, order[begin + k]);
+  const int2 c = kids[begin + k];
+  fit_cross(tris, nodes, node_box, qn, order[begin + k], c.x, c.y);
+}
+
+// the children of the crossing nodes, in their order
+__global__ __launch_bounds__(256) void k_cross_kids(const int32_t* order, const int4* topo, int total, int2* kids)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const int4 t = topo[order[k]];
+  kids[k] = make_int2(t.x, t.y);
 }
 
 // The top levels (at most 2^d nodes at depth d) in one workgroup, deepest first.
@@ -938,11 +952,14 @@ struct TopLevels {
 };
 
 __global__ __launch_bounds__(1024) void k_fit_top(const float4* tris, NodeBuild* nodes,
-                                                  float* node_box, uint4* qn, const int32_t* order, TopLevels t)
+                                                  float* node_box, uint4* qn, const int32_t* order, const int2* kids,
+                                                  TopLevels t)
 {
   for (int l = t.num - 1; l >= 0; --l) {
-    for (int k = threadIdx.x; k < t.count[l]; k += blockDim.x)
-      fit_node(tris, nodes, node_box, qn, order[t.begin[l] + k]);
+    for (int k = threadIdx.x; k < t.count[l]; k += blockDim.x) {
+      const int2 c = kids[t.begin[l] + k];
+      fit_cross(tris, nodes, node_box, qn, order[t.begin[l] + k], c.x, c.y);
+    }
     __threadfence_block();
     __syncthreads();
   }
@@ -1053,7 +1070,7 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
 
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
-                const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
+                const int32_t* cross, const int2* cross_kids, const std::vector<int32_t>& cross_begin, const SceneK* refit,
                 const uint32_t* ids, uint8_t* uheight, hipStream_t s, bool full_boxes)
 {
   if (n < 2 || cross_begin.size() < 2) {
@@ -1088,7 +1105,7 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* q
     const int cnt = cross_begin[d + 1] - cross_begin[d];
     if (cnt > 0)
       hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, s, (const float4*)tris, nodes,
-                         node_box, qnodes, cross, cross_begin[d], cnt);
+                         node_box, qnodes, cross, cross_kids, cross_begin[d], cnt);
   }
   TopLevels t{};
   t.num = levels < kTopLevels ? levels : kTopLevels;
@@ -1099,7 +1116,14 @@ void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* q
     top += t.count[d];
   }
   if (top > 0)
-    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (const float4*)tris, nodes, node_box, qnodes, cross, t);
+    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (const float4*)tris, nodes, node_box, qnodes, cross,
+                       cross_kids, t);
+}
+
+void launch_cross_kids(const int32_t* cross, const int4* topo, int total, int2* kids, hipStream_t s)
+{
+  if (total > 0)
+    hipLaunchKernelGGL(k_cross_kids, dim3(blocks_for(total, 256)), dim3(256), 0, s, cross, topo, total, kids);
 }
 
 

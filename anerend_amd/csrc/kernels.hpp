@@ -59,8 +59,11 @@ int launch_tree_depth(const int4* topo, int n, const int32_t* unit_root, const u
 // are written either way).
 void launch_fit(float4* tris, int n, NodeBuild* nodes, float* node_box, uint4* qnodes, const int4* topo,
                 const int32_t* leaf_parent, const int2* units, int nunits, const int32_t* stray, int nstray,
-                const int32_t* cross, const std::vector<int32_t>& cross_begin, const SceneK* refit,
+                const int32_t* cross, const int2* cross_kids, const std::vector<int32_t>& cross_begin, const SceneK* refit,
                 const uint32_t* ids, uint8_t* uheight, hipStream_t s, bool full_boxes = true);
+// cross_kids[k]: the children of crossing node cross[k], for the total crossing
+// nodes of the fit plan (once per build)
+void launch_cross_kids(const int32_t* cross, const int4* topo, int total, int2* kids, hipStream_t s);
 // trace.hip
 // Sun grid of the probe shadow rays (sungrid.hip): projected along the sun
 // direction, cell (cx, cy) of a point = (sun_cell(X, x0, icw, gx), sun_cell(Y, y0, ich, gy))
