@@ -238,7 +238,7 @@ __device__ __forceinline__ void sun_wave_pairs(const CellRange& c, bool has, F&&
 // kWin cells, the block counts its pairs in LDS and adds one global atomic per
 // touched cell; otherwise every pair takes its own global atomic.
 #ifndef ANE_SUNGRID_WIN
-#define ANE_SUNGRID_WIN 1024
+#define ANE_SUNGRID_WIN 1024  // 512 / 2048 swept: build 0.330 / 0.244 ms against 0.245 on 1M triangles
 #endif
 static constexpr int kWin = ANE_SUNGRID_WIN;
 
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_sun_scatter(const float4* tris, int32_t
 {
   __shared__ float ts[256], tk[256];  // per lane's triangle: the counts of its cells, and their number
   __shared__ float rs[256], rn[256];
-  __shared__ uint32_t wcur[kWin];  // an aggregating block's per-cell counts, then its cursors
+  __shared__ uint32_t wcur[kWin];  // an aggregating block's cursor in each window cell's run
   __shared__ float wtot[kWin];     // and the cells' entry counts (for the estimate)
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int t = threadIdx.x, wb = t & ~63;
