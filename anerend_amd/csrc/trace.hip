@@ -40,7 +40,10 @@ namespace ane {
 #define ANE_TRACE_STACK 16
 #endif
 #ifndef ANE_TRACE_CHUNK
-#define ANE_TRACE_CHUNK 128
+#define ANE_TRACE_CHUNK 256  // probe rays claimed by a wave at a time (64-byte nodes)
+#endif
+#ifndef ANE_TRACE_CHUNK_RAYS
+#define ANE_TRACE_CHUNK_RAYS 128  // caller rays claimed by a wave at a time (64-byte nodes)
 #endif
 #ifndef ANE_TRACE_REFILL
 #define ANE_TRACE_REFILL 32
@@ -71,7 +74,13 @@ namespace ane {
 #endif
 static constexpr int kTB = ANE_TRACE_BLOCK;      // threads per traversal block
 static constexpr int kSS = ANE_TRACE_STACK;      // LDS stack entries per lane (ring, any size)
-static constexpr int kChunk = ANE_TRACE_CHUNK;   // rays claimed by a wave at a time
+// Every claim is one device-scope atomic on the launch's counter, which all 8 XCDs
+// share: with 128-ray chunks the 8.4M probe rays of an update made 65k claims and the
+// trace waited on them (terrain1m trace 1.028 ms; 256: 0.895, 384: 0.906, 512: 0.979,
+// 64: 1.69).  Caller rays (1-2M per pass) keep 128: larger chunks unbalance their
+// launch's end (mirror pass 0.78 -> 0.87 ms at 256).
+static constexpr int kChunk = ANE_TRACE_CHUNK;
+static constexpr int kChunkRays = ANE_TRACE_CHUNK_RAYS;
 static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a refill
 // The quantized-node traversal serves scenes whose nodes miss the caches: its
 // lanes wait on memory rather than on VALU issue, so it refills earlier and claims
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(kTB) __attribute__((amdgpu_waves_per_eu(ANE_TRACE_W
     const uint64_t im = __ballot(idle);
     const uint64_t am = __ballot(ray >= 0);
     const uint32_t need = (uint32_t)__popcll(im);
-    constexpr uint32_t chunk = (uint32_t)(Q ? kChunkQ : kChunk);
+    constexpr uint32_t chunk = (uint32_t)(Q ? kChunkQ : (RAYS ? kChunkRays : kChunk));
     if (need != 0u && (am == 0ull || need >= (uint32_t)(Q ? kRefillQ : kRefill))) {
       if (cbeg >= cend) {  // claim the next chunk (one atomic per wave and chunk)
         uint32_t base = 0;
