@@ -85,7 +85,8 @@ static constexpr int kRefill = ANE_TRACE_REFILL; // idle lanes that trigger a re
 // The quantized-node traversal serves scenes whose nodes miss the caches: its
 // lanes wait on memory rather than on VALU issue, so it refills earlier and claims
 // larger chunks (4.04M-triangle scene: trace 10.0 -> 9.0 ms against 32 / 128), and
-// checks for refills every 7 steps (9.09 -> 8.98 ms against 5)
+// checks for refills every 7 steps (9.09 -> 8.98 ms against 5); chunks of 384 / 512
+// measured 8.98 / 9.60 ms against 8.84
 #ifndef ANE_TRACE_CHUNK_Q
 #define ANE_TRACE_CHUNK_Q 256
 #endif
